@@ -1,0 +1,183 @@
+"""Seeded synthetic multiview inputs: blob phantom, rotated Gaussian PSFs,
+blurred + Poisson-noised views and blending weights.
+
+The generator is shared by the tests, the benchmark and the GPU smoke run so
+that every path consumes identical FP32 arrays.  Arrays are indexed [z, y, x]
+(x fastest, the flat order of the reference's ImgLib2 images).
+
+Sources of the formulas:
+* blending weights: BlendingRealRandomAccess.computeWeight and its 1001-entry
+  cosine look-up table (process/fusion/transformed/weights/BlendingRealRandomAccess.java:46-130),
+  applied along z only (border 0, blending range in voxels);
+* weight normalisation: NormalizingRandomAccess.get / hardWeights with
+  osemspeedup 1 (process/deconvolution/normalization/NormalizingRandomAccess.java:75-214);
+* observed-image clamp to minValueImg = 1 for covered voxels
+  (process/fusion/transformed/AbstractTransformedImgRandomAccess.java:83,
+  MultiViewDeconvolution.java:48-50).
+
+Random numbers come from numpy's PCG64 generator with fixed seeds
+(1000+c phantom, 2000+100c+v Poisson noise of view v).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Workload:
+    name: str
+    shape_zyx: tuple
+    n_views: int
+    psf_size: int
+    sigma_xy: float
+    sigma_z: float
+    angles_deg: list
+    n_blobs: int
+    r_min: float
+    r_max: float
+    peak: float | None
+    weights: str          # "uniform" or "zramp"
+    blend_range: int
+    psf_type: str
+    lam: float
+    iterations: int
+    seed_index: int
+    extra: dict = field(default_factory=dict)
+
+
+# The two single-GPU workloads named by the benchmark specification.
+C1 = Workload("c1_128cube_3views_eb", (128, 128, 128), 3, 31, 1.5, 4.5, [0, 120, 240], 200, 2, 6, None,
+              "uniform", 0, "EFFICIENT_BAYESIAN", 0.0, 10, 1)
+C2 = Workload("c2_256cube_6views_opt2", (256, 256, 256), 6, 41, 1.8, 5.0, [0, 60, 120, 180, 240, 300], 1500, 2, 6,
+              1000.0, "zramp", 20, "OPTIMIZATION_II", 0.0006, 20, 2)
+
+
+def small_workload(n=24, views=3, k=7, psf_type="EFFICIENT_BAYESIAN", lam=0.0, iterations=3, weights="uniform",
+                   seed_index=7, n_blobs=12, peak=None, shape=None):
+    shape = shape or (n, n, n)
+    ang = [i * 360.0 / views for i in range(views)]
+    return Workload(f"small_{shape}_{views}v_k{k}", shape, views, k, 1.0, 2.0, ang, n_blobs, 1.0, 2.5, peak,
+                    weights, max(2, min(shape) // 6), psf_type, lam, iterations, seed_index)
+
+
+def blob_phantom(shape_zyx, n_blobs, r_min, r_max, seed, background=10.0):
+    """Background + Gaussian blobs (sigma = radius), truncated at 4 sigma, summed in FP64."""
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = shape_zyx
+    vol = np.full(shape_zyx, background, np.float64)
+    cz = rng.uniform(0, nz, n_blobs)
+    cy = rng.uniform(0, ny, n_blobs)
+    cx = rng.uniform(0, nx, n_blobs)
+    r = rng.uniform(r_min, r_max, n_blobs)
+    amp = rng.uniform(50.0, 500.0, n_blobs)
+    for i in range(n_blobs):
+        ext = 4.0 * r[i]
+        lo = [max(0, int(math.floor(c - ext))) for c in (cz[i], cy[i], cx[i])]
+        hi = [min(n, int(math.ceil(c + ext)) + 1) for c, n in zip((cz[i], cy[i], cx[i]), shape_zyx)]
+        if any(h <= l for l, h in zip(lo, hi)):
+            continue
+        z = np.arange(lo[0], hi[0]) - cz[i]
+        y = np.arange(lo[1], hi[1]) - cy[i]
+        x = np.arange(lo[2], hi[2]) - cx[i]
+        d2 = z[:, None, None] ** 2 + y[None, :, None] ** 2 + x[None, None, :] ** 2
+        g = amp[i] * np.exp(-d2 / (2.0 * r[i] ** 2))
+        g[d2 > ext * ext] = 0.0
+        vol[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] += g
+    return vol
+
+
+def gaussian_psf(size, sigma_xy, sigma_z, angle_deg):
+    """Anisotropic Gaussian on a size^3 grid rotated about y; FP64, normalised to sum 1, cast to FP32."""
+    c = size // 2
+    t = math.radians(angle_deg)
+    ry = np.array([[math.cos(t), 0, math.sin(t)], [0, 1, 0], [-math.sin(t), 0, math.cos(t)]])
+    cov = ry @ np.diag([sigma_xy ** 2, sigma_xy ** 2, sigma_z ** 2]) @ ry.T   # (x, y, z) order
+    icov = np.linalg.inv(cov)
+    o = np.arange(size) - c
+    z, y, x = np.meshgrid(o, o, o, indexing="ij")
+    p = np.stack([x, y, z], -1).astype(np.float64)
+    q = np.einsum("...i,ij,...j->...", p, icov, p)
+    k = np.exp(-0.5 * q)
+    k /= k.sum()
+    return k.astype(np.float32)
+
+
+def convolve_mirror_f64(vol, psf):
+    """FP64 linear convolution of the mirror-single extended volume with psf (odd size)."""
+    from scipy.signal import fftconvolve
+    c = [s // 2 for s in psf.shape]
+    pad = np.pad(vol, [(ci, s - 1 - ci) for ci, s in zip(c, psf.shape)], mode="reflect")
+    return fftconvolve(pad, psf.astype(np.float64), mode="valid")
+
+
+_LUT = None
+
+
+def _blend_lut():
+    """BlendingRealRandomAccess static look-up table (:51-57), accumulated exactly as the Java loop does."""
+    global _LUT
+    if _LUT is None:
+        lut = np.zeros(1001, np.float64)
+        d = 0.0
+        while d <= 1.0001:
+            lut[int(d * 1000.0 + 0.5)] = (math.cos((1 - d) * math.pi) + 1) / 2
+            d = d + 0.001
+        _LUT = lut
+    return _LUT
+
+
+def blend_weight_1d(n, blending, border=0.0):
+    """computeWeight restricted to one axis (:95-127), float arithmetic as in Java."""
+    lut = _blend_lut()
+    out = np.zeros(n, np.float32)
+    dim_minus1 = np.float32(n - 1)
+    for i in range(n):
+        l = np.float32(i)
+        tmp = min(np.float32(l - np.float32(border)), np.float32(dim_minus1 - l - np.float32(border)))
+        if tmp <= 0:
+            out[i] = 0.0
+            continue
+        rel = np.float32(tmp / np.float32(blending))
+        w = np.float32(1.0)
+        if rel < 1:
+            w = np.float32(w * lut[int(float(rel) * 1000.0 + 0.5)])
+        out[i] = w
+    return out
+
+
+def normalize_weights(ws):
+    """NormalizingRandomAccess.get with hard weights and osemspeedup 1 (:75-110, :205-214)."""
+    ws = np.asarray(ws, np.float32)
+    clipped = np.minimum(1.0, ws.astype(np.float64))
+    sumw = clipped.sum(0)
+    out = np.empty_like(ws)
+    for v in range(ws.shape[0]):
+        mine = clipped[v].astype(np.float32)
+        val = np.where(sumw > 1, (mine.astype(np.float64) / sumw).astype(np.float32), mine).astype(np.float64)
+        out[v] = np.minimum(1.0, val * 1.0).astype(np.float32)
+    return out
+
+
+def make_inputs(w: Workload):
+    """Returns dict(phantom f64, imgs [V,z,y,x] f32, weights f32, psfs [V,k,k,k] f32)."""
+    V = w.n_views
+    ph = blob_phantom(w.shape_zyx, w.n_blobs, w.r_min, w.r_max, 1000 + w.seed_index)
+    psfs = np.stack([gaussian_psf(w.psf_size, w.sigma_xy, w.sigma_z, a) for a in w.angles_deg[:V]])
+    imgs = np.empty((V,) + tuple(w.shape_zyx), np.float32)
+    for v in range(V):
+        b = convolve_mirror_f64(ph, psfs[v])
+        if w.peak is not None:
+            b = b * (w.peak / b.max())
+        rng = np.random.default_rng(2000 + 100 * w.seed_index + v)
+        noisy = rng.poisson(np.maximum(b, 0.0)).astype(np.float64)
+        imgs[v] = np.maximum(1.0, noisy).astype(np.float32)
+    if w.weights == "uniform":
+        weights = np.ones_like(imgs)
+    else:
+        wz = blend_weight_1d(w.shape_zyx[0], w.blend_range)
+        raw = np.broadcast_to(wz[None, :, None, None], imgs.shape).astype(np.float32)
+        weights = normalize_weights(raw)
+    return dict(phantom=ph, imgs=imgs, weights=np.ascontiguousarray(weights), psfs=psfs)

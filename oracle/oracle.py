@@ -1,0 +1,278 @@
+"""ctypes front-end of the CPU oracle (``mvr_oracle.cpp``).
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the
+checker.  The HIP product never calls it.
+
+The oracle restates the reference's CPU path (Java, not executable here:
+no JDK/Maven; see the header of ``mvr_oracle.cpp`` for what is restated from
+third-party definitions).  How each function is pinned (``tests/test_oracle_*.py``):
+
+=============================  ==============================================================
+oracle function                pinned by
+=============================  ==============================================================
+divide_into_portions           known-answer table from the Java formula (FusionTools:1287-1329)
+sum_img_quirk / norm_to_sum1   closed form S/(S+S0) with numpy sums; invariance checks
+mirror                         numpy flip (odd sizes); middle-pair law (even sizes)
+fft1d / rfft3d                 numpy.fft (float64)
+fft_convolve (mirror/zero/1)   brute-force direct convolution (tiny) and scipy.ndimage.convolve
+init_psfs (4 PSF types)        delta-PSF known answers, scipy.signal.convolve(mode='same'),
+                               symmetry invariants
+next_value / next_value_mul    closed forms + exhaustive special cases (<=0, NaN, lambda=0, w=0/1)
+divide_into_blocks/sort_blocks known-answer tables incl. the reference Block.main inputs
+psi_init_avg_precise           closed form with numpy
+deconvolve (Seq/Mul)           block-independence invariant, numpy/scipy global-form iteration,
+                               FP32-vs-FP64 agreement, flux preservation
+=============================  ==============================================================
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libmvr_oracle.so")
+
+PSF_TYPES = {"OPTIMIZATION_II": 0, "OPTIMIZATION_I": 1, "EFFICIENT_BAYESIAN": 2, "INDEPENDENT": 3}
+SCHEMES = {"SEQ": 0, "MUL": 1}
+
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (g++ -ffp-contract=off -fopenmp)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE] + (["-B"] if force else []), check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_set_threads.restype = ctypes.c_int
+        L.oracle_fast_length.argtypes = [ctypes.c_int]
+        L.oracle_divide_into_portions.argtypes = [ctypes.c_longlong, ctypes.c_int, _i64p, _i64p, ctypes.c_int]
+        L.oracle_sum_img_quirk_f32.argtypes = [_f32p, ctypes.c_longlong, ctypes.c_int]
+        L.oracle_sum_img_quirk_f32.restype = ctypes.c_double
+        L.oracle_norm_to_sum1_f32.argtypes = [_f32p, ctypes.c_longlong, ctypes.c_int]
+        L.oracle_norm_to_sum1_f64.argtypes = [_f64p, ctypes.c_longlong, ctypes.c_int]
+        L.oracle_mirror_f32.argtypes = [_f32p, _i32p]
+        L.oracle_fft1d_f64.argtypes = [_f64p, ctypes.c_int, ctypes.c_int]
+        L.oracle_rfft3d_f64.argtypes = [_f64p, _i32p, _f64p]
+        L.oracle_fft_convolve_f32.argtypes = [_f32p, _i32p, ctypes.c_int, ctypes.c_float, _f32p, _i32p, _f32p]
+        L.oracle_fft_convolve_f64.argtypes = [_f64p, _i32p, ctypes.c_int, ctypes.c_double, _f64p, _i32p, _f64p]
+        L.oracle_next_value_f32.argtypes = [ctypes.c_float] * 3 + [ctypes.c_double] + [ctypes.c_float] * 2
+        L.oracle_next_value_f32.restype = ctypes.c_float
+        L.oracle_next_value_f64.argtypes = [ctypes.c_double] * 6
+        L.oracle_next_value_f64.restype = ctypes.c_double
+        L.oracle_next_value_mul_f32.argtypes = [ctypes.c_float, _f32p, _f32p, ctypes.c_int, ctypes.c_double,
+                                                ctypes.c_float, ctypes.c_float]
+        L.oracle_next_value_mul_f32.restype = ctypes.c_float
+        L.oracle_divide_into_blocks.argtypes = [_i64p, _i64p, _i64p, _i64p, ctypes.c_int]
+        L.oracle_sort_blocks.argtypes = [_i64p, _i64p, _i64p, ctypes.c_int, _i32p, _i32p, _i32p]
+        L.oracle_init_psfs_f32.argtypes = [_f32p, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, _f32p, _f32p]
+        L.oracle_init_psfs_f64.argtypes = [_f64p, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, _f64p, _f64p]
+        L.oracle_psi_init_avg_precise.argtypes = [_f32p, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, _f32p]
+        L.oracle_psi_init_avg_precise.restype = ctypes.c_double
+        L.oracle_block_seq_f32.argtypes = [_f32p, _i32p, _f32p, _f32p, _f32p, _f32p, _i32p, ctypes.c_float,
+                                           ctypes.c_float, ctypes.c_float, ctypes.c_int, _f64p]
+        for suf, fp in (("f32", _f32p), ("f64", _f64p)):
+            f = getattr(L, "oracle_deconvolve_" + suf)
+            f.argtypes = [_i32p, _f32p, _f32p, _f32p, _f32p, _f32p, fp, _f32p, _f64p, fp, _f64p]
+        _lib = L
+    return _lib
+
+
+def set_threads(n: int) -> int:
+    return lib().oracle_set_threads(int(n))
+
+
+def _dims_xyz(a: np.ndarray) -> np.ndarray:
+    """numpy arrays are indexed [z, y, x] (x fastest, ImgLib2 flat order)."""
+    return np.array([a.shape[2], a.shape[1], a.shape[0]], dtype=np.int32)
+
+
+def fast_length(n: int) -> int:
+    return lib().oracle_fast_length(int(n))
+
+
+def divide_into_portions(n: int, nT: int):
+    cap = max(1, nT) + int(n // (64 ** 3)) + 2
+    s = np.zeros(cap, np.int64)
+    l = np.zeros(cap, np.int64)
+    k = lib().oracle_divide_into_portions(int(n), int(nT), s, l, cap)
+    return list(zip(s[:k].tolist(), l[:k].tolist()))
+
+
+def sum_img_quirk(a: np.ndarray, nT: int) -> float:
+    a = np.ascontiguousarray(a, dtype=np.float32).ravel()
+    return lib().oracle_sum_img_quirk_f32(a, a.size, nT)
+
+
+def norm_to_sum1(a: np.ndarray, nT: int) -> np.ndarray:
+    if a.dtype == np.float64:
+        out = np.ascontiguousarray(a, dtype=np.float64).copy()
+        lib().oracle_norm_to_sum1_f64(out.ravel(), out.size, nT)
+    else:
+        out = np.ascontiguousarray(a, dtype=np.float32).copy()
+        lib().oracle_norm_to_sum1_f32(out.ravel(), out.size, nT)
+    return out
+
+
+def mirror(a: np.ndarray) -> np.ndarray:
+    out = np.ascontiguousarray(a, dtype=np.float32).copy()
+    lib().oracle_mirror_f32(out.ravel(), _dims_xyz(out))
+    return out
+
+
+def fft1d(x: np.ndarray, inverse: bool = False) -> np.ndarray:
+    buf = np.ascontiguousarray(np.stack([x.real, x.imag], -1).ravel(), dtype=np.float64)
+    lib().oracle_fft1d_f64(buf, x.size, 1 if inverse else -1)
+    return buf[0::2] + 1j * buf[1::2]
+
+
+def rfft3d(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    pz, py, px = x.shape
+    out = np.zeros(pz * py * (px // 2 + 1) * 2, np.float64)
+    lib().oracle_rfft3d_f64(x.ravel(), np.array([px, py, pz], np.int32), out)
+    return (out[0::2] + 1j * out[1::2]).reshape(pz, py, px // 2 + 1)
+
+
+EXT_MIRROR, EXT_VALUE = 0, 1
+
+
+def fft_convolve(img: np.ndarray, kernel: np.ndarray, ext: str = "mirror", value: float = 0.0,
+                 dtype=np.float32) -> np.ndarray:
+    """util.FFTConvolution: out[x] = sum_j K[j] ext(img)[x + c - j]."""
+    img = np.ascontiguousarray(img, dtype=dtype)
+    kernel = np.ascontiguousarray(kernel, dtype=dtype)
+    out = np.zeros_like(img)
+    e = EXT_MIRROR if ext == "mirror" else EXT_VALUE
+    f = lib().oracle_fft_convolve_f32 if dtype == np.float32 else lib().oracle_fft_convolve_f64
+    f(img.ravel(), _dims_xyz(img), e, value, kernel.ravel(), _dims_xyz(kernel), out.ravel())
+    return out
+
+
+def next_value(last, integral, weight, lam, min_i, max_i, dtype=np.float32):
+    if dtype == np.float32:
+        return np.float32(lib().oracle_next_value_f32(last, integral, weight, lam, min_i, max_i))
+    return lib().oracle_next_value_f64(last, integral, weight, lam, min_i, max_i)
+
+
+def next_value_mul(last, integrals, weights, lam, min_i, max_i):
+    i = np.ascontiguousarray(integrals, np.float32)
+    w = np.ascontiguousarray(weights, np.float32)
+    return np.float32(lib().oracle_next_value_mul_f32(last, i, w, i.size, lam, min_i, max_i))
+
+
+def divide_into_blocks(img_xyz, kernel_size_xyz, block_size_xyz):
+    """Returns a list of dicts (x,y,z order) or None when the block is too small."""
+    cap = 1
+    for n, k, b in zip(img_xyz, kernel_size_xyz, block_size_xyz):
+        e = b - k + 1
+        cap *= max(1, -(-n // e) if e > 0 else 1)
+    out = np.zeros(15 * cap, np.int64)
+    k = lib().oracle_divide_into_blocks(np.array(img_xyz, np.int64), np.array(kernel_size_xyz, np.int64),
+                                        np.array(block_size_xyz, np.int64), out, cap)
+    if k < 0:
+        return None
+    res = []
+    for i in range(k):
+        r = out[15 * i:15 * i + 15]
+        res.append(dict(blockSize=tuple(r[0:3]), offset=tuple(r[3:6]), effectiveSize=tuple(r[6:9]),
+                        effectiveOffset=tuple(r[9:12]), effectiveLocalOffset=tuple(r[12:15])))
+    return res
+
+
+def sort_blocks(img_xyz, kernel_size_xyz, block_size_xyz, min_required=1):
+    blocks = divide_into_blocks(img_xyz, kernel_size_xyz, block_size_xyz)
+    n = len(blocks)
+    batch_of = np.zeros(n, np.int32)
+    order = np.zeros(n, np.int32)
+    dim = np.zeros(1, np.int32)
+    nb = lib().oracle_sort_blocks(np.array(img_xyz, np.int64), np.array(kernel_size_xyz, np.int64),
+                                  np.array(block_size_xyz, np.int64), int(min_required), batch_of, order, dim)
+    batches = [[] for _ in range(nb)]
+    for i in order.tolist():
+        batches[batch_of[i]].append(i)
+    return batches, int(dim[0])
+
+
+def init_psfs(kernels: np.ndarray, psf_type: str, nT: int, dtype=np.float32):
+    """DeconViewPSF.init for all views (list order).  kernels: [V, kz, ky, kx]."""
+    k = np.ascontiguousarray(kernels, dtype=dtype)
+    V = k.shape[0]
+    kd = np.array([k.shape[3], k.shape[2], k.shape[1]], np.int32)
+    k1 = np.zeros_like(k)
+    k2 = np.zeros_like(k)
+    f = lib().oracle_init_psfs_f32 if dtype == np.float32 else lib().oracle_init_psfs_f64
+    f(k.ravel(), V, kd, PSF_TYPES[psf_type], nT, k1.ravel(), k2.ravel())
+    return k1, k2
+
+
+def psi_init_avg_precise(imgs: np.ndarray, nT: int):
+    imgs = np.ascontiguousarray(imgs, np.float32)
+    V = imgs.shape[0]
+    mx = np.zeros(V, np.float32)
+    avg = lib().oracle_psi_init_avg_precise(imgs.ravel(), V, imgs[0].size, nT, mx)
+    return avg, mx
+
+
+def block_seq(psi_blk, img_blk, w_blk, k1, k2, lam, min_value, max_i, nT):
+    """One ComputeBlockSeqThreadCPU.runIteration on a block (kernels already final)."""
+    psi = np.ascontiguousarray(psi_blk, np.float32).copy()
+    stats = np.zeros(2, np.float64)
+    lib().oracle_block_seq_f32(psi.ravel(), _dims_xyz(psi), np.ascontiguousarray(img_blk, np.float32).ravel(),
+                               np.ascontiguousarray(w_blk, np.float32).ravel(),
+                               np.ascontiguousarray(k1, np.float32).ravel(),
+                               np.ascontiguousarray(k2, np.float32).ravel(), _dims_xyz(k1), lam, min_value, max_i,
+                               nT, stats)
+    return psi, stats
+
+
+def deconvolve(imgs, weights, kernels, *, psf_type="EFFICIENT_BAYESIAN", scheme="SEQ", block_size=None,
+               lam=0.0, min_value=1e-4, iterations=10, nT=8, psi0=None, filter_blocks=True, dtype=np.float32):
+    """Run the restated reference driver (MultiViewDeconvolution{Seq,Mul}).
+
+    imgs, weights: [V, z, y, x] float32.  kernels: [V, kz, ky, kx] float32 (raw PSFs,
+    normalised inside as DeconViewPSF.init does).  block_size (x,y,z) defaults to
+    one block covering the volume: N + 2(K-1).
+    Returns dict(psi, max, stats[it, v, 2] or [it, 2], k2, avg).
+    """
+    imgs = np.ascontiguousarray(imgs, np.float32)
+    weights = np.ascontiguousarray(weights, np.float32)
+    kernels = np.ascontiguousarray(kernels, np.float32)
+    V, nz, ny, nx = imgs.shape
+    kz, ky, kx = kernels.shape[1:]
+    if block_size is None:
+        block_size = (nx + 2 * (kx - 1), ny + 2 * (ky - 1), nz + 2 * (kz - 1))
+    ip = np.array([nx, ny, nz, kx, ky, kz, *block_size, V, PSF_TYPES[psf_type], SCHEMES[scheme], iterations, nT,
+                   0 if psi0 is None else 1, 1 if filter_blocks else 0], np.int32)
+    fp = np.array([lam, min_value], np.float32)
+    psi_in = np.ascontiguousarray(psi0 if psi0 is not None else np.zeros(1), np.float32)
+    psi = np.zeros((nz, ny, nx), dtype)
+    mx = np.zeros(V, np.float32)
+    nst = iterations * (V if scheme == "SEQ" else 1) * 2
+    stats = np.zeros(nst, np.float64)
+    k2 = np.zeros((V, kz, ky, kx), dtype)
+    avg = np.zeros(1, np.float64)
+    f = lib().oracle_deconvolve_f32 if dtype == np.float32 else lib().oracle_deconvolve_f64
+    rc = f(ip, fp, imgs.ravel(), weights.ravel(), kernels.ravel(), psi_in.ravel(), psi.ravel(), mx, stats, k2.ravel(),
+           avg)
+    if rc != 0:
+        raise ValueError("oracle deconvolve failed (block smaller than kernel?)")
+    shape = (iterations, V, 2) if scheme == "SEQ" else (iterations, 2)
+    return dict(psi=psi, max=mx, stats=stats.reshape(shape), k2=k2, avg=float(avg[0]))
