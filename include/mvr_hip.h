@@ -1,0 +1,133 @@
+/* mvr_hip.h -- C ABI of libmvr_hip.so, the MI355X (gfx950) multiview
+ * Richardson-Lucy deconvolution path.
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success and a negative value on error; the message is in
+ *     mvr_last_error() (thread-local);
+ *   - arrays are float32 in ImgLib2 flat order (x fastest), dims are given
+ *     as {x, y, z} unless the name says ZYX;
+ *   - array arguments may be host memory or device memory of the handle's
+ *     GPU (hipMemcpyDefault semantics); they are only borrowed for the call;
+ *   - calls are synchronous; one handle must not be used by two threads at
+ *     once, different handles may run concurrently (one per Java block thread).
+ *
+ * Citations are relative to src/main/java/net/preibisch/mvrecon/process/ of
+ * the reference (PreibischLab/multiview-reconstruction).
+ */
+#ifndef MVR_HIP_H
+#define MVR_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* PSF types, enum order of DeconViewPSF.PSFTYPE (deconvolution/DeconViewPSF.java:52) */
+#define MVR_PSF_OPTIMIZATION_II 0
+#define MVR_PSF_OPTIMIZATION_I 1
+#define MVR_PSF_EFFICIENT_BAYESIAN 2
+#define MVR_PSF_INDEPENDENT 3
+/* iteration schemes: MultiViewDeconvolutionSeq / MultiViewDeconvolutionMul */
+#define MVR_SCHEME_SEQ 0
+#define MVR_SCHEME_MUL 1
+/* PsiInit: PsiInitAvgPrecise (deconvolution/init/PsiInitAvgPrecise.java:52-112),
+ * PsiInitFromRAI with the precise max (deconvolution/init/PsiInitFromRAI.java:65-82) */
+#define MVR_PSI_INIT_AVG_PRECISE 0
+#define MVR_PSI_INIT_FROM_ARRAY 1
+/* extension modes of mvr_fft_convolve (util/FFTConvolution.java usage in
+ * ComputeBlockSeqThreadCPU.java:171-209 and DeconViewPSF.java:152-225) */
+#define MVR_EXT_MIRROR_SINGLE 0
+#define MVR_EXT_VALUE 1
+
+/* ---- device queries: replace CUDAStandardFunctions (cuda/CUDAStandardFunctions.java:27-46) ---- */
+/* getNumDevicesCUDA: number of HIP devices, -1 if the runtime failed */
+int mvr_get_num_devices(void);
+/* getNameDeviceCUDA: device name into name[len] */
+int mvr_get_device_name(int dev, char* name, int len);
+/* getMemDeviceCUDA / getFreeMemDeviceCUDA: bytes, -1 on error */
+long long mvr_get_mem_device(int dev);
+long long mvr_get_free_mem_device(int dev);
+/* getCUDAcomputeCapabilityMajor/MinorVersion: gfx architecture (9 / 50 for gfx950) */
+int mvr_get_arch_major(int dev);
+int mvr_get_arch_minor(int dev);
+
+/* ---- replaces CUDAFourierConvolution.convolution3DfftCUDAInPlace (cuda/CUDAFourierConvolution.java:32),
+ * called by ComputeBlockSeqThreadCUDA.convolve1/convolve2 (deconvolution/iteration/sequential/
+ * ComputeBlockSeqThreadCUDA.java:171-208).  In-place 3D convolution of im with kernel, dims as {z,y,x}
+ * like CUDATools.getCUDACoordinates.  The image is extended by mirror-single reflection (the CPU
+ * path's convolve1); the closed CUDA library's border behaviour is not specified by the reference. */
+int mvr_convolution3d_fft_inplace(float* im, const int* imDimZYX, const float* kernel, const int* kernelDimZYX,
+                                  int dev);
+
+/* ---- one util.FFTConvolution.convolve (util/FFTConvolution.java:490-506): out = kernel (*) ext(img) ---- */
+int mvr_fft_convolve(int dev, const float* img, const int nXYZ[3], int ext, float extValue, const float* kernel,
+                     const int kXYZ[3], float* out);
+
+/* ---- DeconViewPSF.init for all views in list order (deconvolution/DeconViewPSF.java:119-254,
+ * deconvolution/DeconViews.java:69-70): kernel1 normalised to sum 1 (AdjustInput.normToSum1 with the
+ * numeric thread count nT), kernel2 per PSF type; compound-kernel convolutions run on the GPU.
+ * kernels/k1Out/k2Out hold numViews kernels of kXYZ each, back to back. ---- */
+int mvr_init_psfs(int dev, const float* kernels, int numViews, const int kXYZ[3], int psfType, int nT, float* k1Out,
+                  float* k2Out);
+
+/* ---- per-block compute thread: replaces ComputeBlockSeqThreadCPU / ComputeBlockMulThreadCPU
+ * (deconvolution/iteration/sequential/ComputeBlockSeqThreadCPU.java:53-169,
+ *  deconvolution/iteration/mul/ComputeBlockMulThreadCPU.java:87-188), one per Java block thread. ---- */
+typedef struct mvr_block_thread mvr_block_thread;
+/* constructor (minValue, lambda, id, blockSize) */
+int mvr_block_thread_create(mvr_block_thread** out, int dev, const int blockSizeXYZ[3], float minValue, float lambda,
+                            int id);
+/* DeconViewPSF.getKernel1FFT/getKernel2FFT (deconvolution/DeconViewPSF.java:79-111): spectra of the
+ * view's final kernel1/kernel2 at block size, cached per view id */
+int mvr_block_thread_set_view_kernels(mvr_block_thread* t, int view, const float* kernel1, const float* kernel2,
+                                      const int kXYZ[3]);
+/* ComputeBlockSeqThread.runIteration (iteration/sequential/ComputeBlockSeqThread.java:54-61): psiBlock
+ * (= getPsiBlockTmp, mirror-extended psi) is updated in place; imgBlock/weightBlock are the zero-extended
+ * view image and weights over the block; stats = {sumChange, maxChange} (IterationStatistics) */
+int mvr_block_run_iteration_seq(mvr_block_thread* t, int view, float* psiBlock, const float* imgBlock,
+                                const float* weightBlock, float maxIntensityView, double stats[2]);
+/* ComputeBlockMulThread.runIteration (iteration/mul/ComputeBlockMulThread.java:56-62) over views 0..numViews-1 */
+int mvr_block_run_iteration_mul(mvr_block_thread* t, int numViews, float* psiBlock, const float* const* imgBlocks,
+                                const float* const* weightBlocks, const float* maxIntensityViews, double stats[2]);
+void mvr_block_thread_destroy(mvr_block_thread* t);
+
+/* ---- resident deconvolution: the whole MultiViewDeconvolution{Seq,Mul} run on one GPU
+ * (deconvolution/MultiViewDeconvolution.java:90-212, MultiViewDeconvolutionSeq.java:58-180,
+ *  MultiViewDeconvolutionMul.java:116-245).  psi, views, weights and kernel spectra stay in HBM; the
+ * psi volume is treated as one block whose faces are mirror-extended, which equals the reference's
+ * blocked result on every pasted voxel (margin K-1 per block). ---- */
+typedef struct mvr_decon mvr_decon;
+/* DeconViews + MultiViewDeconvolution ctor parameters; nT is the numeric thread count of the
+ * kernel normalisation (Threads.numThreads(), Threads.java:40) */
+int mvr_decon_create(mvr_decon** out, int dev, const int psiDimXYZ[3], int numViews, int psfType, int scheme,
+                     float lambda, float minValue, int nT);
+/* DeconView(image, weight, kernel) (deconvolution/DeconView.java:118-184) */
+int mvr_decon_set_view(mvr_decon* d, int view, const float* img, const float* weight, const float* kernel,
+                       const int kXYZ[3]);
+/* DeconViewPSF.init for all views + PsiInit.runInitialization (MultiViewDeconvolution.java:102-135) */
+int mvr_decon_init(mvr_decon* d, int psiInit, const float* psi0);
+/* runIterations: `iterations` more iterations */
+int mvr_decon_run(mvr_decon* d, int iterations);
+int mvr_decon_get_psi(mvr_decon* d, float* out);
+/* PsiInit.getMax / getAvg */
+int mvr_decon_get_max(mvr_decon* d, float* maxOut, double* avgOut);
+/* DeconViewPSF.getKernel1/getKernel2 after init */
+int mvr_decon_get_kernels(mvr_decon* d, int view, float* kernel1, float* kernel2);
+/* IterationStatistics {sumChange, maxChange} per view (Seq) or per iteration (Mul), in run order;
+ * returns the number of doubles available */
+int mvr_decon_get_stats(mvr_decon* d, double* out, int cap);
+/* restore psi to the initial estimate (repeatable runs) */
+int mvr_decon_reset_psi(mvr_decon* d);
+/* FFT geometry: out[0..2] = FFT lengths, out[3] = half-spectrum row pitch */
+int mvr_decon_get_geometry(mvr_decon* d, int out[4]);
+/* per-pass GPU timing with HIP events on the library's stream; categories:
+ * 0 x_fwd, 1 y_fwd, 2 z_conv, 3 y_inv, 4 x_ratio, 5 x_update, 6 x_store, 7 update_mul, 8 reduce */
+int mvr_decon_profile(mvr_decon* d, int enable);
+int mvr_decon_pass_times(mvr_decon* d, double msTotal[9], long long launches[9]);
+void mvr_decon_destroy(mvr_decon* d);
+
+const char* mvr_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

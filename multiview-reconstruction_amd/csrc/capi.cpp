@@ -1,0 +1,282 @@
+// capi.cpp -- extern "C" entry points declared in include/mvr_hip.h.
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/mvr_hip.h"
+#include "host.hpp"
+
+using namespace mvr;
+
+struct mvr_block_thread { BlockThread t; };
+struct mvr_decon { Decon d; };
+
+#define NULLCHECK(p)                              \
+  do {                                            \
+    if (!(p)) {                                   \
+      set_error("null handle or argument: " #p);  \
+      return -1;                                  \
+    }                                             \
+  } while (0)
+
+extern "C" {
+
+const char* mvr_last_error(void) { return last_error(); }
+
+int mvr_get_num_devices(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return -1;
+  return n;
+}
+
+int mvr_get_device_name(int dev, char* name, int len) {
+  NULLCHECK(name);
+  hipDeviceProp_t p;
+  MVR_HIP(hipGetDeviceProperties(&p, dev));
+  const char* src = p.name[0] ? p.name : p.gcnArchName;
+  std::strncpy(name, src, len > 0 ? len - 1 : 0);
+  if (len > 0) name[len - 1] = 0;
+  return 0;
+}
+
+long long mvr_get_mem_device(int dev) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, dev) != hipSuccess) return -1;
+  return (long long)p.totalGlobalMem;
+}
+
+long long mvr_get_free_mem_device(int dev) {
+  if (hipSetDevice(dev) != hipSuccess) return -1;
+  size_t f = 0, t = 0;
+  if (hipMemGetInfo(&f, &t) != hipSuccess) return -1;
+  return (long long)f;
+}
+
+int mvr_get_arch_major(int dev) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, dev) != hipSuccess) return -1;
+  return p.major;
+}
+
+int mvr_get_arch_minor(int dev) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, dev) != hipSuccess) return -1;
+  return p.minor;
+}
+
+int mvr_fft_convolve(int dev, const float* img, const int n[3], int ext, float extValue, const float* kernel,
+                     const int k[3], float* out) {
+  NULLCHECK(img);
+  NULLCHECK(kernel);
+  NULLCHECK(out);
+  if (ext != MVR_EXT_MIRROR_SINGLE && ext != MVR_EXT_VALUE) { set_error("unknown extension"); return -1; }
+  MVR_HIP(hipSetDevice(dev));
+  Plan plan;
+  if (plan.create(n, k)) return -1;
+  const size_t nv = (size_t)n[0] * n[1] * n[2], nk = (size_t)k[0] * k[1] * k[2];
+  hipStream_t s = nullptr;
+  float *dImg = nullptr, *dK = nullptr, *dOut = nullptr;
+  float2 *spec = nullptr, *work = nullptr;
+  int rc = -1;
+  do {
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) || hipMalloc(&dImg, nv * 4) || hipMalloc(&dK, nk * 4) ||
+        hipMalloc(&dOut, nv * 4) || hipMalloc(&spec, plan.ncpx() * 8) || hipMalloc(&work, plan.ncpx() * 8)) {
+      set_error("mvr_fft_convolve: allocation failed");
+      break;
+    }
+    if (hipMemcpyAsync(dImg, img, nv * 4, hipMemcpyDefault, s) || hipMemcpyAsync(dK, kernel, nk * 4, hipMemcpyDefault, s))
+      break;
+    if (plan.kernel_spectrum(dK, spec, s)) break;
+    GatherParams g{};
+    g.src = dImg;
+    for (int d = 0; d < 3; ++d) { g.N[d] = n[d]; g.org[d] = -plan.dom.m[d]; }
+    g.mode = ext == MVR_EXT_MIRROR_SINGLE ? GATHER_MIRROR : GATHER_VALUE;
+    g.value = extValue;
+    if (plan.convolve(g, spec, work, dOut, s)) break;
+    if (hipMemcpyAsync(out, dOut, nv * 4, hipMemcpyDefault, s) || hipStreamSynchronize(s)) {
+      set_error("mvr_fft_convolve: readback failed");
+      break;
+    }
+    rc = 0;
+  } while (0);
+  for (void* p : {(void*)dImg, (void*)dK, (void*)dOut, (void*)spec, (void*)work})
+    if (p) (void)hipFree(p);
+  if (s) (void)hipStreamDestroy(s);
+  plan.destroy();
+  return rc;
+}
+
+int mvr_convolution3d_fft_inplace(float* im, const int* imDimZYX, const float* kernel, const int* kernelDimZYX,
+                                  int dev) {
+  NULLCHECK(imDimZYX);
+  NULLCHECK(kernelDimZYX);
+  const int n[3] = {imDimZYX[2], imDimZYX[1], imDimZYX[0]};
+  const int k[3] = {kernelDimZYX[2], kernelDimZYX[1], kernelDimZYX[0]};
+  return mvr_fft_convolve(dev, im, n, MVR_EXT_MIRROR_SINGLE, 0.f, kernel, k, im);
+}
+
+int mvr_init_psfs(int dev, const float* kernels, int V, const int k[3], int psfType, int nT, float* k1Out,
+                  float* k2Out) {
+  NULLCHECK(kernels);
+  NULLCHECK(k);
+  NULLCHECK(k1Out);
+  NULLCHECK(k2Out);
+  if (V < 1 || psfType < 0 || psfType > 3) { set_error("bad view count or PSF type"); return -1; }
+  MVR_HIP(hipSetDevice(dev));
+  const size_t nk = (size_t)k[0] * k[1] * k[2];
+  std::vector<std::vector<float>> k1(V), k2;
+  for (int v = 0; v < V; ++v) k1[v].assign(kernels + nk * v, kernels + nk * (v + 1));
+  hipStream_t s = nullptr;
+  MVR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  const int rc = init_psfs(dev, k1, k, psfType, nT, k2, s);
+  (void)hipStreamDestroy(s);
+  if (rc) return rc;
+  for (int v = 0; v < V; ++v) {
+    std::memcpy(k1Out + nk * v, k1[v].data(), nk * 4);
+    std::memcpy(k2Out + nk * v, k2[v].data(), nk * 4);
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------- block thread
+int mvr_block_thread_create(mvr_block_thread** out, int dev, const int b[3], float minValue, float lambda, int id) {
+  NULLCHECK(out);
+  NULLCHECK(b);
+  auto* h = new (std::nothrow) mvr_block_thread();
+  if (!h) { set_error("out of host memory"); return -1; }
+  if (h->t.create(dev, b, minValue, lambda, id)) { h->t.destroy(); delete h; return -1; }
+  *out = h;
+  return 0;
+}
+
+int mvr_block_thread_set_view_kernels(mvr_block_thread* t, int view, const float* k1, const float* k2,
+                                      const int k[3]) {
+  NULLCHECK(t);
+  NULLCHECK(k1);
+  NULLCHECK(k2);
+  NULLCHECK(k);
+  return t->t.set_view_kernels(view, k1, k2, k);
+}
+
+int mvr_block_run_iteration_seq(mvr_block_thread* t, int view, float* psi, const float* img, const float* w,
+                                float maxI, double stats[2]) {
+  NULLCHECK(t);
+  NULLCHECK(psi);
+  NULLCHECK(img);
+  NULLCHECK(w);
+  return t->t.run_seq(view, psi, img, w, maxI, stats);
+}
+
+int mvr_block_run_iteration_mul(mvr_block_thread* t, int nv, float* psi, const float* const* imgs,
+                                const float* const* ws, const float* maxIs, double stats[2]) {
+  NULLCHECK(t);
+  NULLCHECK(psi);
+  NULLCHECK(imgs);
+  NULLCHECK(ws);
+  NULLCHECK(maxIs);
+  return t->t.run_mul(nv, psi, imgs, ws, maxIs, stats);
+}
+
+void mvr_block_thread_destroy(mvr_block_thread* t) {
+  if (!t) return;
+  t->t.destroy();
+  delete t;
+}
+
+// ---------------------------------------------------------------- resident driver
+int mvr_decon_create(mvr_decon** out, int dev, const int n[3], int V, int psfType, int scheme, float lambda,
+                     float minValue, int nT) {
+  NULLCHECK(out);
+  NULLCHECK(n);
+  auto* h = new (std::nothrow) mvr_decon();
+  if (!h) { set_error("out of host memory"); return -1; }
+  if (h->d.create(dev, n, V, psfType, scheme, lambda, minValue, nT)) { h->d.destroy(); delete h; return -1; }
+  *out = h;
+  return 0;
+}
+
+int mvr_decon_set_view(mvr_decon* d, int v, const float* img, const float* w, const float* k, const int kd[3]) {
+  NULLCHECK(d);
+  NULLCHECK(kd);
+  return d->d.set_view(v, img, w, k, kd);
+}
+
+int mvr_decon_init(mvr_decon* d, int psiInit, const float* psi0) {
+  NULLCHECK(d);
+  if (psiInit != MVR_PSI_INIT_AVG_PRECISE && psiInit != MVR_PSI_INIT_FROM_ARRAY) {
+    set_error("unknown psi init");
+    return -1;
+  }
+  return d->d.init(psiInit, psi0);
+}
+
+int mvr_decon_run(mvr_decon* d, int iterations) {
+  NULLCHECK(d);
+  return d->d.run(iterations);
+}
+
+int mvr_decon_get_psi(mvr_decon* d, float* out) {
+  NULLCHECK(d);
+  NULLCHECK(out);
+  if (!d->d.ready) { set_error("init first"); return -1; }
+  MVR_HIP(hipSetDevice(d->d.dev));
+  MVR_HIP(hipMemcpyAsync(out, d->d.psi, d->d.nvox() * 4, hipMemcpyDefault, d->d.stream));
+  MVR_HIP(hipStreamSynchronize(d->d.stream));
+  return 0;
+}
+
+int mvr_decon_get_max(mvr_decon* d, float* maxOut, double* avgOut) {
+  NULLCHECK(d);
+  if (!d->d.ready) { set_error("init first"); return -1; }
+  if (maxOut) for (int v = 0; v < d->d.V; ++v) maxOut[v] = d->d.maxI[v];
+  if (avgOut) *avgOut = d->d.avg;
+  return 0;
+}
+
+int mvr_decon_get_kernels(mvr_decon* d, int v, float* k1, float* k2) {
+  NULLCHECK(d);
+  if (!d->d.ready || v < 0 || v >= d->d.V) { set_error("init first / bad view"); return -1; }
+  if (k1) std::memcpy(k1, d->d.k1[v].data(), d->d.k1[v].size() * 4);
+  if (k2) std::memcpy(k2, d->d.k2[v].data(), d->d.k2[v].size() * 4);
+  return 0;
+}
+
+int mvr_decon_get_stats(mvr_decon* d, double* out, int cap) {
+  NULLCHECK(d);
+  const int n = (int)d->d.stats.size();
+  if (out) for (int i = 0; i < n && i < cap; ++i) out[i] = d->d.stats[i];
+  return n;
+}
+
+int mvr_decon_reset_psi(mvr_decon* d) {
+  NULLCHECK(d);
+  return d->d.reset_psi();
+}
+
+int mvr_decon_get_geometry(mvr_decon* d, int out[4]) {
+  NULLCHECK(d);
+  NULLCHECK(out);
+  if (!d->d.ready) { set_error("init first"); return -1; }
+  for (int k = 0; k < 3; ++k) out[k] = d->d.plan.dom.P[k];
+  out[3] = d->d.plan.dom.HxP;
+  return 0;
+}
+
+int mvr_decon_profile(mvr_decon* d, int enable) {
+  NULLCHECK(d);
+  d->d.timer.on = enable != 0;
+  return 0;
+}
+
+int mvr_decon_pass_times(mvr_decon* d, double ms[9], long long n[9]) {
+  NULLCHECK(d);
+  return d->d.timer.collect(ms, n);
+}
+
+void mvr_decon_destroy(mvr_decon* d) {
+  if (!d) return;
+  d->d.destroy();
+  delete d;
+}
+
+}  // extern "C"

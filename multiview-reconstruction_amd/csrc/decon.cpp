@@ -1,0 +1,510 @@
+// decon.cpp -- resident multiview driver and per-block compute threads.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "host.hpp"
+
+namespace mvr {
+
+#define MVR_CHECK(cond, msg)  \
+  do {                        \
+    if (!(cond)) {            \
+      set_error(msg);         \
+      return -1;              \
+    }                         \
+  } while (0)
+
+// ============================================================ resident driver
+
+int Decon::create(int dev_, const int N_[3], int V_, int psfType_, int scheme_, float lambda_, float minValue_,
+                  int nT_) {
+  dev = dev_;
+  MVR_CHECK(V_ >= 1 && V_ <= 32, "numViews must be in [1, 32]");
+  MVR_CHECK(psfType_ >= 0 && psfType_ <= 3, "unknown PSF type");
+  MVR_CHECK(scheme_ == SCHEME_SEQ || scheme_ == SCHEME_MUL, "unknown iteration scheme");
+  for (int d = 0; d < 3; ++d) {
+    MVR_CHECK(N_[d] >= 1, "psi dimensions must be positive");
+    N[d] = N_[d];
+  }
+  V = V_;
+  psfType = psfType_;
+  scheme = scheme_;
+  lambda = lambda_;
+  minValue = minValue_;
+  nT = nT_;
+  MVR_HIP(hipSetDevice(dev));
+  MVR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  img.assign(V, nullptr);
+  w.assign(V, nullptr);
+  kraw.assign(V, {});
+  for (int v = 0; v < V; ++v) {
+    MVR_HIP(hipMalloc(&img[v], nvox() * 4));
+    MVR_HIP(hipMalloc(&w[v], nvox() * 4));
+  }
+  MVR_HIP(hipMalloc(&psi, nvox() * 4));
+  MVR_HIP(hipMalloc(&psi0, nvox() * 4));
+  return 0;
+}
+
+int Decon::set_view(int v, const float* im, const float* wt, const float* kernel, const int Kd[3]) {
+  MVR_CHECK(v >= 0 && v < V, "view index out of range");
+  MVR_CHECK(im && wt && kernel, "null input");
+  for (int d = 0; d < 3; ++d) {
+    MVR_CHECK(Kd[d] >= 1 && Kd[d] <= N[d] + 64 * 1024, "bad kernel size");
+    if (K[0] != 0) MVR_CHECK(K[d] == Kd[d], "all views must share one PSF size");
+  }
+  for (int d = 0; d < 3; ++d) K[d] = Kd[d];
+  MVR_HIP(hipSetDevice(dev));
+  MVR_HIP(hipMemcpyAsync(img[v], im, nvox() * 4, hipMemcpyDefault, stream));
+  MVR_HIP(hipMemcpyAsync(w[v], wt, nvox() * 4, hipMemcpyDefault, stream));
+  const size_t nk = (size_t)K[0] * K[1] * K[2];
+  kraw[v].resize(nk);
+  MVR_HIP(hipMemcpyAsync(kraw[v].data(), kernel, nk * 4, hipMemcpyDefault, stream));
+  MVR_HIP(hipStreamSynchronize(stream));
+  ready = false;
+  return 0;
+}
+
+int Decon::ensure_partials(int n) {
+  if (n <= partialCap) return 0;
+  if (partial) (void)hipFree(partial);
+  MVR_HIP(hipMalloc(&partial, sizeof(double2) * n));
+  partialCap = n;
+  return 0;
+}
+
+int Decon::init(int psiInit, const float* psiIn) {
+  MVR_HIP(hipSetDevice(dev));
+  for (int v = 0; v < V; ++v) MVR_CHECK(!kraw[v].empty(), "every view needs set_view before init");
+  // PSFs (DeconViews ctor -> DeconViewPSF.init)
+  k1 = kraw;
+  if (init_psfs(dev, k1, K, psfType, nT, k2, stream)) return -1;
+  // plan over the whole psi volume, mirror-single extension at its faces
+  plan.destroy();
+  if (plan.create(N, K)) return -1;
+  for (auto p : spec1) if (p) (void)hipFree(p);
+  for (auto p : spec2) if (p) (void)hipFree(p);
+  spec1.assign(V, nullptr);
+  spec2.assign(V, nullptr);
+  const size_t nk = (size_t)K[0] * K[1] * K[2];
+  float* dk = nullptr;
+  MVR_HIP(hipMalloc(&dk, nk * 4));
+  for (int v = 0; v < V; ++v) {
+    MVR_HIP(hipMalloc(&spec1[v], plan.ncpx() * 8));
+    MVR_HIP(hipMalloc(&spec2[v], plan.ncpx() * 8));
+    MVR_HIP(hipMemcpyAsync(dk, k1[v].data(), nk * 4, hipMemcpyHostToDevice, stream));
+    if (plan.kernel_spectrum(dk, spec1[v], stream)) return -1;
+    MVR_HIP(hipMemcpyAsync(dk, k2[v].data(), nk * 4, hipMemcpyHostToDevice, stream));
+    if (plan.kernel_spectrum(dk, spec2[v], stream)) return -1;
+    MVR_HIP(hipStreamSynchronize(stream));
+  }
+  (void)hipFree(dk);
+  if (S) (void)hipFree(S);
+  MVR_HIP(hipMalloc(&S, plan.ncpx() * 8));
+  if (scheme == SCHEME_MUL) {
+    if (Spsi) (void)hipFree(Spsi);
+    if (T2) (void)hipFree(T2);
+    MVR_HIP(hipMalloc(&Spsi, plan.ncpx() * 8));
+    MVR_HIP(hipMalloc(&T2, nvox() * 4 * V));
+  }
+  // PsiInitAvgPrecise (D/init/PsiInitAvgPrecise.java:52-112)
+  {
+    const int maxParts = 1024;
+    double* psum = nullptr;
+    unsigned long long* pcnt = nullptr;
+    float* pmax = nullptr;
+    MVR_HIP(hipMalloc(&psum, maxParts * 8));
+    MVR_HIP(hipMalloc(&pcnt, maxParts * 8));
+    MVR_HIP(hipMalloc(&pmax, (size_t)maxParts * V * 4));
+    std::vector<const float*> ptrs(img.begin(), img.end());
+    const int np = launch_psi_init_partials(ptrs.data(), V, nvox(), psum, pcnt, pmax, stream);
+    MVR_HIP(hipGetLastError());
+    std::vector<double> hs(np);
+    std::vector<unsigned long long> hc(np);
+    std::vector<float> hm((size_t)np * V);
+    MVR_HIP(hipMemcpyAsync(hs.data(), psum, np * 8, hipMemcpyDeviceToHost, stream));
+    MVR_HIP(hipMemcpyAsync(hc.data(), pcnt, np * 8, hipMemcpyDeviceToHost, stream));
+    MVR_HIP(hipMemcpyAsync(hm.data(), pmax, (size_t)np * V * 4, hipMemcpyDeviceToHost, stream));
+    MVR_HIP(hipStreamSynchronize(stream));
+    (void)hipFree(psum);
+    (void)hipFree(pcnt);
+    (void)hipFree(pmax);
+    double s = 0.0;
+    unsigned long long c = 0;
+    maxI.assign(V, 0.f);
+    for (int i = 0; i < np; ++i) {
+      s += hs[i];
+      c += hc[i];
+      for (int v = 0; v < V; ++v) maxI[v] = std::max(maxI[v], hm[(size_t)i * V + v]);
+    }
+    avg = s / (double)c;
+    if (std::isnan(avg)) avg = 1.0;
+  }
+  if (psiInit == 0) {
+    launch_fill(psi, nvox(), (float)avg, stream);
+  } else {
+    MVR_CHECK(psiIn != nullptr, "psi init from array needs an array");
+    MVR_HIP(hipMemcpyAsync(psi, psiIn, nvox() * 4, hipMemcpyDefault, stream));
+  }
+  MVR_HIP(hipMemcpyAsync(psi0, psi, nvox() * 4, hipMemcpyDeviceToDevice, stream));
+  MVR_HIP(hipStreamSynchronize(stream));
+  stats.clear();
+  ready = true;
+  return 0;
+}
+
+int Decon::reset_psi() {
+  MVR_CHECK(ready, "init first");
+  MVR_HIP(hipSetDevice(dev));
+  MVR_HIP(hipMemcpyAsync(psi, psi0, nvox() * 4, hipMemcpyDeviceToDevice, stream));
+  MVR_HIP(hipStreamSynchronize(stream));
+  stats.clear();
+  return 0;
+}
+
+// One view of one Seq iteration over the whole psi volume.
+int Decon::view_seq(int v, int statIdx) {
+  const Dom& d = plan.dom;
+  const StridedGeom gy = plan.geom_y(), gz = plan.geom_z();
+  GatherParams g{};
+  g.src = psi;
+  for (int k = 0; k < 3; ++k) { g.N[k] = N[k]; g.org[k] = -d.m[k]; }
+  g.mode = GATHER_MIRROR;
+  timer.begin(stream, PASS_X_FWD);
+  plan.xf(d, g, S, plan.twM, plan.wr, stream);
+  timer.end(stream);
+  timer.begin(stream, PASS_Y_FWD);
+  plan.sy(gy, 0, S, S, nullptr, 1.f, plan.twY, stream);
+  timer.end(stream);
+  timer.begin(stream, PASS_Z_CONV);
+  plan.sz(gz, 2, S, S, spec1[v], 1.f, plan.twZ, stream);
+  timer.end(stream);
+  timer.begin(stream, PASS_Y_INV);
+  plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+  timer.end(stream);
+  RatioParams q{};
+  q.img = img[v];
+  for (int k = 0; k < 3; ++k) { q.N[k] = N[k]; q.o[k] = 0; }
+  timer.begin(stream, PASS_X_RATIO);
+  plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
+  timer.end(stream);
+  timer.begin(stream, PASS_Y_FWD);
+  plan.sy(gy, 0, S, S, nullptr, 1.f, plan.twY, stream);
+  timer.end(stream);
+  timer.begin(stream, PASS_Z_CONV);
+  plan.sz(gz, 2, S, S, spec2[v], 1.f, plan.twZ, stream);
+  timer.end(stream);
+  timer.begin(stream, PASS_Y_INV);
+  plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+  timer.end(stream);
+  UpdateParams u{};
+  u.psi = psi;
+  u.w = w[v];
+  for (int k = 0; k < 3; ++k) { u.N[k] = N[k]; u.o[k] = 0; }
+  u.lambda = (double)lambda;
+  u.minValue = minValue;
+  u.maxI = maxI[v];
+  u.partial = partial;
+  timer.begin(stream, PASS_X_UPDATE);
+  const int np = plan.xu(d, u, S, plan.twM, plan.wr, stream);
+  timer.end(stream);
+  timer.begin(stream, PASS_REDUCE);
+  launch_reduce_stats(partial, np, statsDev + statIdx, stream);
+  timer.end(stream);
+  MVR_HIP(hipGetLastError());
+  return 0;
+}
+
+// One Mul iteration (ComputeBlockMulThreadCPU.runIteration semantics on the whole volume).
+int Decon::iteration_mul(int statIdx) {
+  const Dom& d = plan.dom;
+  const StridedGeom gy = plan.geom_y(), gz = plan.geom_z();
+  GatherParams g{};
+  g.src = psi;
+  for (int k = 0; k < 3; ++k) { g.N[k] = N[k]; g.org[k] = -d.m[k]; }
+  g.mode = GATHER_MIRROR;
+  // the forward transform of psi is shared by all views
+  timer.begin(stream, PASS_X_FWD);
+  plan.xf(d, g, Spsi, plan.twM, plan.wr, stream);
+  timer.end(stream);
+  timer.begin(stream, PASS_Y_FWD);
+  plan.sy(gy, 0, Spsi, Spsi, nullptr, 1.f, plan.twY, stream);
+  timer.end(stream);
+  std::vector<const float*> t2p(V), wp(V);
+  for (int v = 0; v < V; ++v) {
+    timer.begin(stream, PASS_Z_CONV);
+    plan.sz(gz, 2, Spsi, S, spec1[v], 1.f, plan.twZ, stream);
+    timer.end(stream);
+    timer.begin(stream, PASS_Y_INV);
+    plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+    timer.end(stream);
+    RatioParams q{};
+    q.img = img[v];
+    for (int k = 0; k < 3; ++k) { q.N[k] = N[k]; q.o[k] = 0; }
+    timer.begin(stream, PASS_X_RATIO);
+    plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
+    timer.end(stream);
+    timer.begin(stream, PASS_Y_FWD);
+    plan.sy(gy, 0, S, S, nullptr, 1.f, plan.twY, stream);
+    timer.end(stream);
+    timer.begin(stream, PASS_Z_CONV);
+    plan.sz(gz, 2, S, S, spec2[v], 1.f, plan.twZ, stream);
+    timer.end(stream);
+    timer.begin(stream, PASS_Y_INV);
+    plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+    timer.end(stream);
+    StoreParams st{T2 + nvox() * v};
+    timer.begin(stream, PASS_X_STORE);
+    plan.xs(d, st, S, plan.twM, plan.wr, stream);
+    timer.end(stream);
+    t2p[v] = T2 + nvox() * v;
+    wp[v] = w[v];
+  }
+  double m = 0;
+  for (int v = 0; v < V; ++v) m += maxI[v];
+  m /= (double)V;
+  const int o[3] = {0, 0, 0};
+  int np = 0;
+  if (ensure_partials(2048)) return -1;
+  timer.begin(stream, PASS_UPDATE_MUL);
+  launch_update_mul(psi, t2p.data(), wp.data(), V, N, o, N, (double)lambda, minValue, (float)m, partial, &np, stream);
+  timer.end(stream);
+  timer.begin(stream, PASS_REDUCE);
+  launch_reduce_stats(partial, np, statsDev + statIdx, stream);
+  timer.end(stream);
+  MVR_HIP(hipGetLastError());
+  return 0;
+}
+
+int Decon::run(int iterations) {
+  MVR_CHECK(ready, "init first");
+  MVR_CHECK(iterations >= 0, "negative iteration count");
+  MVR_HIP(hipSetDevice(dev));
+  const int perIt = scheme == SCHEME_SEQ ? V : 1;
+  const int need = iterations * perIt;
+  if (need > statsCap) {
+    if (statsDev) (void)hipFree(statsDev);
+    MVR_HIP(hipMalloc(&statsDev, sizeof(double2) * std::max(1, need)));
+    statsCap = std::max(1, need);
+  }
+  const int rowsPerWG = 4;  // smallest x-pass row group
+  if (ensure_partials((int)std::max<size_t>(2048, plan.rows_inner() / rowsPerWG + 1))) return -1;
+  for (int it = 0; it < iterations; ++it) {
+    if (scheme == SCHEME_SEQ) {
+      for (int v = 0; v < V; ++v)
+        if (view_seq(v, it * V + v)) return -1;
+    } else {
+      if (iteration_mul(it)) return -1;
+    }
+  }
+  std::vector<double2> h(std::max(need, 1));
+  if (need > 0) MVR_HIP(hipMemcpyAsync(h.data(), statsDev, sizeof(double2) * need, hipMemcpyDeviceToHost, stream));
+  MVR_HIP(hipStreamSynchronize(stream));
+  for (int i = 0; i < need; ++i) { stats.push_back(h[i].x); stats.push_back(h[i].y); }
+  timer.collect(nullptr, nullptr);
+  return 0;
+}
+
+void Decon::destroy() {
+  (void)hipSetDevice(dev);
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto p : img) if (p) (void)hipFree(p);
+  for (auto p : w) if (p) (void)hipFree(p);
+  for (auto p : spec1) if (p) (void)hipFree(p);
+  for (auto p : spec2) if (p) (void)hipFree(p);
+  for (void* p : {(void*)psi, (void*)psi0, (void*)S, (void*)Spsi, (void*)T2, (void*)partial, (void*)statsDev})
+    if (p) (void)hipFree(p);
+  img.clear(); w.clear(); spec1.clear(); spec2.clear();
+  psi = psi0 = nullptr; S = Spsi = nullptr; T2 = nullptr; partial = nullptr; statsDev = nullptr;
+  plan.destroy();
+  timer.release();
+  if (stream) (void)hipStreamDestroy(stream);
+  stream = nullptr;
+}
+
+// ============================================================ per-block compute thread
+
+int BlockThread::create(int dev_, const int B_[3], float minValue_, float lambda_, int id_) {
+  dev = dev_;
+  id = id_;
+  for (int d = 0; d < 3; ++d) {
+    MVR_CHECK(B_[d] >= 1, "block size must be positive");
+    B[d] = B_[d];
+  }
+  minValue = minValue_;
+  lambda = lambda_;
+  MVR_HIP(hipSetDevice(dev));
+  MVR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  const size_t nb = (size_t)B[0] * B[1] * B[2];
+  MVR_HIP(hipMalloc(&psiB, nb * 4));
+  MVR_HIP(hipMalloc(&imgB, nb * 4));
+  MVR_HIP(hipMalloc(&wB, nb * 4));
+  MVR_HIP(hipMalloc(&statDev, sizeof(double2)));
+  return 0;
+}
+
+int BlockThread::ensure_partials(int n) {
+  if (n <= partialCap) return 0;
+  if (partial) (void)hipFree(partial);
+  MVR_HIP(hipMalloc(&partial, sizeof(double2) * n));
+  partialCap = n;
+  return 0;
+}
+
+// DeconViewPSF.getKernel{1,2}FFT: spectra at block size, computed once per view
+int BlockThread::set_view_kernels(int v, const float* K1, const float* K2, const int Kd[3]) {
+  MVR_CHECK(v >= 0 && v < 1024, "view index out of range");
+  MVR_HIP(hipSetDevice(dev));
+  if (plan.xf == nullptr) {
+    for (int d = 0; d < 3; ++d) K[d] = Kd[d];
+    if (plan.create(B, K)) return -1;
+    MVR_HIP(hipMalloc(&S, plan.ncpx() * 8));
+  } else {
+    for (int d = 0; d < 3; ++d) MVR_CHECK(K[d] == Kd[d], "all views must share one PSF size");
+  }
+  if ((int)spec1.size() <= v) {
+    spec1.resize(v + 1, nullptr);
+    spec2.resize(v + 1, nullptr);
+  }
+  const size_t nk = (size_t)K[0] * K[1] * K[2];
+  float* dk = nullptr;
+  MVR_HIP(hipMalloc(&dk, nk * 4));
+  for (int which = 0; which < 2; ++which) {
+    float2*& sp = which == 0 ? spec1[v] : spec2[v];
+    if (!sp) MVR_HIP(hipMalloc(&sp, plan.ncpx() * 8));
+    MVR_HIP(hipMemcpyAsync(dk, which == 0 ? K1 : K2, nk * 4, hipMemcpyDefault, stream));
+    if (plan.kernel_spectrum(dk, sp, stream)) return -1;
+  }
+  MVR_HIP(hipStreamSynchronize(stream));
+  (void)hipFree(dk);
+  return 0;
+}
+
+int BlockThread::run_seq(int v, float* psiBlock, const float* imgBlock, const float* wBlock, float maxI,
+                         double stats[2]) {
+  MVR_CHECK(plan.xf != nullptr && v >= 0 && v < (int)spec1.size() && spec1[v], "set_view_kernels first");
+  MVR_HIP(hipSetDevice(dev));
+  const size_t nb = (size_t)B[0] * B[1] * B[2];
+  MVR_HIP(hipMemcpyAsync(psiB, psiBlock, nb * 4, hipMemcpyDefault, stream));
+  MVR_HIP(hipMemcpyAsync(imgB, imgBlock, nb * 4, hipMemcpyDefault, stream));
+  MVR_HIP(hipMemcpyAsync(wB, wBlock, nb * 4, hipMemcpyDefault, stream));
+  const Dom& d = plan.dom;
+  const StridedGeom gy = plan.geom_y(), gz = plan.geom_z();
+  // convolve1: psi block with mirror-single extension of the block itself (ComputeBlockSeqThreadCPU.java:171-189)
+  GatherParams g{};
+  g.src = psiB;
+  for (int k = 0; k < 3; ++k) { g.N[k] = B[k]; g.org[k] = -d.m[k]; }
+  g.mode = GATHER_MIRROR;
+  plan.xf(d, g, S, plan.twM, plan.wr, stream);
+  plan.sy(gy, 0, S, S, nullptr, 1.f, plan.twY, stream);
+  plan.sz(gz, 2, S, S, spec1[v], 1.f, plan.twZ, stream);
+  plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+  RatioParams q{};
+  q.img = imgB;
+  for (int k = 0; k < 3; ++k) { q.N[k] = B[k]; q.o[k] = 0; }
+  plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
+  plan.sy(gy, 0, S, S, nullptr, 1.f, plan.twY, stream);
+  plan.sz(gz, 2, S, S, spec2[v], 1.f, plan.twZ, stream);
+  plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+  if (ensure_partials((int)(plan.rows_inner() / 4 + 1))) return -1;
+  UpdateParams u{};
+  u.psi = psiB;
+  u.w = wB;
+  for (int k = 0; k < 3; ++k) { u.N[k] = B[k]; u.o[k] = 0; }
+  u.lambda = (double)lambda;
+  u.minValue = minValue;
+  u.maxI = maxI;
+  u.partial = partial;
+  const int np = plan.xu(d, u, S, plan.twM, plan.wr, stream);
+  launch_reduce_stats(partial, np, statDev, stream);
+  MVR_HIP(hipGetLastError());
+  double2 h;
+  MVR_HIP(hipMemcpyAsync(psiBlock, psiB, nb * 4, hipMemcpyDefault, stream));
+  MVR_HIP(hipMemcpyAsync(&h, statDev, sizeof(double2), hipMemcpyDeviceToHost, stream));
+  MVR_HIP(hipStreamSynchronize(stream));
+  if (stats) { stats[0] = h.x; stats[1] = h.y; }
+  return 0;
+}
+
+int BlockThread::run_mul(int nv, float* psiBlock, const float* const* imgBlocks, const float* const* wBlocks,
+                         const float* maxIs, double stats[2]) {
+  MVR_CHECK(nv >= 1 && nv <= 32 && nv <= (int)spec1.size(), "set_view_kernels for every view first");
+  for (int v = 0; v < nv; ++v) MVR_CHECK(spec1[v] != nullptr, "set_view_kernels for every view first");
+  MVR_HIP(hipSetDevice(dev));
+  const size_t nb = (size_t)B[0] * B[1] * B[2];
+  while ((int)imgV.size() < nv) {
+    float *a = nullptr, *b = nullptr;
+    MVR_HIP(hipMalloc(&a, nb * 4));
+    MVR_HIP(hipMalloc(&b, nb * 4));
+    imgV.push_back(a);
+    wV.push_back(b);
+  }
+  if (!T2) MVR_HIP(hipMalloc(&T2, nb * 4 * 32));
+  if (!Spsi) MVR_HIP(hipMalloc(&Spsi, plan.ncpx() * 8));
+  MVR_HIP(hipMemcpyAsync(psiB, psiBlock, nb * 4, hipMemcpyDefault, stream));
+  for (int v = 0; v < nv; ++v) {
+    MVR_HIP(hipMemcpyAsync(imgV[v], imgBlocks[v], nb * 4, hipMemcpyDefault, stream));
+    MVR_HIP(hipMemcpyAsync(wV[v], wBlocks[v], nb * 4, hipMemcpyDefault, stream));
+  }
+  const Dom& d = plan.dom;
+  const StridedGeom gy = plan.geom_y(), gz = plan.geom_z();
+  GatherParams g{};
+  g.src = psiB;
+  for (int k = 0; k < 3; ++k) { g.N[k] = B[k]; g.org[k] = -d.m[k]; }
+  g.mode = GATHER_MIRROR;
+  plan.xf(d, g, Spsi, plan.twM, plan.wr, stream);
+  plan.sy(gy, 0, Spsi, Spsi, nullptr, 1.f, plan.twY, stream);
+  std::vector<const float*> t2p(nv), wp(nv);
+  for (int v = 0; v < nv; ++v) {
+    plan.sz(gz, 2, Spsi, S, spec1[v], 1.f, plan.twZ, stream);
+    plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+    RatioParams q{};
+    q.img = imgV[v];
+    for (int k = 0; k < 3; ++k) { q.N[k] = B[k]; q.o[k] = 0; }
+    plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
+    plan.sy(gy, 0, S, S, nullptr, 1.f, plan.twY, stream);
+    plan.sz(gz, 2, S, S, spec2[v], 1.f, plan.twZ, stream);
+    plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+    StoreParams st{T2 + nb * v};
+    plan.xs(d, st, S, plan.twM, plan.wr, stream);
+    t2p[v] = T2 + nb * v;
+    wp[v] = wV[v];
+  }
+  // mean of the per-view maxima, rounded to float (ComputeBlockMulThreadCPU.java:144-149)
+  double m = 0;
+  for (int v = 0; v < nv; ++v) m += maxIs[v];
+  m /= (double)nv;
+  const int o[3] = {0, 0, 0};
+  int np = 0;
+  if (ensure_partials(2048)) return -1;
+  launch_update_mul(psiB, t2p.data(), wp.data(), nv, B, o, B, (double)lambda, minValue, (float)m, partial, &np,
+                    stream);
+  launch_reduce_stats(partial, np, statDev, stream);
+  MVR_HIP(hipGetLastError());
+  double2 h;
+  MVR_HIP(hipMemcpyAsync(psiBlock, psiB, nb * 4, hipMemcpyDefault, stream));
+  MVR_HIP(hipMemcpyAsync(&h, statDev, sizeof(double2), hipMemcpyDeviceToHost, stream));
+  MVR_HIP(hipStreamSynchronize(stream));
+  if (stats) { stats[0] = h.x; stats[1] = h.y; }
+  return 0;
+}
+
+void BlockThread::destroy() {
+  (void)hipSetDevice(dev);
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto p : spec1) if (p) (void)hipFree(p);
+  for (auto p : spec2) if (p) (void)hipFree(p);
+  for (auto p : imgV) if (p) (void)hipFree(p);
+  for (auto p : wV) if (p) (void)hipFree(p);
+  for (void* p : {(void*)psiB, (void*)imgB, (void*)wB, (void*)T2, (void*)S, (void*)Spsi, (void*)partial,
+                  (void*)statDev})
+    if (p) (void)hipFree(p);
+  spec1.clear(); spec2.clear(); imgV.clear(); wV.clear();
+  psiB = imgB = wB = T2 = nullptr; S = Spsi = nullptr; partial = nullptr; statDev = nullptr;
+  plan.destroy();
+  if (stream) (void)hipStreamDestroy(stream);
+  stream = nullptr;
+}
+
+}  // namespace mvr

@@ -1,0 +1,103 @@
+// engine.hpp -- shared declarations of the HIP deconvolution engine.
+//
+// Geometry of one convolution plan ("domain"):
+//   * the output region is D[0] x D[1] x D[2] voxels of a source volume,
+//     starting at volume coordinate o;
+//   * the FFT grid is P[0] x P[1] x P[2] (P[d] >= D[d] + K[d] - 1); FFT
+//     position p corresponds to volume coordinate o + p - m, m[d] = K[d]-1-K[d]/2
+//     (the left reach of the kernel), so the output region is p in [m, m + D);
+//   * the half spectrum is stored [z][y][hx], hx < P[0]/2 + 1, with row pitch HxP.
+// A linear convolution evaluated this way is exact on the output region for
+// any extension of the input outside it (util/FFTConvolution.java:508-543 pads
+// to at least N + K - 1 for the same reason).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace mvr {
+
+struct Dom {
+  int P[3];
+  int HxP;
+  int D[3];
+  int m[3];
+};
+
+enum GatherMode : int { GATHER_MIRROR = 0, GATHER_VALUE = 1, GATHER_KERNEL = 2 };
+
+// Source of the forward transform: value at FFT position p is
+//   MIRROR: src[mirror(org + p)]                (extendMirrorSingle)
+//   VALUE : src[org + p] inside the volume, `value` outside (extendValue / extendZero)
+//   KERNEL: K[(p + K/2) mod P] if < K, else 0    (kernel centre moved to the origin)
+struct GatherParams {
+  const float* src;
+  int N[3];
+  int org[3];
+  int mode;
+  float value;
+};
+
+// Inverse-transform epilogues.  `o` is the volume coordinate of the output
+// region's first voxel; N are the dims of the volume that img/psi/w index.
+struct RatioParams {
+  const float* img;
+  int N[3];
+  int o[3];
+};
+
+struct UpdateParams {
+  float* psi;
+  const float* w;
+  int N[3];
+  int o[3];
+  double lambda;
+  float minValue;
+  float maxI;
+  double2* partial;  // per work-group {sum change, max change}
+};
+
+struct StoreParams {
+  float* out;        // compact D[0]*D[1]*D[2] array
+};
+
+typedef void (*XFwdFn)(const Dom&, const GatherParams&, float2* S, const float2* twM, const float2* wr,
+                       hipStream_t s);
+typedef void (*XRatioFn)(const Dom&, const RatioParams&, const float2* Sin, float2* Sout, const float2* twM,
+                         const float2* wr, hipStream_t s);
+typedef int (*XUpdateFn)(const Dom&, const UpdateParams&, const float2* S, const float2* twM, const float2* wr,
+                         hipStream_t s);  // returns #work-groups (= #partials)
+typedef void (*XStoreFn)(const Dom&, const StoreParams&, const float2* S, const float2* twM, const float2* wr,
+                         hipStream_t s);
+// strided passes: y (plane = z) or z (plane = y)
+struct StridedGeom {
+  size_t planeStride;  // complex elements between consecutive planes
+  size_t lineStride;   // complex elements between consecutive pencil elements
+  int ncols;           // valid columns (hx) per line
+  int nplanes;
+};
+// mode: 0 forward, 1 inverse, 2 forward * spec * inverse (spec layout = data layout),
+// 3 forward then scale
+typedef void (*StridedFn)(const StridedGeom&, int mode, const float2* in, float2* out, const float2* spec,
+                          float scale, const float2* tw, hipStream_t s);
+
+XFwdFn get_x_fwd(int M);
+XRatioFn get_x_ratio(int M);
+XUpdateFn get_x_update(int M);
+XStoreFn get_x_store(int M);
+StridedFn get_strided(int L);
+// lengths with compiled kernels, ascending; terminated by 0
+const int* strided_lengths();
+const int* half_lengths();
+
+// misc kernels (misc.hip)
+void launch_fill(float* dst, size_t n, float v, hipStream_t s);
+void launch_reduce_stats(const double2* partial, int n, double2* out, hipStream_t s);
+int launch_psi_init_partials(const float* const* imgs, int V, size_t n, double* psum, unsigned long long* pcount,
+                             float* pmax, hipStream_t s);  // returns #partials
+void launch_update_mul(float* psi, const float* const* T2, const float* const* w, int V, const int N[3],
+                       const int o[3], const int D[3], double lambda, float minValue, float maxI, double2* partial,
+                       int* nPartialOut, hipStream_t s);
+void launch_scale_copy_kernel_spectrum(const float2* in, float2* out, size_t n, float scale, hipStream_t s);
+
+}  // namespace mvr

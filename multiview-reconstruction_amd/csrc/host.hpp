@@ -1,0 +1,157 @@
+// host.hpp -- host-side objects of the HIP deconvolution library.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace mvr {
+
+// thread-local last error (read through mvr_last_error)
+void set_error(const std::string& msg);
+const char* last_error();
+
+#define MVR_HIP(call)                                                                                 \
+  do {                                                                                                \
+    hipError_t e_ = (call);                                                                           \
+    if (e_ != hipSuccess) {                                                                           \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));                                   \
+      return -1;                                                                                      \
+    }                                                                                                 \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// One FFT-convolution plan: output region D, kernel size K (see engine.hpp).
+// ---------------------------------------------------------------------------
+struct Plan {
+  Dom dom{};
+  int K[3] = {0, 0, 0};
+  int M = 0, Hx = 0;
+  float2* twM = nullptr;  // length-M twiddles (x)
+  float2* wr = nullptr;   // exp(-2 pi i k / P0), k = 0..M
+  float2* twY = nullptr;
+  float2* twZ = nullptr;
+  XFwdFn xf = nullptr;
+  XRatioFn xr = nullptr;
+  XUpdateFn xu = nullptr;
+  XStoreFn xs = nullptr;
+  StridedFn sy = nullptr, sz = nullptr;
+
+  // chooses the FFT lengths; P[d] >= D[d] + K[d] - 1 (or the given minimum per axis)
+  int create(const int D[3], const int K[3], const int* Pmin = nullptr);
+  void destroy();
+  size_t ncpx() const { return (size_t)dom.HxP * dom.P[1] * dom.P[2]; }
+  size_t rows_inner() const { return (size_t)dom.D[1] * dom.D[2]; }
+  StridedGeom geom_y() const;
+  StridedGeom geom_z() const;
+  float inv_scale() const { return 2.0f / ((float)dom.P[0] * (float)dom.P[1] * (float)dom.P[2]); }
+
+  // spectrum of a device kernel (K layout), scaled by the inverse-FFT factor
+  int kernel_spectrum(const float* dKernel, float2* spec, hipStream_t s) const;
+  // out (compact D) = K (*) ext(src) with src a volume of dims N; org = volume coord of FFT position 0
+  int convolve(const GatherParams& g, const float2* spec, float2* work, float* out, hipStream_t s) const;
+};
+
+// DeconViewPSF.init for all views in list order (D/DeconViewPSF.java:119-254,
+// D/DeconViews.java:69-70).  k1 are normalised in place; k2 receives the
+// second kernel per view.  Compound kernels are convolved on the GPU.
+int init_psfs(int dev, std::vector<std::vector<float>>& k1, const int K[3], int psfType, int nT,
+              std::vector<std::vector<float>>& k2, hipStream_t s);
+double sum_img_quirk(const float* a, size_t n, int nT);
+void norm_to_sum1(float* a, size_t n, int nT);
+void mirror_all(float* a, const int dim[3]);
+
+enum { PSF_OPT_II = 0, PSF_OPT_I = 1, PSF_EFFICIENT_BAYESIAN = 2, PSF_INDEPENDENT = 3 };
+enum { SCHEME_SEQ = 0, SCHEME_MUL = 1 };
+
+// per-pass timing categories (resident driver)
+enum Pass { PASS_X_FWD = 0, PASS_Y_FWD, PASS_Z_CONV, PASS_Y_INV, PASS_X_RATIO, PASS_X_UPDATE, PASS_X_STORE,
+            PASS_UPDATE_MUL, PASS_REDUCE, PASS_COUNT };
+
+struct Timer {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> cat;
+  size_t used = 0;
+  void begin(hipStream_t s, int c);
+  void end(hipStream_t s);
+  int collect(double* ms, long long* n);  // accumulates into arrays of PASS_COUNT
+  void release();
+  double ms[PASS_COUNT] = {0};
+  long long cnt[PASS_COUNT] = {0};
+};
+
+// ---------------------------------------------------------------------------
+// Resident multiview deconvolution (MultiViewDeconvolution{Seq,Mul} over one
+// domain covering the whole psi volume; D/MultiViewDeconvolution.java:90-212).
+// ---------------------------------------------------------------------------
+struct Decon {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int N[3] = {0, 0, 0};
+  int V = 0, psfType = 0, scheme = 0, nT = 8;
+  float lambda = 0.f, minValue = 1e-4f;
+  int K[3] = {0, 0, 0};
+  std::vector<float*> img, w;
+  std::vector<std::vector<float>> kraw, k1, k2;
+  float* psi = nullptr;
+  float* psi0 = nullptr;
+  Plan plan;
+  std::vector<float2*> spec1, spec2;
+  float2* S = nullptr;
+  float2* Spsi = nullptr;
+  float* T2 = nullptr;
+  double2* partial = nullptr;
+  int partialCap = 0;
+  double2* statsDev = nullptr;
+  int statsCap = 0;
+  std::vector<double> stats;
+  std::vector<float> maxI;
+  double avg = 0;
+  bool ready = false;
+  Timer timer;
+
+  int create(int dev, const int N[3], int V, int psfType, int scheme, float lambda, float minValue, int nT);
+  int set_view(int v, const float* img, const float* w, const float* kernel, const int K[3]);
+  int init(int psiInit, const float* psiIn);
+  int run(int iterations);
+  int reset_psi();
+  void destroy();
+
+  size_t nvox() const { return (size_t)N[0] * N[1] * N[2]; }
+  int ensure_partials(int n);
+  int view_seq(int v, int statIdx);
+  int iteration_mul(int statIdx);
+};
+
+// ---------------------------------------------------------------------------
+// Per-block compute thread (ComputeBlockSeqThread / ComputeBlockMulThread SPI,
+// D/iteration/sequential/ComputeBlockSeqThreadCPU.java:53-169,
+// D/iteration/mul/ComputeBlockMulThreadCPU.java:87-188).
+// ---------------------------------------------------------------------------
+struct BlockThread {
+  int dev = 0, id = 0;
+  hipStream_t stream = nullptr;
+  int B[3] = {0, 0, 0};
+  float lambda = 0.f, minValue = 1e-4f;
+  int K[3] = {0, 0, 0};
+  Plan plan;
+  std::vector<float2*> spec1, spec2;
+  float *psiB = nullptr, *imgB = nullptr, *wB = nullptr, *T2 = nullptr;
+  std::vector<float*> imgV, wV;
+  float2* S = nullptr;
+  float2* Spsi = nullptr;
+  double2* partial = nullptr;
+  int partialCap = 0;
+  double2* statDev = nullptr;
+
+  int create(int dev, const int B[3], float minValue, float lambda, int id);
+  int set_view_kernels(int v, const float* K1, const float* K2, const int K[3]);
+  int run_seq(int v, float* psiBlock, const float* imgBlock, const float* wBlock, float maxI, double stats[2]);
+  int run_mul(int nv, float* psiBlock, const float* const* imgBlocks, const float* const* wBlocks,
+              const float* maxIs, double stats[2]);
+  void destroy();
+  int ensure_partials(int n);
+};
+
+}  // namespace mvr

@@ -1,0 +1,169 @@
+// misc.hip -- pointwise and reduction kernels of the deconvolution engine.
+#include "passes.hpp"
+
+namespace mvr {
+
+__global__ void k_fill(float* __restrict__ dst, size_t n, float v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = v;
+}
+
+void launch_fill(float* dst, size_t n, float v, hipStream_t s) {
+  const int nb = (int)std::min<size_t>((n + kThreads - 1) / kThreads, 4096);
+  hipLaunchKernelGGL(k_fill, dim3(nb > 0 ? nb : 1), dim3(kThreads), 0, s, dst, n, v);
+}
+
+// Deterministic final reduction of per-work-group {sum, max} partials
+// (IterationStatistics accumulation, ComputeBlockSeqThreadCPU.java:159-166).
+__global__ void k_reduce_stats(const double2* __restrict__ partial, int n, double2* __restrict__ out) {
+  __shared__ double ss[kThreads], sm[kThreads];
+  double s = 0.0, m = -1.0;
+  for (int i = threadIdx.x; i < n; i += kThreads) { s += partial[i].x; m = fmax(m, partial[i].y); }
+  ss[threadIdx.x] = s;
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = kThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      ss[threadIdx.x] += ss[threadIdx.x + off];
+      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = make_double2(ss[0], sm[0]);
+}
+
+void launch_reduce_stats(const double2* partial, int n, double2* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_stats, dim3(1), dim3(kThreads), 0, s, partial, n, out);
+}
+
+// PsiInitAvgPrecise per voxel (D/init/PsiInitAvgPreciseThread.java:127-155): the
+// mean of the positive view values is summed (double) over voxels with any
+// positive value; max of the positive values per view.  Per-work-group
+// partials are reduced on the host in a fixed order.
+constexpr int kMaxViews = 32;
+struct ViewPtrs { const float* p[kMaxViews]; };
+
+__global__ __launch_bounds__(kThreads) void k_psi_init(ViewPtrs imgs, int V, size_t n, double* __restrict__ psum,
+                                                       unsigned long long* __restrict__ pcount,
+                                                       float* __restrict__ pmax) {
+  __shared__ double ssum[kThreads / 64];
+  __shared__ unsigned long long scnt[kThreads / 64];
+  __shared__ float smax[kThreads / 64][kMaxViews];
+  double sum = 0.0;
+  unsigned long long cnt = 0;
+  float mx[kMaxViews];
+  for (int v = 0; v < V; ++v) mx[v] = 0.f;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+    double s = 0.0;
+    int c = 0;
+    for (int v = 0; v < V; ++v) {
+      const float val = imgs.p[v][i];
+      if (val > 0.f) { mx[v] = fmaxf(mx[v], val); s += (double)val; ++c; }
+    }
+    if (c > 0) { sum += s / (double)c; ++cnt; }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    sum += __shfl_down(sum, off);
+    cnt += __shfl_down(cnt, off);
+    for (int v = 0; v < V; ++v) mx[v] = fmaxf(mx[v], __shfl_down(mx[v], off));
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    ssum[wv] = sum;
+    scnt[wv] = cnt;
+    for (int v = 0; v < V; ++v) smax[wv][v] = mx[v];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0;
+    unsigned long long b = 0;
+    for (int i = 0; i < kThreads / 64; ++i) { a += ssum[i]; b += scnt[i]; }
+    psum[blockIdx.x] = a;
+    pcount[blockIdx.x] = b;
+    for (int v = 0; v < V; ++v) {
+      float m = 0.f;
+      for (int i = 0; i < kThreads / 64; ++i) m = fmaxf(m, smax[i][v]);
+      pmax[(size_t)blockIdx.x * V + v] = m;
+    }
+  }
+}
+
+int launch_psi_init_partials(const float* const* imgs, int V, size_t n, double* psum, unsigned long long* pcount,
+                             float* pmax, hipStream_t s) {
+  ViewPtrs vp{};
+  for (int v = 0; v < V && v < kMaxViews; ++v) vp.p[v] = imgs[v];
+  const int nb = (int)std::min<size_t>((n + kThreads - 1) / kThreads, 1024);
+  hipLaunchKernelGGL(k_psi_init, dim3(nb), dim3(kThreads), 0, s, vp, V, n, psum, pcount, pmax);
+  return nb;
+}
+
+// Mul-scheme update (DeconvolutionMethods.computeNextValueMul,
+// D/iteration/sequential/DeconvolutionMethods.java:370-419): geometric mean of the
+// per-view integrals (double product, pow 1/V), min(1, sum of weights).
+struct ViewPtrsW { const float* t2[kMaxViews]; const float* w[kMaxViews]; };
+
+__global__ __launch_bounds__(kThreads) void k_update_mul(float* __restrict__ psi, ViewPtrsW vp, int V, int3 N,
+                                                         int3 o, int3 D, double lambda, float minI, float maxI,
+                                                         double2* __restrict__ partial) {
+  const size_t nvox = (size_t)D.x * D.y * D.z;
+  double sumc = 0.0, mxc = -1.0;
+  const double invV = 1.0 / (double)V;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < nvox; i += (size_t)gridDim.x * kThreads) {
+    const int lx = (int)(i % D.x);
+    const size_t r = i / D.x;
+    const int ly = (int)(r % D.y), lz = (int)(r / D.y);
+    const size_t gi = ((size_t)(o.z + lz) * N.y + (o.y + ly)) * N.x + (o.x + lx);
+    double sumW = 0.0, prod = 1.0;
+    for (int v = 0; v < V; ++v) {
+      prod = __dmul_rn(prod, (double)vp.t2[v][i]);
+      sumW = __dadd_rn(sumW, (double)vp.w[v][gi]);
+    }
+    prod = pow(prod, invV);
+    sumW = fmin(1.0, sumW);
+    const float last = psi[gi];
+    const float value = __fmul_rn(last, (float)prod);
+    float adjusted;
+    if (value > 0.f) {
+      if (lambda > 0.0) {
+        const double vn = (double)__fdiv_rn(value, maxI);
+        const double t =
+            __ddiv_rn(__dadd_rn(__dsqrt_rn(__dadd_rn(1.0, __dmul_rn(__dmul_rn(2.0, lambda), vn))), -1.0), lambda);
+        adjusted = __fmul_rn((float)t, maxI);
+      } else {
+        adjusted = value;
+      }
+    } else {
+      adjusted = minI;
+    }
+    const float next = isnan(adjusted) ? minI : fmaxf(minI, adjusted);
+    const float res = __fadd_rn(last, __fmul_rn(__fsub_rn(next, last), (float)sumW));
+    psi[gi] = res;
+    const float ch = __fsub_rn(res, last);
+    sumc += (double)ch;
+    mxc = fmax(mxc, (double)ch);
+  }
+  block_reduce_stats(sumc, mxc, partial + blockIdx.x);
+}
+
+void launch_update_mul(float* psi, const float* const* T2, const float* const* w, int V, const int N[3],
+                       const int o[3], const int D[3], double lambda, float minValue, float maxI, double2* partial,
+                       int* nPartialOut, hipStream_t s) {
+  ViewPtrsW vp{};
+  for (int v = 0; v < V && v < kMaxViews; ++v) { vp.t2[v] = T2[v]; vp.w[v] = w[v]; }
+  const size_t nvox = (size_t)D[0] * D[1] * D[2];
+  const int nb = (int)std::min<size_t>((nvox + kThreads - 1) / kThreads, 2048);
+  hipLaunchKernelGGL(k_update_mul, dim3(nb), dim3(kThreads), 0, s, psi, vp, V, make_int3(N[0], N[1], N[2]),
+                     make_int3(o[0], o[1], o[2]), make_int3(D[0], D[1], D[2]), lambda, minValue, maxI, partial);
+  *nPartialOut = nb;
+}
+
+__global__ void k_scale_copy(const float2* __restrict__ in, float2* __restrict__ out, size_t n, float scale) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = cscale(in[i], scale);
+}
+
+void launch_scale_copy_kernel_spectrum(const float2* in, float2* out, size_t n, float scale, hipStream_t s) {
+  const int nb = (int)std::min<size_t>((n + kThreads - 1) / kThreads, 4096);
+  hipLaunchKernelGGL(k_scale_copy, dim3(nb > 0 ? nb : 1), dim3(kThreads), 0, s, in, out, n, scale);
+}
+
+}  // namespace mvr

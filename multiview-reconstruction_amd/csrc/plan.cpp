@@ -1,0 +1,153 @@
+// plan.cpp -- FFT-convolution plans, error state and pass timers.
+#include <cmath>
+#include <vector>
+
+#include "host.hpp"
+
+namespace mvr {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+static int pick_length(int need, bool half) {
+  for (const int* p = strided_lengths(); *p; ++p) {
+    if (*p < need) continue;
+    if (half) {
+      bool ok = false;
+      for (const int* h = half_lengths(); *h; ++h) ok |= (*h * 2 == *p);
+      if (!ok) continue;
+    }
+    return *p;
+  }
+  return -1;
+}
+
+static int upload_twiddles(float2** dst, int n, int period) {
+  std::vector<float2> h(n);
+  for (int k = 0; k < n; ++k) {
+    const double a = -2.0 * M_PI * (double)k / (double)period;
+    h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  MVR_HIP(hipMalloc(dst, sizeof(float2) * n));
+  MVR_HIP(hipMemcpy(*dst, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int Plan::create(const int D[3], const int Kd[3], const int* Pmin) {
+  for (int d = 0; d < 3; ++d) {
+    if (D[d] < 1 || Kd[d] < 1) { set_error("plan: empty domain or kernel"); return -1; }
+    K[d] = Kd[d];
+    dom.D[d] = D[d];
+    dom.m[d] = K[d] - 1 - K[d] / 2;
+    int need = D[d] + K[d] - 1;
+    if (Pmin && Pmin[d] > need) need = Pmin[d];
+    const int P = pick_length(need, d == 0);
+    if (P < 0) { set_error("plan: no compiled FFT length >= " + std::to_string(need)); return -1; }
+    dom.P[d] = P;
+  }
+  M = dom.P[0] / 2;
+  Hx = M + 1;
+  dom.HxP = (Hx + 15) / 16 * 16;
+  xf = get_x_fwd(M);
+  xr = get_x_ratio(M);
+  xu = get_x_update(M);
+  xs = get_x_store(M);
+  sy = get_strided(dom.P[1]);
+  sz = get_strided(dom.P[2]);
+  if (!xf || !xr || !xu || !xs || !sy || !sz) { set_error("plan: missing kernel instantiation"); return -1; }
+  if (upload_twiddles(&twM, M, M) || upload_twiddles(&wr, M + 1, dom.P[0]) || upload_twiddles(&twY, dom.P[1], dom.P[1]) ||
+      upload_twiddles(&twZ, dom.P[2], dom.P[2]))
+    return -1;
+  return 0;
+}
+
+void Plan::destroy() {
+  for (float2** p : {&twM, &wr, &twY, &twZ})
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+}
+
+StridedGeom Plan::geom_y() const {
+  StridedGeom g;
+  g.planeStride = (size_t)dom.P[1] * dom.HxP;
+  g.lineStride = dom.HxP;
+  g.ncols = Hx;
+  g.nplanes = dom.P[2];
+  return g;
+}
+
+StridedGeom Plan::geom_z() const {
+  StridedGeom g;
+  g.planeStride = dom.HxP;
+  g.lineStride = (size_t)dom.P[1] * dom.HxP;
+  g.ncols = Hx;
+  g.nplanes = dom.P[1];
+  return g;
+}
+
+int Plan::kernel_spectrum(const float* dKernel, float2* spec, hipStream_t s) const {
+  GatherParams g{};
+  g.src = dKernel;
+  for (int d = 0; d < 3; ++d) { g.N[d] = K[d]; g.org[d] = 0; }
+  g.mode = GATHER_KERNEL;
+  xf(dom, g, spec, twM, wr, s);
+  sy(geom_y(), 0, spec, spec, nullptr, 1.f, twY, s);
+  sz(geom_z(), 3, spec, spec, nullptr, inv_scale(), twZ, s);
+  MVR_HIP(hipGetLastError());
+  return 0;
+}
+
+int Plan::convolve(const GatherParams& g, const float2* spec, float2* work, float* out, hipStream_t s) const {
+  xf(dom, g, work, twM, wr, s);
+  sy(geom_y(), 0, work, work, nullptr, 1.f, twY, s);
+  sz(geom_z(), 2, work, work, spec, 1.f, twZ, s);
+  sy(geom_y(), 1, work, work, nullptr, 1.f, twY, s);
+  StoreParams st{out};
+  xs(dom, st, work, twM, wr, s);
+  MVR_HIP(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------------------------ timers
+void Timer::begin(hipStream_t s, int c) {
+  if (!on) return;
+  if (used + 2 > ev.size()) {
+    size_t old = ev.size();
+    ev.resize(old + 256);
+    for (size_t i = old; i < ev.size(); ++i) (void)hipEventCreate(&ev[i]);
+  }
+  (void)hipEventRecord(ev[used], s);
+  cat.push_back(c);
+  used++;
+}
+void Timer::end(hipStream_t s) {
+  if (!on) return;
+  (void)hipEventRecord(ev[used], s);
+  used++;
+}
+int Timer::collect(double* outMs, long long* outN) {
+  if (on) {
+    (void)hipEventSynchronize(ev[used ? used - 1 : 0]);
+    for (size_t i = 0; i + 1 < used; i += 2) {
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+      ms[cat[i / 2]] += t;
+      cnt[cat[i / 2]] += 1;
+    }
+    used = 0;
+    cat.clear();
+  }
+  for (int c = 0; c < PASS_COUNT; ++c) {
+    if (outMs) outMs[c] = ms[c];
+    if (outN) outN[c] = cnt[c];
+  }
+  return 0;
+}
+void Timer::release() {
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  ev.clear();
+  cat.clear();
+  used = 0;
+}
+
+}  // namespace mvr

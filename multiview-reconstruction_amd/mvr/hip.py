@@ -1,0 +1,271 @@
+"""ctypes binding of libmvr_hip.so (the C ABI declared in include/mvr_hip.h).
+
+The library is built in-tree by ``multiview-reconstruction_amd/build.py``.  Loading
+fails loudly when it is missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "lib", "libmvr_hip.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "mvr_hip.h")
+
+PSF_TYPES = {"OPTIMIZATION_II": 0, "OPTIMIZATION_I": 1, "EFFICIENT_BAYESIAN": 2, "INDEPENDENT": 3}
+SCHEMES = {"SEQ": 0, "MUL": 1}
+PASSES = ["x_fwd", "y_fwd", "z_conv", "y_inv", "x_ratio", "x_update", "x_store", "update_mul", "reduce"]
+
+_lib = None
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int)
+
+
+class MvrError(RuntimeError):
+    pass
+
+
+def header_functions():
+    """Names of all functions declared in include/mvr_hip.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(mvr_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MvrError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(the HIP library has no fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.mvr_last_error.restype = ctypes.c_char_p
+        L.mvr_get_num_devices.restype = ctypes.c_int
+        L.mvr_get_device_name.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.mvr_get_mem_device.argtypes = [ctypes.c_int]
+        L.mvr_get_mem_device.restype = ctypes.c_longlong
+        L.mvr_get_free_mem_device.argtypes = [ctypes.c_int]
+        L.mvr_get_free_mem_device.restype = ctypes.c_longlong
+        L.mvr_get_arch_major.argtypes = [ctypes.c_int]
+        L.mvr_get_arch_minor.argtypes = [ctypes.c_int]
+        L.mvr_convolution3d_fft_inplace.argtypes = [vp, _i32p, vp, _i32p, ctypes.c_int]
+        L.mvr_fft_convolve.argtypes = [ctypes.c_int, vp, _i32p, ctypes.c_int, ctypes.c_float, vp, _i32p, vp]
+        L.mvr_init_psfs.argtypes = [ctypes.c_int, vp, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, vp, vp]
+        L.mvr_block_thread_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _i32p, ctypes.c_float,
+                                              ctypes.c_float, ctypes.c_int]
+        L.mvr_block_thread_set_view_kernels.argtypes = [vp, ctypes.c_int, vp, vp, _i32p]
+        L.mvr_block_run_iteration_seq.argtypes = [vp, ctypes.c_int, vp, vp, vp, ctypes.c_float, _f64p]
+        L.mvr_block_run_iteration_mul.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                                  _f32p, _f64p]
+        L.mvr_block_thread_destroy.argtypes = [vp]
+        L.mvr_block_thread_destroy.restype = None
+        L.mvr_decon_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int]
+        L.mvr_decon_set_view.argtypes = [vp, ctypes.c_int, vp, vp, vp, _i32p]
+        L.mvr_decon_init.argtypes = [vp, ctypes.c_int, vp]
+        L.mvr_decon_run.argtypes = [vp, ctypes.c_int]
+        L.mvr_decon_get_psi.argtypes = [vp, vp]
+        L.mvr_decon_get_max.argtypes = [vp, _f32p, _f64p]
+        L.mvr_decon_get_kernels.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.mvr_decon_get_stats.argtypes = [vp, _f64p, ctypes.c_int]
+        L.mvr_decon_reset_psi.argtypes = [vp]
+        L.mvr_decon_get_geometry.argtypes = [vp, _i32p]
+        L.mvr_decon_profile.argtypes = [vp, ctypes.c_int]
+        L.mvr_decon_pass_times.argtypes = [vp, _f64p, ctypes.POINTER(ctypes.c_longlong)]
+        L.mvr_decon_destroy.argtypes = [vp]
+        L.mvr_decon_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise MvrError(lib().mvr_last_error().decode())
+    return rc
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _dims(shape_zyx):
+    return (ctypes.c_int * 3)(shape_zyx[2], shape_zyx[1], shape_zyx[0])
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def num_devices() -> int:
+    return lib().mvr_get_num_devices()
+
+
+def device_name(dev=0) -> str:
+    buf = ctypes.create_string_buffer(256)
+    _check(lib().mvr_get_device_name(dev, buf, 256))
+    return buf.value.decode()
+
+
+def fft_convolve(img, kernel, ext="mirror", value=0.0, dev=0):
+    img = _f32(img)
+    kernel = _f32(kernel)
+    out = np.empty_like(img)
+    e = 0 if ext == "mirror" else 1
+    _check(lib().mvr_fft_convolve(dev, _ptr(img), _dims(img.shape), e, value, _ptr(kernel), _dims(kernel.shape),
+                                  _ptr(out)))
+    return out
+
+
+def convolution3d_fft_inplace(img, kernel, dev=0):
+    """Same call shape as CUDAFourierConvolution.convolution3DfftCUDAInPlace (dims as z,y,x)."""
+    img = _f32(img).copy()
+    kernel = _f32(kernel)
+    imd = (ctypes.c_int * 3)(*img.shape)
+    kd = (ctypes.c_int * 3)(*kernel.shape)
+    _check(lib().mvr_convolution3d_fft_inplace(_ptr(img), imd, _ptr(kernel), kd, dev))
+    return img
+
+
+def init_psfs(kernels, psf_type, nT=8, dev=0):
+    """DeconViewPSF.init for all views: returns (kernel1 normalised, kernel2), each [V, kz, ky, kx]."""
+    k = _f32(kernels)
+    V = k.shape[0]
+    k1 = np.empty_like(k)
+    k2 = np.empty_like(k)
+    _check(lib().mvr_init_psfs(dev, _ptr(k), V, _dims(k.shape[1:]), PSF_TYPES[psf_type], nT, _ptr(k1), _ptr(k2)))
+    return k1, k2
+
+
+class BlockThread:
+    """ComputeBlockSeqThread / ComputeBlockMulThread running on one GPU."""
+
+    def __init__(self, block_size_xyz, min_value=1e-4, lam=0.0, thread_id=0, dev=0):
+        h = ctypes.c_void_p()
+        _check(lib().mvr_block_thread_create(ctypes.byref(h), dev, (ctypes.c_int * 3)(*block_size_xyz), min_value,
+                                             lam, thread_id))
+        self.h = h
+        self.block_size = tuple(block_size_xyz)
+
+    def set_view_kernels(self, view, k1, k2):
+        k1, k2 = _f32(k1), _f32(k2)
+        _check(lib().mvr_block_thread_set_view_kernels(self.h, view, _ptr(k1), _ptr(k2), _dims(k1.shape)))
+
+    def run_iteration(self, view, psi_block, img_block, weight_block, max_intensity):
+        psi = _f32(psi_block).copy()
+        st = (ctypes.c_double * 2)()
+        _check(lib().mvr_block_run_iteration_seq(self.h, view, _ptr(psi), _ptr(_f32(img_block)),
+                                                 _ptr(_f32(weight_block)), float(max_intensity), st))
+        return psi, (st[0], st[1])
+
+    def run_iteration_mul(self, psi_block, img_blocks, weight_blocks, max_intensities):
+        psi = _f32(psi_block).copy()
+        imgs = [_f32(a) for a in img_blocks]
+        ws = [_f32(a) for a in weight_blocks]
+        n = len(imgs)
+        ip = (ctypes.c_void_p * n)(*[a.ctypes.data for a in imgs])
+        wp = (ctypes.c_void_p * n)(*[a.ctypes.data for a in ws])
+        mx = np.ascontiguousarray(max_intensities, np.float32)
+        st = (ctypes.c_double * 2)()
+        _check(lib().mvr_block_run_iteration_mul(self.h, n, _ptr(psi), ip, wp, mx.ctypes.data_as(_f32p), st))
+        return psi, (st[0], st[1])
+
+    def close(self):
+        if self.h:
+            lib().mvr_block_thread_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Decon:
+    """Resident MultiViewDeconvolution{Seq,Mul} (psi, views and spectra in HBM)."""
+
+    def __init__(self, shape_zyx, n_views, psf_type="EFFICIENT_BAYESIAN", scheme="SEQ", lam=0.0, min_value=1e-4,
+                 nT=8, dev=0):
+        h = ctypes.c_void_p()
+        _check(lib().mvr_decon_create(ctypes.byref(h), dev, _dims(shape_zyx), n_views, PSF_TYPES[psf_type],
+                                      SCHEMES[scheme], lam, min_value, nT))
+        self.h = h
+        self.shape = tuple(shape_zyx)
+        self.V = n_views
+        self.scheme = scheme
+
+    def set_view(self, view, img, weight, kernel):
+        img, weight, kernel = _f32(img), _f32(weight), _f32(kernel)
+        assert img.shape == self.shape and weight.shape == self.shape
+        _check(lib().mvr_decon_set_view(self.h, view, _ptr(img), _ptr(weight), _ptr(kernel), _dims(kernel.shape)))
+
+    def set_view_device(self, view, img_ptr, weight_ptr, kernel, ):
+        kernel = _f32(kernel)
+        _check(lib().mvr_decon_set_view(self.h, view, ctypes.c_void_p(img_ptr), ctypes.c_void_p(weight_ptr),
+                                        _ptr(kernel), _dims(kernel.shape)))
+
+    def init(self, psi0=None):
+        if psi0 is None:
+            _check(lib().mvr_decon_init(self.h, 0, None))
+        else:
+            p = _f32(psi0)
+            _check(lib().mvr_decon_init(self.h, 1, _ptr(p)))
+
+    def run(self, iterations):
+        _check(lib().mvr_decon_run(self.h, iterations))
+
+    def psi(self):
+        out = np.empty(self.shape, np.float32)
+        _check(lib().mvr_decon_get_psi(self.h, _ptr(out)))
+        return out
+
+    def max_avg(self):
+        mx = np.zeros(self.V, np.float32)
+        avg = ctypes.c_double()
+        _check(lib().mvr_decon_get_max(self.h, mx.ctypes.data_as(_f32p), ctypes.byref(avg)))
+        return mx, avg.value
+
+    def kernels(self, view, kshape_zyx):
+        k1 = np.zeros(kshape_zyx, np.float32)
+        k2 = np.zeros(kshape_zyx, np.float32)
+        _check(lib().mvr_decon_get_kernels(self.h, view, _ptr(k1), _ptr(k2)))
+        return k1, k2
+
+    def stats(self):
+        n = _check(lib().mvr_decon_get_stats(self.h, None, 0))
+        buf = np.zeros(max(n, 1), np.float64)
+        lib().mvr_decon_get_stats(self.h, buf.ctypes.data_as(_f64p), n)
+        per = self.V if self.scheme == "SEQ" else 1
+        return buf[:n].reshape(-1, per, 2) if n else np.zeros((0, per, 2))
+
+    def reset_psi(self):
+        _check(lib().mvr_decon_reset_psi(self.h))
+
+    def geometry(self):
+        g = (ctypes.c_int * 4)()
+        _check(lib().mvr_decon_get_geometry(self.h, g))
+        return dict(P=(g[0], g[1], g[2]), row_pitch=g[3])
+
+    def profile(self, on=True):
+        _check(lib().mvr_decon_profile(self.h, 1 if on else 0))
+
+    def pass_times(self):
+        ms = (ctypes.c_double * 9)()
+        n = (ctypes.c_longlong * 9)()
+        _check(lib().mvr_decon_pass_times(self.h, ms, n))
+        return {PASSES[i]: (ms[i], n[i]) for i in range(9)}
+
+    def close(self):
+        if self.h:
+            lib().mvr_decon_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

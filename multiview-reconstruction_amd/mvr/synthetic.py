@@ -156,7 +156,8 @@ def normalize_weights(ws):
     out = np.empty_like(ws)
     for v in range(ws.shape[0]):
         mine = clipped[v].astype(np.float32)
-        val = np.where(sumw > 1, (mine.astype(np.float64) / sumw).astype(np.float32), mine).astype(np.float64)
+        hard = np.divide(mine.astype(np.float64), sumw, out=np.zeros_like(sumw), where=sumw > 1).astype(np.float32)
+        val = np.where(sumw > 1, hard, mine).astype(np.float64)
         out[v] = np.minimum(1.0, val * 1.0).astype(np.float32)
     return out
 

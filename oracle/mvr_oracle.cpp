@@ -1052,6 +1052,23 @@ int oracle_block_seq_f32(float* psiBlk, const int* bd, const float* img, const f
   return 0;
 }
 
+// One ComputeBlockMulThreadCPU.runIteration on a block over V views (kernels already final).
+int oracle_block_mul_f32(float* psiBlk, const int* bd, int V, const float* imgs, const float* ws, const float* k1,
+                         const float* k2, const int* kd, float lambda, float minValue, const float* maxI, int nT,
+                         double* stats) {
+  const i64 nb = (i64)bd[0] * bd[1] * bd[2], nk = (i64)kd[0] * kd[1] * kd[2];
+  std::vector<std::vector<float>> ib(V), wb(V);
+  std::vector<KernelFFT<float>> f1(V), f2(V);
+  for (int v = 0; v < V; ++v) {
+    ib[v].assign(imgs + v * nb, imgs + (v + 1) * nb);
+    wb[v].assign(ws + v * nb, ws + (v + 1) * nb);
+    kernel_fft(k1 + v * nk, kd, bd, f1[v]);
+    kernel_fft(k2 + v * nk, kd, bd, f2[v]);
+  }
+  block_iteration_mul(psiBlk, bd, ib, wb, f1, f2, (double)lambda, minValue, maxI, nT, stats);
+  return 0;
+}
+
 #define DECON_ENTRY(SUF, T)                                                                                        \
   int oracle_deconvolve_##SUF(const int* ip, const float* fp, const float* imgs, const float* weights,              \
                               const float* kernels, const float* psi0, T* psiOut, float* maxOut, double* statsOut,  \

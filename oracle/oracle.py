@@ -88,6 +88,8 @@ def lib():
         L.oracle_psi_init_avg_precise.restype = ctypes.c_double
         L.oracle_block_seq_f32.argtypes = [_f32p, _i32p, _f32p, _f32p, _f32p, _f32p, _i32p, ctypes.c_float,
                                            ctypes.c_float, ctypes.c_float, ctypes.c_int, _f64p]
+        L.oracle_block_mul_f32.argtypes = [_f32p, _i32p, ctypes.c_int, _f32p, _f32p, _f32p, _f32p, _i32p,
+                                           ctypes.c_float, ctypes.c_float, _f32p, ctypes.c_int, _f64p]
         for suf, fp in (("f32", _f32p), ("f64", _f64p)):
             f = getattr(L, "oracle_deconvolve_" + suf)
             f.argtypes = [_i32p, _f32p, _f32p, _f32p, _f32p, _f32p, fp, _f32p, _f64p, fp, _f64p]
@@ -240,6 +242,20 @@ def block_seq(psi_blk, img_blk, w_blk, k1, k2, lam, min_value, max_i, nT):
                                np.ascontiguousarray(k1, np.float32).ravel(),
                                np.ascontiguousarray(k2, np.float32).ravel(), _dims_xyz(k1), lam, min_value, max_i,
                                nT, stats)
+    return psi, stats
+
+
+def block_mul(psi_blk, img_blks, w_blks, k1s, k2s, lam, min_value, max_is, nT):
+    """One ComputeBlockMulThreadCPU.runIteration on a block (kernels already final)."""
+    psi = np.ascontiguousarray(psi_blk, np.float32).copy()
+    imgs = np.ascontiguousarray(img_blks, np.float32)
+    ws = np.ascontiguousarray(w_blks, np.float32)
+    k1s = np.ascontiguousarray(k1s, np.float32)
+    k2s = np.ascontiguousarray(k2s, np.float32)
+    stats = np.zeros(2, np.float64)
+    lib().oracle_block_mul_f32(psi.ravel(), _dims_xyz(psi), imgs.shape[0], imgs.ravel(), ws.ravel(), k1s.ravel(),
+                               k2s.ravel(), _dims_xyz(k1s[0]), lam, min_value,
+                               np.ascontiguousarray(max_is, np.float32), nT, stats)
     return psi, stats
 
 

@@ -1,0 +1,62 @@
+"""CPU checks of the C ABI library and of the host-side block logic (no GPU calls)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from mvr import decon, hip
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(hip.LIB_PATH):
+        pytest.fail("libmvr_hip.so is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(hip.LIB_PATH)
+    names = hip.header_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_header_declares_reference_replacements():
+    names = hip.header_functions()
+    for n in ["mvr_get_num_devices", "mvr_convolution3d_fft_inplace", "mvr_block_run_iteration_seq",
+              "mvr_block_run_iteration_mul", "mvr_decon_run", "mvr_init_psfs"]:
+        assert n in names
+
+
+def test_python_block_generator_matches_oracle(oracle):
+    for img, ks, bs in [((1024, 1024, 1), (16, 32, 1), (384, 384, 1)), ((100, 60, 40), (11, 11, 11), (30, 30, 30)),
+                        ((256, 256, 256), (81, 81, 81), (336, 336, 336)), ((70, 33, 20), (9, 5, 3), (20, 20, 20))]:
+        a = decon.divide_into_blocks(img, ks, bs)
+        b = oracle.divide_into_blocks(img, ks, bs)
+        assert len(a) == len(b)
+        for x, y in zip(a, b):
+            assert x.offset == y["offset"] and x.effective_size == y["effectiveSize"]
+            assert x.effective_offset == y["effectiveOffset"] and x.effective_local_offset == y["effectiveLocalOffset"]
+    assert decon.divide_into_blocks((50, 50, 50), (81, 81, 81), (80, 80, 80)) is None
+
+
+@pytest.mark.parametrize("min_req", [1, 8, 100])
+def test_python_sorter_matches_oracle(oracle, min_req):
+    img, ks, bs = (100, 60, 40), (11, 11, 11), (30, 30, 30)
+    blocks = decon.divide_into_blocks(img, ks, bs)
+    ours = decon.sort_blocks_by_smallest_footprint(blocks, img, min_req)
+    theirs, _ = oracle.sort_blocks(img, ks, bs, min_req)
+    assert [[blocks.index(b) for b in batch] for batch in ours] == theirs
+
+
+def test_copy_paste_roundtrip():
+    rng = np.random.default_rng(1)
+    psi = rng.normal(size=(17, 19, 23)).astype(np.float32)
+    blocks = decon.divide_into_blocks((23, 19, 17), (5, 5, 5), (12, 12, 12))
+    out = np.zeros_like(psi)
+    for b in blocks:
+        decon.paste_block(out, b, decon.copy_block_mirror(psi, b))
+    np.testing.assert_array_equal(out, psi)
+
+
+def test_mirror_copy_law():
+    psi = np.arange(5, dtype=np.float32).reshape(1, 1, 5)
+    b = decon.Block((9, 1, 1), (-2, 0, 0), (5, 1, 1), (0, 0, 0), (2, 0, 0))
+    np.testing.assert_array_equal(decon.copy_block_mirror(psi, b)[0, 0], [2, 1, 0, 1, 2, 3, 4, 3, 2])

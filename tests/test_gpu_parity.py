@@ -1,0 +1,189 @@
+"""Parity of the HIP library (called through its C ABI) with the CPU oracle.
+
+Tolerances: FP32 results within rel-L2 1e-4 of the oracle's FP32 restatement
+after 10 iterations (the parity bar of the path); single FFT convolutions and
+one block iteration within 1e-5; every deconvolution also checked against the
+FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from mvr import decon, hip
+from mvr import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(2024)
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_device_query():
+    assert hip.num_devices() >= 1
+    name = hip.device_name(0)
+    print("device", name)
+    assert hip.lib().mvr_get_mem_device(0) > 0
+
+
+@pytest.mark.parametrize("shape,kshape,ext,value", [
+    ((20, 22, 18), (7, 9, 5), "mirror", 0.0),
+    ((16, 16, 16), (5, 5, 5), "value", 1.0),
+    ((12, 10, 30), (3, 4, 6), "value", 0.0),
+    ((33, 7, 40), (9, 3, 11), "mirror", 0.0),
+    ((1, 8, 8), (1, 3, 3), "mirror", 0.0),
+])
+def test_fft_convolve_matches_oracle(oracle, shape, kshape, ext, value):
+    img = RNG.uniform(0.5, 2.0, shape).astype(np.float32)
+    k = RNG.uniform(0.0, 1.0, kshape).astype(np.float32)
+    got = hip.fft_convolve(img, k, ext, value)
+    want = oracle.fft_convolve(img.astype(np.float64), k.astype(np.float64), ext, value, dtype=np.float64)
+    assert rel(got, want) < 3e-6
+
+
+def test_inplace_convolution_is_mirror_convolution(oracle):
+    img = RNG.uniform(0.5, 2.0, (24, 20, 16)).astype(np.float32)
+    k = RNG.uniform(0.0, 1.0, (5, 7, 3)).astype(np.float32)
+    got = hip.convolution3d_fft_inplace(img, k)
+    want = oracle.fft_convolve(img.astype(np.float64), k.astype(np.float64), "mirror", dtype=np.float64)
+    assert rel(got, want) < 3e-6
+
+
+@pytest.mark.parametrize("psf_type", ["INDEPENDENT", "OPTIMIZATION_II", "OPTIMIZATION_I", "EFFICIENT_BAYESIAN"])
+def test_init_psfs_matches_oracle(oracle, psf_type):
+    ks = np.stack([syn.gaussian_psf(9, 1.2, 2.5, a) for a in (0, 100, 230)])
+    k1, k2 = hip.init_psfs(ks, psf_type, 8)
+    o1, o2 = oracle.init_psfs(ks, psf_type, 8)
+    np.testing.assert_array_equal(k1, o1)          # normToSum1 is host arithmetic: bit-exact
+    assert rel(k2, o2) < 2e-6
+
+
+def _block_case(V=3, B=(24, 26, 22), K=(5, 7, 5), seed=5):
+    rng = np.random.default_rng(seed)
+    Bz = B[::-1]
+    psi = rng.uniform(5, 30, Bz).astype(np.float32)
+    imgs = rng.uniform(1, 60, (V,) + Bz).astype(np.float32)
+    imgs[0, :, :3] = 0.0
+    ws = rng.uniform(0, 1, (V,) + Bz).astype(np.float32)
+    ks = np.stack([syn.gaussian_psf(K[0], 1.0, 2.0, a) for a in np.linspace(0, 300, V)])
+    ks = ks[:, :, 1:-1, :] if K[1] < K[0] else np.pad(ks, ((0, 0), (0, 0), (1, 1), (0, 0)))
+    return psi, imgs, ws, ks
+
+
+@pytest.mark.parametrize("lam", [0.0, 6e-4])
+def test_block_seq_matches_oracle(oracle, lam):
+    psi, imgs, ws, ks = _block_case()
+    k1, k2 = oracle.init_psfs(ks, "EFFICIENT_BAYESIAN", 8)
+    B = psi.shape[::-1]
+    t = hip.BlockThread(B, 1e-4, lam)
+    for v in range(3):
+        t.set_view_kernels(v, k1[v], k2[v])
+    cur_g, cur_o = psi.copy(), psi.copy()
+    for v in range(3):
+        cur_g, st_g = t.run_iteration(v, cur_g, imgs[v], ws[v], 55.0)
+        cur_o, st_o = oracle.block_seq(cur_o, imgs[v], ws[v], k1[v], k2[v], lam, 1e-4, 55.0, 8)
+        assert rel(cur_g, cur_o) < 1e-5
+        assert st_g[0] == pytest.approx(st_o[0], rel=1e-3, abs=1e-3)
+        assert st_g[1] == pytest.approx(st_o[1], rel=1e-4)
+    t.close()
+
+
+def test_block_mul_matches_oracle(oracle):
+    psi, imgs, ws, ks = _block_case()
+    k1, k2 = oracle.init_psfs(ks, "OPTIMIZATION_I", 8)
+    B = psi.shape[::-1]
+    t = hip.BlockThread(B, 1e-4, 6e-4)
+    for v in range(3):
+        t.set_view_kernels(v, k1[v], k2[v])
+    mx = np.array([60.0, 58.0, 59.5], np.float32)
+    g, st_g = t.run_iteration_mul(psi, imgs, ws, mx)
+    o, st_o = oracle.block_mul(psi, imgs, ws, k1, k2, 6e-4, 1e-4, mx, 8)
+    assert rel(g, o) < 1e-5
+    assert st_g[1] == pytest.approx(st_o[1], rel=1e-4)
+    t.close()
+
+
+def _small_problem(n=(28, 30, 26), V=3, K=7, seed=11, weights="uniform"):
+    w = syn.small_workload(views=V, k=K, seed_index=seed, n_blobs=25, weights=weights, shape=n)
+    return syn.make_inputs(w)
+
+
+@pytest.mark.parametrize("psf_type,scheme,lam", [
+    ("EFFICIENT_BAYESIAN", "SEQ", 0.0),
+    ("OPTIMIZATION_II", "SEQ", 6e-4),
+    ("OPTIMIZATION_I", "SEQ", 0.006),
+    ("INDEPENDENT", "SEQ", 0.0),
+    ("EFFICIENT_BAYESIAN", "MUL", 6e-4),
+    ("OPTIMIZATION_II", "MUL", 0.0),
+])
+def test_resident_decon_matches_oracle(oracle, psf_type, scheme, lam):
+    inp = _small_problem(weights="zramp")
+    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+    it = 10
+    ref = oracle.deconvolve(imgs, ws, ks, psf_type=psf_type, scheme=scheme, lam=lam, iterations=it)
+    ref64 = oracle.deconvolve(imgs, ws, ks, psf_type=psf_type, scheme=scheme, lam=lam, iterations=it,
+                              dtype=np.float64)
+    d = hip.Decon(imgs.shape[1:], imgs.shape[0], psf_type, scheme, lam)
+    for v in range(imgs.shape[0]):
+        d.set_view(v, imgs[v], ws[v], ks[v])
+    d.init()
+    mx, avg = d.max_avg()
+    np.testing.assert_array_equal(mx, ref["max"])
+    assert avg == pytest.approx(ref["avg"], rel=1e-12)
+    d.run(it)
+    psi = d.psi()
+    e32, e64 = rel(psi, ref["psi"]), rel(psi, ref64["psi"])
+    print(f"{psf_type} {scheme} lam={lam}: rel-L2 vs oracle FP32 {e32:.2e}, vs FP64 {e64:.2e}, "
+          f"oracle FP32 vs FP64 {rel(ref['psi'], ref64['psi']):.2e}")
+    assert e32 < 1e-4
+    assert e64 < 1e-4
+    assert d.stats().shape == ((it, 3, 2) if scheme == "SEQ" else (it, 1, 2))
+    d.close()
+
+
+def test_resident_psi_init_from_array(oracle):
+    inp = _small_problem()
+    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+    psi0 = RNG.uniform(10, 40, imgs.shape[1:]).astype(np.float32)
+    ref = oracle.deconvolve(imgs, ws, ks, iterations=3, psi0=psi0)
+    d = hip.Decon(imgs.shape[1:], 3)
+    for v in range(3):
+        d.set_view(v, imgs[v], ws[v], ks[v])
+    d.init(psi0)
+    d.run(3)
+    assert rel(d.psi(), ref["psi"]) < 1e-4
+    d.reset_psi()
+    np.testing.assert_array_equal(d.psi(), psi0)
+    d.close()
+
+
+@pytest.mark.parametrize("scheme", ["SEQ", "MUL"])
+def test_blocked_driver_matches_oracle(oracle, scheme):
+    """The Java-side driver (mirrored in mvr.decon) with several blocks per view, every
+    block iteration in the HIP library, against the oracle with the same block size."""
+    inp = _small_problem(n=(30, 26, 28), K=5)
+    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+    bs = (22, 22, 22)
+    ref = oracle.deconvolve(imgs, ws, ks, scheme=scheme, iterations=4, block_size=bs, lam=6e-4)
+    views = [decon.DeconView(imgs[v], ws[v], ks[v], bs) for v in range(3)]
+    assert views[0].num_blocks > 1
+    dv = decon.DeconViews(views, "EFFICIENT_BAYESIAN")
+    m = decon.MultiViewDeconvolution(dv, 4, decon.ComputeBlockThreadHIPFactory(bs, 6e-4), scheme)
+    m.run_iterations()
+    assert rel(m.psi, ref["psi"]) < 1e-4
+    m.close()
+
+
+def test_errors_are_reported():
+    with pytest.raises(hip.MvrError):
+        hip.Decon((8, 8, 8), 0)
+    d = hip.Decon((8, 8, 8), 1)
+    with pytest.raises(hip.MvrError):
+        d.run(1)                                         # init first
+    with pytest.raises(hip.MvrError):
+        hip.BlockThread((8, 8, 8)).run_iteration(0, np.zeros((8, 8, 8)), np.zeros((8, 8, 8)),
+                                                 np.zeros((8, 8, 8)), 1.0)
+    d.close()
