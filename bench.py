@@ -1,0 +1,149 @@
+#!/usr/bin/env python3
+"""Benchmark of the multiview Richardson-Lucy deconvolution path on one MI355X.
+
+Workload (default): 256^3 float32 volume, 6 views, 41^3 rotated Gaussian PSFs,
+Optimization-II compound kernels, Tikhonov lambda = 0.0006, 20 iterations, one
+block (blob phantom of 1500 blobs, Poisson noise at peak 1000, cosine z-ramp
+weights).  A step is one full deconvolution of that workload: psi reset to
+its initial estimate, then 20 Seq iterations over all 6 views, with the views,
+weights, PSF spectra and psi resident in HBM.
+
+Printed: one JSON line with value = voxel-iterations/s (psi voxels x
+iterations / second) and, for the dominant kernel, its achieved HBM bandwidth
+from algorithmic bytes (HIP events around each of its launches in the timed
+region), plus a CPU baseline: the oracle restatement of the reference's CPU
+path timed on a bounded sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(REPO, "multiview-reconstruction_amd")]
+
+METRIC = "RL voxel-iters/s; wall time to rel-L2≤1e-2, learned-init vs plain, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic HBM bytes per FFT-extent voxel for each pass (complex half spectrum = 4 B per real voxel):
+#   x_fwd   : psi gather 4 + spectrum write 4
+#   y_fwd/y_inv: spectrum read 4 + write 4
+#   z_conv  : spectrum read 4 + kernel spectrum read 4 + write 4
+#   x_ratio : spectrum read 4 + view image 4 + spectrum write 4
+#   x_update: spectrum read 4 + psi read 4 + weight 4 + psi write 4
+PASS_BYTES = {"x_fwd": 8, "y_fwd": 8, "z_conv": 12, "y_inv": 8, "x_ratio": 12, "x_update": 16, "x_store": 8}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-views", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this path "
+              "is described in DESIGN.md and not built", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from mvr import hip
+    from mvr import synthetic as syn
+
+    w = syn.C2 if args.workload == "c2" else syn.C1
+    t0 = time.time()
+    inp = syn.make_inputs(w)
+    log(f"inputs generated in {time.time() - t0:.1f}s")
+    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+    V = w.n_views
+    nz, ny, nx = w.shape_zyx
+    nvox = nx * ny * nz
+
+    d = hip.Decon(w.shape_zyx, V, w.psf_type, "SEQ", w.lam)
+    for v in range(V):
+        d.set_view(v, imgs[v], ws[v], ks[v])
+    t0 = time.time()
+    d.init()
+    log(f"psf init + spectra + psi init in {time.time() - t0:.2f}s, fft grid {d.geometry()}")
+
+    # warmup; the first warmup step times every pass to find the dominant kernel
+    d.profile(None)
+    for _ in range(max(1, args.warmup)):
+        d.reset_psi()
+        d.run(w.iterations)
+    times = d.pass_times()
+    steps_w = max(1, args.warmup)
+    per_pass = {k: v[0] / steps_w for k, v in times.items() if v[1]}
+    dominant = max(per_pass, key=per_pass.get)
+    log("per-pass ms per step: " + ", ".join(f"{k} {v:.2f}" for k, v in per_pass.items()))
+
+    # timed region: only the dominant kernel carries HIP events
+    d.profile([dominant])
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        d.reset_psi()
+        d.run(w.iterations)
+    elapsed = time.perf_counter() - t0
+    ms_dom, n_dom = d.pass_times()[dominant]
+    d.profile([])
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = nvox * w.iterations * args.steps / elapsed
+
+    geo = d.geometry()
+    K = ks.shape[1:]
+    ext = (nz + K[0] - 1) * (ny + K[1] - 1) * (nx + K[2] - 1)
+    alg_bytes = PASS_BYTES[dominant] * ext
+    avg_s = ms_dom / 1e3 / max(1, n_dom)
+    achieved = alg_bytes / avg_s / 1e9
+    traffic = None
+    tfile = os.path.join(REPO, "profiles", f"{dominant}_{w.name}_traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("bytes_per_launch")
+
+    psi = d.psi()
+    finite = bool(np.isfinite(psi).all())
+    d.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        threads = oracle.set_threads(threads)
+        sv = max(1, min(V, args.cpu_sample_views))
+        r = oracle.deconvolve(imgs, ws, ks, psf_type=w.psf_type, lam=w.lam, iterations=1, sample_views=sv)
+        cpu_val = nvox * (sv / V) / r["loop_seconds"]
+        cpu = {"value": cpu_val, "unit": "voxel-iters/s", "cores": threads, "kind": "port",
+               "sample": f"{sv} of {V} view updates of iteration 1 of the same workload, reference block "
+                         f"structure (one {nx + 2 * (K[2] - 1)}^3 block, padded FFT), oracle restatement "
+                         f"in FP32 on {threads} OpenMP threads; loop time {r['loop_seconds']:.2f}s"}
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "voxel-iters/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": w.name, "volume_zyx": list(w.shape_zyx), "views": V, "psf": list(K),
+                   "psf_type": w.psf_type, "lambda": w.lam, "iterations": w.iterations, "scheme": "SEQ",
+                   "fft_grid_xyz": list(geo["P"]), "parallelism": "single GPU", "psi_finite": finite},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_s * 1e6, "launches": n_dom,
+                     "per_pass_ms_per_step": per_pass},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

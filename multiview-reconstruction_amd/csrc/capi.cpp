@@ -262,9 +262,9 @@ int mvr_decon_get_geometry(mvr_decon* d, int out[4]) {
   return 0;
 }
 
-int mvr_decon_profile(mvr_decon* d, int enable) {
+int mvr_decon_profile(mvr_decon* d, unsigned mask) {
   NULLCHECK(d);
-  d->d.timer.on = enable != 0;
+  d->d.timer.reset(mask);
   return 0;
 }
 

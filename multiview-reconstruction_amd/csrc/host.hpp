@@ -69,7 +69,8 @@ enum Pass { PASS_X_FWD = 0, PASS_Y_FWD, PASS_Z_CONV, PASS_Y_INV, PASS_X_RATIO, P
             PASS_UPDATE_MUL, PASS_REDUCE, PASS_COUNT };
 
 struct Timer {
-  bool on = false;
+  bool on = false, pending = false;
+  unsigned mask = 0;
   std::vector<hipEvent_t> ev;
   std::vector<int> cat;
   size_t used = 0;
@@ -77,6 +78,7 @@ struct Timer {
   void end(hipStream_t s);
   int collect(double* ms, long long* n);  // accumulates into arrays of PASS_COUNT
   void release();
+  void reset(unsigned mask);  // enable the categories in mask (bit c = category c), clear totals
   double ms[PASS_COUNT] = {0};
   long long cnt[PASS_COUNT] = {0};
 };

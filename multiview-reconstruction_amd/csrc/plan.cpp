@@ -110,7 +110,8 @@ int Plan::convolve(const GatherParams& g, const float2* spec, float2* work, floa
 
 // ------------------------------------------------------------------ timers
 void Timer::begin(hipStream_t s, int c) {
-  if (!on) return;
+  pending = on && ((mask >> c) & 1u);
+  if (!pending) return;
   if (used + 2 > ev.size()) {
     size_t old = ev.size();
     ev.resize(old + 256);
@@ -121,13 +122,19 @@ void Timer::begin(hipStream_t s, int c) {
   used++;
 }
 void Timer::end(hipStream_t s) {
-  if (!on) return;
+  if (!pending) return;
   (void)hipEventRecord(ev[used], s);
   used++;
+  pending = false;
+}
+void Timer::reset(unsigned m) {
+  mask = m;
+  on = m != 0;
+  for (int c = 0; c < PASS_COUNT; ++c) { ms[c] = 0; cnt[c] = 0; }
 }
 int Timer::collect(double* outMs, long long* outN) {
-  if (on) {
-    (void)hipEventSynchronize(ev[used ? used - 1 : 0]);
+  if (used > 0) {
+    (void)hipEventSynchronize(ev[used - 1]);
     for (size_t i = 0; i + 1 < used; i += 2) {
       float t = 0.f;
       (void)hipEventElapsedTime(&t, ev[i], ev[i + 1]);

@@ -74,7 +74,7 @@ def lib():
         L.mvr_decon_get_stats.argtypes = [vp, _f64p, ctypes.c_int]
         L.mvr_decon_reset_psi.argtypes = [vp]
         L.mvr_decon_get_geometry.argtypes = [vp, _i32p]
-        L.mvr_decon_profile.argtypes = [vp, ctypes.c_int]
+        L.mvr_decon_profile.argtypes = [vp, ctypes.c_uint]
         L.mvr_decon_pass_times.argtypes = [vp, _f64p, ctypes.POINTER(ctypes.c_longlong)]
         L.mvr_decon_destroy.argtypes = [vp]
         L.mvr_decon_destroy.restype = None
@@ -250,8 +250,15 @@ class Decon:
         _check(lib().mvr_decon_get_geometry(self.h, g))
         return dict(P=(g[0], g[1], g[2]), row_pitch=g[3])
 
-    def profile(self, on=True):
-        _check(lib().mvr_decon_profile(self.h, 1 if on else 0))
+    def profile(self, passes=None):
+        """Time the given passes (names of PASSES; None = all, [] = off); clears the totals."""
+        if passes is None:
+            mask = (1 << len(PASSES)) - 1
+        else:
+            mask = 0
+            for p in passes:
+                mask |= 1 << PASSES.index(p)
+        _check(lib().mvr_decon_profile(self.h, mask))
 
     def pass_times(self):
         ms = (ctypes.c_double * 9)()

@@ -107,10 +107,16 @@ def gaussian_psf(size, sigma_xy, sigma_z, angle_deg):
 
 def convolve_mirror_f64(vol, psf):
     """FP64 linear convolution of the mirror-single extended volume with psf (odd size)."""
-    from scipy.signal import fftconvolve
+    from scipy import fft as sfft
     c = [s // 2 for s in psf.shape]
     pad = np.pad(vol, [(ci, s - 1 - ci) for ci, s in zip(c, psf.shape)], mode="reflect")
-    return fftconvolve(pad, psf.astype(np.float64), mode="valid")
+    # circular convolution of length >= the padded extent only wraps into the first K-1
+    # outputs, which the 'valid' crop discards
+    L = [sfft.next_fast_len(n, real=True) for n in pad.shape]
+    f = sfft.rfftn(pad, L, workers=-1) * sfft.rfftn(psf.astype(np.float64), L, workers=-1)
+    full = sfft.irfftn(f, L, workers=-1)
+    k = psf.shape
+    return full[k[0] - 1:k[0] - 1 + vol.shape[0], k[1] - 1:k[1] - 1 + vol.shape[1], k[2] - 1:k[2] - 1 + vol.shape[2]]
 
 
 _LUT = None

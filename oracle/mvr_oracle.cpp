@@ -789,12 +789,13 @@ struct Params {
   int nT;
   int psiInit;           // 0 = AvgPrecise, 1 = from array (PsiInitFromRAI, precise)
   int filterBlocks;      // DeconView filterBlocksForContent
+  int sampleViews;       // > 0: stop after this many view passes (timing samples of the Seq scheme)
 };
 
 template <typename T>
 static int deconvolve(const Params& p, const float* const* imgs, const float* const* weights,
                       const float* const* kernels, const float* psi0, T* psiOut, float* maxOut,
-                      double* statsOut, T* k2Out, double* avgOut) {
+                      double* statsOut, T* k2Out, double* avgOut, double* loopSeconds) {
   const int V = p.V;
   const i64 N[3] = {p.N[0], p.N[1], p.N[2]};
   const i64 nvox = N[0] * N[1] * N[2];
@@ -826,10 +827,11 @@ static int deconvolve(const Params& p, const float* const* imgs, const float* co
   init_psfs(k1, kd, p.psfType, p.nT, k2);
   if (k2Out)
     for (int v = 0; v < V; ++v) std::copy(k2[v].begin(), k2[v].end(), k2Out + v * nk);
-  // kernel spectra at block size (DeconViewPSF.getKernel{1,2}FFT, :79-111)
+  // kernel spectra at block size (DeconViewPSF.getKernel{1,2}FFT, :79-111), for the views that run
   const int bd[3] = {p.blockSize[0], p.blockSize[1], p.blockSize[2]};
   std::vector<KernelFFT<T>> k1f(V), k2f(V);
-  for (int v = 0; v < V; ++v) {
+  const int usedViews = (p.sampleViews > 0 && p.scheme == 0) ? std::min(V, p.sampleViews) : V;
+  for (int v = 0; v < usedViews; ++v) {
     kernel_fft(k1[v].data(), kd[v].data(), bd, k1f[v]);
     kernel_fft(k2[v].data(), kd[v].data(), bd, k2f[v]);
   }
@@ -844,9 +846,15 @@ static int deconvolve(const Params& p, const float* const* imgs, const float* co
   }
   const i64 nb = (i64)bd[0] * bd[1] * bd[2];
   std::vector<T> psiBlk(nb);
+  int viewPasses = 0;
+#ifdef _OPENMP
+  const double t0 = omp_get_wtime();
+#endif
   for (int it = 0; it < p.iterations; ++it) {
     if (p.scheme == 0) {
       for (int v = 0; v < V; ++v) {
+        if (p.sampleViews > 0 && viewPasses >= p.sampleViews) break;
+        ++viewPasses;
         const size_t totalBlocks = blocks[v].size();
         double sumChange = 0, maxChange = -1;
         std::vector<std::pair<int, std::vector<T>>> prevQ, curQ;
@@ -902,6 +910,11 @@ static int deconvolve(const Params& p, const float* const* imgs, const float* co
       if (statsOut) { statsOut[2 * it] = sumChange; statsOut[2 * it + 1] = maxChange; }
     }
   }
+#ifdef _OPENMP
+  if (loopSeconds) *loopSeconds = omp_get_wtime() - t0;
+#else
+  if (loopSeconds) *loopSeconds = 0;
+#endif
   std::copy(psi.begin(), psi.end(), psiOut);
   return 0;
 }
@@ -1072,16 +1085,17 @@ int oracle_block_mul_f32(float* psiBlk, const int* bd, int V, const float* imgs,
 #define DECON_ENTRY(SUF, T)                                                                                        \
   int oracle_deconvolve_##SUF(const int* ip, const float* fp, const float* imgs, const float* weights,              \
                               const float* kernels, const float* psi0, T* psiOut, float* maxOut, double* statsOut,  \
-                              T* k2Out, double* avgOut) {                                                           \
+                              T* k2Out, double* avgOut, double* loopSeconds) {                                      \
     Params p;                                                                                                      \
     for (int d = 0; d < 3; ++d) { p.N[d] = ip[d]; p.kdim[d] = ip[3 + d]; p.blockSize[d] = ip[6 + d]; }             \
     p.V = ip[9]; p.psfType = ip[10]; p.scheme = ip[11]; p.iterations = ip[12]; p.nT = ip[13];                      \
-    p.psiInit = ip[14]; p.filterBlocks = ip[15];                                                                   \
+    p.psiInit = ip[14]; p.filterBlocks = ip[15]; p.sampleViews = ip[16];                                           \
     p.lambda = fp[0]; p.minValue = fp[1];                                                                          \
     const i64 nvox = (i64)p.N[0] * p.N[1] * p.N[2], nk = (i64)p.kdim[0] * p.kdim[1] * p.kdim[2];                  \
     std::vector<const float*> pi(p.V), pw(p.V), pk(p.V);                                                           \
     for (int v = 0; v < p.V; ++v) { pi[v] = imgs + v * nvox; pw[v] = weights + v * nvox; pk[v] = kernels + v * nk; } \
-    return deconvolve<T>(p, pi.data(), pw.data(), pk.data(), psi0, psiOut, maxOut, statsOut, k2Out, avgOut);      \
+    return deconvolve<T>(p, pi.data(), pw.data(), pk.data(), psi0, psiOut, maxOut, statsOut, k2Out, avgOut,       \
+                         loopSeconds);                                                                             \
   }
 DECON_ENTRY(f32, float)
 DECON_ENTRY(f64, double)

@@ -92,7 +92,7 @@ def lib():
                                            ctypes.c_float, ctypes.c_float, _f32p, ctypes.c_int, _f64p]
         for suf, fp in (("f32", _f32p), ("f64", _f64p)):
             f = getattr(L, "oracle_deconvolve_" + suf)
-            f.argtypes = [_i32p, _f32p, _f32p, _f32p, _f32p, _f32p, fp, _f32p, _f64p, fp, _f64p]
+            f.argtypes = [_i32p, _f32p, _f32p, _f32p, _f32p, _f32p, fp, _f32p, _f64p, fp, _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -260,13 +260,15 @@ def block_mul(psi_blk, img_blks, w_blks, k1s, k2s, lam, min_value, max_is, nT):
 
 
 def deconvolve(imgs, weights, kernels, *, psf_type="EFFICIENT_BAYESIAN", scheme="SEQ", block_size=None,
-               lam=0.0, min_value=1e-4, iterations=10, nT=8, psi0=None, filter_blocks=True, dtype=np.float32):
+               lam=0.0, min_value=1e-4, iterations=10, nT=8, psi0=None, filter_blocks=True, dtype=np.float32,
+               sample_views=0):
     """Run the restated reference driver (MultiViewDeconvolution{Seq,Mul}).
 
     imgs, weights: [V, z, y, x] float32.  kernels: [V, kz, ky, kx] float32 (raw PSFs,
     normalised inside as DeconViewPSF.init does).  block_size (x,y,z) defaults to
     one block covering the volume: N + 2(K-1).
-    Returns dict(psi, max, stats[it, v, 2] or [it, 2], k2, avg).
+    sample_views > 0 (Seq only) stops after that many view passes, for timing samples.
+    Returns dict(psi, max, stats[it, v, 2] or [it, 2], k2, avg, loop_seconds).
     """
     imgs = np.ascontiguousarray(imgs, np.float32)
     weights = np.ascontiguousarray(weights, np.float32)
@@ -276,7 +278,7 @@ def deconvolve(imgs, weights, kernels, *, psf_type="EFFICIENT_BAYESIAN", scheme=
     if block_size is None:
         block_size = (nx + 2 * (kx - 1), ny + 2 * (ky - 1), nz + 2 * (kz - 1))
     ip = np.array([nx, ny, nz, kx, ky, kz, *block_size, V, PSF_TYPES[psf_type], SCHEMES[scheme], iterations, nT,
-                   0 if psi0 is None else 1, 1 if filter_blocks else 0], np.int32)
+                   0 if psi0 is None else 1, 1 if filter_blocks else 0, sample_views], np.int32)
     fp = np.array([lam, min_value], np.float32)
     psi_in = np.ascontiguousarray(psi0 if psi0 is not None else np.zeros(1), np.float32)
     psi = np.zeros((nz, ny, nx), dtype)
@@ -285,10 +287,11 @@ def deconvolve(imgs, weights, kernels, *, psf_type="EFFICIENT_BAYESIAN", scheme=
     stats = np.zeros(nst, np.float64)
     k2 = np.zeros((V, kz, ky, kx), dtype)
     avg = np.zeros(1, np.float64)
+    secs = np.zeros(1, np.float64)
     f = lib().oracle_deconvolve_f32 if dtype == np.float32 else lib().oracle_deconvolve_f64
     rc = f(ip, fp, imgs.ravel(), weights.ravel(), kernels.ravel(), psi_in.ravel(), psi.ravel(), mx, stats, k2.ravel(),
-           avg)
+           avg, secs)
     if rc != 0:
         raise ValueError("oracle deconvolve failed (block smaller than kernel?)")
     shape = (iterations, V, 2) if scheme == "SEQ" else (iterations, 2)
-    return dict(psi=psi, max=mx, stats=stats.reshape(shape), k2=k2, avg=float(avg[0]))
+    return dict(psi=psi, max=mx, stats=stats.reshape(shape), k2=k2, avg=float(avg[0]), loop_seconds=float(secs[0]))
