@@ -51,7 +51,7 @@ def generate():
     files = []
     for gi, grp in enumerate(_groups(HALF, 10)):
         body = "".join(X_FUNCS.format(M=m) for m in grp)
-        files.append((f"x_{gi}.hip", f'#include "passes.hpp"\nnamespace mvr {{\n{body}}}\n'))
+        files.append((f"x_{gi}.hip", f'#include "xpasses.hpp"\nnamespace mvr {{\n{body}}}\n'))
     for gi, grp in enumerate(_groups(STRIDED, 10)):
         body = "".join(S_FUNCS.format(L=l) for l in grp)
         files.append((f"s_{gi}.hip", f'#include "passes.hpp"\nnamespace mvr {{\n{body}}}\n'))

@@ -13,6 +13,7 @@
 #pragma once
 #include "engine.hpp"
 #include "fft_core.hpp"
+#include "fft_reg.hpp"
 
 namespace mvr {
 
@@ -364,23 +365,134 @@ __global__ __launch_bounds__(kThreads) void k_strided(StridedGeom gm, const floa
   }
 }
 
+// ------------------------------------------------------------------ y / z passes, two-step register FFT
+// Pencil length L = A x B.  Forward: thread (w, b) loads x[b + B a] (a < A) straight
+// from HBM, does the A-point DFT in registers, multiplies by W_L^{b k1} and parks
+// the values in LDS; thread (w, k1) then reads its B values and does the
+// B-point DFT, which yields X[k1 + A k2].  The inverse runs the same two steps
+// in the opposite order, so in the convolution pass the spectrum product and
+// the first inverse step stay in the registers of the forward's second step:
+// one LDS round trip per transform.
+template <int L>
+__host__ __device__ constexpr int s2_width() {
+  return L <= 320 ? 16 : 8;
+}
+template <int L>
+__host__ __device__ constexpr int s2_threads() {
+  return s2_width<L>() * split2(L).A;
+}
+
+template <int L, int MODE>
+__global__ __launch_bounds__(s2_threads<L>()) void k_strided2(StridedGeom gm, const float2* __restrict__ in,
+                                                               float2* __restrict__ out,
+                                                               const float2* __restrict__ spec, float scale,
+                                                               const float2* __restrict__ twg) {
+  constexpr Split2 S = split2(L);
+  constexpr int A = S.A, B = S.B;
+  constexpr int W = s2_width<L>();
+  constexpr int NT = W * A;
+  __shared__ float2 tile[L * W];
+  __shared__ float2 tw[L];
+  for (int i = threadIdx.x; i < L; i += NT) tw[i] = twg[i];
+  const int w = threadIdx.x % W, q = threadIdx.x / W;
+  const int col0 = blockIdx.x * W;
+  const bool colOk = w < gm.ncols - col0;
+  const float2* src = in + (size_t)blockIdx.y * gm.planeStride + col0 + w;
+  float2* dst = out + (size_t)blockIdx.y * gm.planeStride + col0 + w;
+  const size_t ls = gm.lineStride;
+  float2 v[A];
+  if constexpr (MODE != 1) {
+    // ---- forward step 1: thread q = b < B
+    if (q < B) {
+#pragma unroll
+      for (int a = 0; a < A; ++a) v[a] = colOk ? src[(size_t)(q + B * a) * ls] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();  // twiddle table
+    if (q < B) {
+      rdft<A, -1>(v);
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) {
+        const float2 t = k1 == 0 ? v[k1] : cmul(v[k1], tw[q * k1]);
+        tile[(q + B * k1) * W + w] = t;
+      }
+    }
+    __syncthreads();
+    // ---- forward step 2: thread q = k1 < A
+    float2 u[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) u[b] = tile[(b + B * q) * W + w];
+    rdft<B, -1>(u);  // u[k2] = X[q + A k2]
+    if constexpr (MODE == 0 || MODE == 3) {
+      if (colOk) {
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) {
+          float2 x = u[k2];
+          if constexpr (MODE == 3) x = cscale(x, scale);
+          dst[(size_t)(q + A * k2) * ls] = x;
+        }
+      }
+      return;
+    } else {
+      // spectrum product, then inverse step 1 over k2 in registers
+      const float2* sp = spec + (size_t)blockIdx.y * gm.planeStride + col0 + w;
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) {
+        const float2 kv = colOk ? sp[(size_t)(q + A * k2) * ls] : make_float2(0.f, 0.f);
+        u[k2] = cmul(u[k2], kv);
+      }
+      rdft<B, 1>(u);  // u[n2] = sum_k2 X[q + A k2] W_B^{-n2 k2}
+      __syncthreads();  // every thread has read its forward values from the tile
+#pragma unroll
+      for (int n2 = 0; n2 < B; ++n2) {
+        const float2 t = n2 == 0 ? u[n2] : cmul(u[n2], cconj(tw[n2 * q]));
+        tile[(q + A * n2) * W + w] = t;
+      }
+    }
+  } else {
+    // ---- inverse only: thread q = k1 < A holds X[q + A k2]
+    float2 u[B];
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) u[k2] = colOk ? src[(size_t)(q + A * k2) * ls] : make_float2(0.f, 0.f);
+    __syncthreads();  // twiddle table
+    rdft<B, 1>(u);
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+      const float2 t = n2 == 0 ? u[n2] : cmul(u[n2], cconj(tw[n2 * q]));
+      tile[(q + A * n2) * W + w] = t;
+    }
+  }
+  if constexpr (MODE == 1 || MODE == 2) {
+    __syncthreads();
+    // ---- inverse step 2: thread q = n2 < B does the A-point DFT over k1 -> x[n2 + B n1]
+    if (q < B) {
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) v[k1] = tile[(k1 + A * q) * W + w];
+      rdft<A, 1>(v);
+      if (colOk) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) dst[(size_t)(q + B * n1) * ls] = v[n1];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ host launchers
 template <int M>
-void launch_x_fwd(const Dom& d, const GatherParams& g, float2* S, const float2* twM, const float2* wr,
+void launch_x1_fwd(const Dom& d, const GatherParams& g, float2* S, const float2* twM, const float2* wr,
                   hipStream_t s) {
   constexpr int RW = kThreads / x_threads_per_row(M);
   const int rows = d.P[1] * d.P[2];
   hipLaunchKernelGGL(k_x_fwd<M>, dim3((rows + RW - 1) / RW), dim3(kThreads), 0, s, d, g, S, twM, wr);
 }
 template <int M>
-void launch_x_ratio(const Dom& d, const RatioParams& q, const float2* Sin, float2* Sout, const float2* twM,
+void launch_x1_ratio(const Dom& d, const RatioParams& q, const float2* Sin, float2* Sout, const float2* twM,
                     const float2* wr, hipStream_t s) {
   constexpr int RW = kThreads / x_threads_per_row(M);
   const int rows = d.P[1] * d.P[2];
   hipLaunchKernelGGL(k_x_ratio<M>, dim3((rows + RW - 1) / RW), dim3(kThreads), 0, s, d, q, Sin, Sout, twM, wr);
 }
 template <int M>
-int launch_x_update(const Dom& d, const UpdateParams& u, const float2* S, const float2* twM, const float2* wr,
+int launch_x1_update(const Dom& d, const UpdateParams& u, const float2* S, const float2* twM, const float2* wr,
                     hipStream_t s) {
   constexpr int RW = kThreads / x_threads_per_row(M);
   const int rows = d.D[1] * d.D[2];
@@ -389,7 +501,7 @@ int launch_x_update(const Dom& d, const UpdateParams& u, const float2* S, const 
   return nb;
 }
 template <int M>
-void launch_x_store(const Dom& d, const StoreParams& st, const float2* S, const float2* twM, const float2* wr,
+void launch_x1_store(const Dom& d, const StoreParams& st, const float2* S, const float2* twM, const float2* wr,
                     hipStream_t s) {
   constexpr int RW = kThreads / x_threads_per_row(M);
   const int rows = d.D[1] * d.D[2];
@@ -398,6 +510,18 @@ void launch_x_store(const Dom& d, const StoreParams& st, const float2* S, const 
 template <int L>
 void launch_strided(const StridedGeom& gm, int mode, const float2* in, float2* out, const float2* spec, float scale,
                     const float2* tw, hipStream_t s) {
+  if constexpr (split2(L).A > 0 && L * s2_width<L>() * 8 <= 96 * 1024) {
+    constexpr int W2 = s2_width<L>();
+    constexpr int NT = s2_threads<L>();
+    const dim3 g2((gm.ncols + W2 - 1) / W2, gm.nplanes);
+    switch (mode) {
+      case 0: hipLaunchKernelGGL((k_strided2<L, 0>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
+      case 1: hipLaunchKernelGGL((k_strided2<L, 1>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
+      case 2: hipLaunchKernelGGL((k_strided2<L, 2>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
+      default: hipLaunchKernelGGL((k_strided2<L, 3>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
+    }
+    return;
+  }
   constexpr int W = strided_width(L);
   const dim3 grid((gm.ncols + W - 1) / W, gm.nplanes);
   switch (mode) {

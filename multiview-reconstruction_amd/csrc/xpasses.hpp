@@ -1,0 +1,443 @@
+// xpasses.hpp -- x (real axis) passes with the two-step register FFT.
+//
+// A work-group owns RW rows of one x-pass.  Global traffic is staged through
+// LDS with flat, coalesced mappings (row data is contiguous in HBM); the
+// length-M complex transform of a row (M = P0/2, real FFT by the even/odd
+// split) runs as M = A x B with one LDS exchange, and the C2R pre-processing
+// is folded into the first inverse step.  Index arithmetic that depends on
+// runtime sizes is done once per row (row tables in LDS) or with a
+// multiply-shift divider, never per element with '/' or '%'.
+#pragma once
+#include "passes.hpp"
+
+namespace mvr {
+
+// exact n / d for n < 2^32 / d (d >= 1)
+struct FastDiv {
+  unsigned d, m;
+  __host__ __device__ FastDiv() : d(1), m(0) {}
+  __host__ __device__ explicit FastDiv(unsigned dd) : d(dd), m((unsigned)((0x100000000ull + dd - 1) / dd)) {}
+  __device__ __forceinline__ unsigned div(unsigned n) const {
+    return d == 1 ? n : (unsigned)(((unsigned long long)n * m) >> 32);
+  }
+};
+
+// {sum, max} of a work-group of NT threads (NT <= 256, any multiple of 1)
+template <int NT>
+__device__ __forceinline__ void block_reduce_stats_nt(double s, double mx, double2* out) {
+  __shared__ double rs[256], rm[256];
+  rs[threadIdx.x] = s;
+  rm[threadIdx.x] = mx;
+  for (int i = NT + (int)threadIdx.x; i < 256; i += NT) {
+    rs[i] = 0.0;
+    rm[i] = -1.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      rs[threadIdx.x] += rs[threadIdx.x + off];
+      rm[threadIdx.x] = fmax(rm[threadIdx.x], rm[threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = make_double2(rs[0], rm[0]);
+}
+
+constexpr long long kOuterRow = -(1ll << 62);
+
+template <int M>
+struct XCfg {
+  static constexpr Split2 S = split2(M);
+  static constexpr int A = S.A, B = S.B;
+  static constexpr int T = A;                      // threads per row
+  static constexpr int RW = 256 / A;               // rows per work-group
+  static constexpr int NT = RW * T;                // threads per work-group
+  static constexpr int RS = M + 1;                 // LDS row stride (complex)
+  static constexpr int PX = 2 * M;
+};
+
+template <int M>
+__host__ __device__ constexpr bool x2_ok() {
+  return split2(M).A > 0 && split2(M).B > 1;
+}
+
+// forward length-M transform of LDS row `rb` (natural order) -> out[k2] = Z[q + A k2] (thread q < A)
+template <int M>
+__device__ __forceinline__ void x2_forward(float2* rb, const float2* tw, int q, float2* out) {
+  using C = XCfg<M>;
+  constexpr int A = C::A, B = C::B;
+  float2 v[A];
+  if (q < B) {
+#pragma unroll
+    for (int a = 0; a < A; ++a) v[a] = rb[q + B * a];
+    rdft<A, -1>(v);
+  }
+  __syncthreads();
+  if (q < B) {
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) rb[q + B * k1] = k1 == 0 ? v[k1] : cmul(v[k1], tw[q * k1]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < B; ++b) out[b] = rb[b + B * q];
+  rdft<B, -1>(out);
+}
+
+// inverse of a half spectrum X[0..M] held in LDS row `rb`: the C2R pre-processing
+// Z[k] = E[k] + i O[k] is applied on load; returns z[q + B n1] (thread q < B) in out[n1]
+template <int M>
+__device__ __forceinline__ void x2_inverse_half(float2* rb, const float2* tw, const float2* twr, int q, float2* out) {
+  using C = XCfg<M>;
+  constexpr int A = C::A, B = C::B;
+  float2 u[B];
+#pragma unroll
+  for (int k2 = 0; k2 < B; ++k2) {
+    const int k = q + A * k2;
+    const float2 a = rb[k];
+    const float2 b = cconj(rb[M - k]);
+    const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+    const float2 o = cmul(make_float2(0.5f * (a.x - b.x), 0.5f * (a.y - b.y)), cconj(twr[k]));
+    u[k2] = make_float2(e.x - o.y, e.y + o.x);
+  }
+  rdft<B, 1>(u);
+  __syncthreads();
+#pragma unroll
+  for (int n2 = 0; n2 < B; ++n2) rb[q + A * n2] = n2 == 0 ? u[n2] : cmul(u[n2], cconj(tw[n2 * q]));
+  __syncthreads();
+  if (q < B) {
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) out[k1] = rb[k1 + A * q];
+    rdft<A, 1>(out);
+  }
+}
+
+// registers -> natural order in LDS.  Forward result: Z[q + A k2] from out[k2].
+template <int M>
+__device__ __forceinline__ void x2_park_forward(float2* rb, int q, const float2* z) {
+  using C = XCfg<M>;
+  __syncthreads();
+#pragma unroll
+  for (int k2 = 0; k2 < C::B; ++k2) rb[q + C::A * k2] = z[k2];
+  __syncthreads();
+}
+// inverse result: z[q + B n1] from out[n1] (thread q < B)
+template <int M>
+__device__ __forceinline__ void x2_park_inverse(float2* rb, int q, const float2* z) {
+  using C = XCfg<M>;
+  __syncthreads();
+  if (q < C::B) {
+#pragma unroll
+    for (int n1 = 0; n1 < C::A; ++n1) rb[q + C::B * n1] = z[n1];
+  }
+  __syncthreads();
+}
+
+// R2C post-processing of every row in LDS, written to HBM with a flat mapping
+template <int M>
+__device__ __forceinline__ void x2_post_store(const float2* buf, const float2* twr, const size_t* dstRow,
+                                              float2* __restrict__ S) {
+  using C = XCfg<M>;
+  for (int idx = threadIdx.x; idx < C::RW * (M + 1); idx += C::NT) {
+    const int r = idx / (M + 1), k = idx - r * (M + 1);
+    const size_t o = dstRow[r];
+    if (o == ~(size_t)0) continue;
+    const float2* rb = buf + r * C::RS;
+    const float2 zk = rb[k == M ? 0 : k];
+    const float2 zc = cconj(rb[k == 0 ? 0 : M - k]);
+    const float2 e = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+    const float2 dd = csub(zk, zc);
+    const float2 od = make_float2(0.5f * dd.y, -0.5f * dd.x);
+    S[o + k] = cadd(e, cmul(twr[k], od));
+  }
+}
+
+template <int M>
+__device__ __forceinline__ void x2_load_twiddles(float2* tw, float2* twr, const float2* twM, const float2* wr) {
+  using C = XCfg<M>;
+  for (int i = threadIdx.x; i < M; i += C::NT) tw[i] = twM[i];
+  for (int i = threadIdx.x; i <= M; i += C::NT) twr[i] = wr[i];
+}
+
+// ------------------------------------------------------------------ forward with gather
+template <int M>
+__global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, float2* __restrict__ S,
+                                                         const float2* __restrict__ twM,
+                                                         const float2* __restrict__ wr) {
+  using C = XCfg<M>;
+  __shared__ float2 buf[C::RW * C::RS];
+  __shared__ float2 tw[M];
+  __shared__ float2 twr[M + 1];
+  __shared__ int xmap[C::PX];
+  __shared__ long long srcRow[C::RW];
+  __shared__ size_t dstRow[C::RW];
+  x2_load_twiddles<M>(tw, twr, twM, wr);
+  const int nrows = d.P[1] * d.P[2];
+  const int row0 = blockIdx.x * C::RW;
+  if (g.mode == GATHER_KERNEL) {
+    // kernel placement (setup only): per element
+    float* fb = reinterpret_cast<float*>(buf);
+    for (int idx = threadIdx.x; idx < C::RW * C::PX; idx += C::NT) {
+      const int r = idx / C::PX, x = idx - r * C::PX;
+      const int row = row0 + r;
+      float v = 0.f;
+      if (row < nrows) v = gather_value(g, d, x, row % d.P[1], row / d.P[1]);
+      fb[r * 2 * C::RS + x] = v;
+    }
+    if (threadIdx.x < C::RW) {
+      const int row = row0 + threadIdx.x;
+      dstRow[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : ~(size_t)0;
+    }
+  } else {
+    for (int x = threadIdx.x; x < C::PX; x += C::NT) {
+      const int p = g.org[0] + x;
+      xmap[x] = g.mode == GATHER_MIRROR ? mirror_index(p, g.N[0]) : ((unsigned)p < (unsigned)g.N[0] ? p : -1);
+    }
+    if (threadIdx.x < C::RW) {
+      const int row = row0 + threadIdx.x;
+      long long off = -1;
+      if (row < nrows) {
+        const int y = row % d.P[1], z = row / d.P[1];
+        int gy = g.org[1] + y, gz = g.org[2] + z;
+        if (g.mode == GATHER_MIRROR) {
+          gy = mirror_index(gy, g.N[1]);
+          gz = mirror_index(gz, g.N[2]);
+          off = ((long long)gz * g.N[1] + gy) * g.N[0];
+        } else if ((unsigned)gy < (unsigned)g.N[1] && (unsigned)gz < (unsigned)g.N[2]) {
+          off = ((long long)gz * g.N[1] + gy) * g.N[0];
+        }
+      }
+      srcRow[threadIdx.x] = off;
+      dstRow[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : ~(size_t)0;
+    }
+    __syncthreads();
+    float* fb = reinterpret_cast<float*>(buf);
+    for (int idx = threadIdx.x; idx < C::RW * C::PX; idx += C::NT) {
+      const int r = idx / C::PX, x = idx - r * C::PX;
+      const long long off = srcRow[r];
+      const int gx = xmap[x];
+      fb[r * 2 * C::RS + x] = (off >= 0 && gx >= 0) ? g.src[off + gx] : g.value;
+    }
+  }
+  __syncthreads();
+  const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
+  float2* rb = buf + r * C::RS;
+  float2 z[C::B];
+  x2_forward<M>(rb, tw, q, z);
+  x2_park_forward<M>(rb, q, z);
+  x2_post_store<M>(buf, twr, dstRow, S);
+}
+
+// ------------------------------------------------------------------ inverse, quotient, forward
+// ratio = img > 0 ? img / blurred : 1 on the output region, 1 elsewhere
+// (DeconvolutionMethods.computeQuotient, D/iteration/sequential/DeconvolutionMethods.java:46-98)
+template <int M>
+__global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_, const float2* __restrict__ Sin,
+                                                           float2* __restrict__ Sout,
+                                                           const float2* __restrict__ twM,
+                                                           const float2* __restrict__ wr) {
+  using C = XCfg<M>;
+  __shared__ float2 buf[C::RW * C::RS];
+  __shared__ float2 tw[M];
+  __shared__ float2 twr[M + 1];
+  // img index of FFT position x is imgRow + x (may be negative: x starts at m0); kOuterRow: ratio row of ones
+  __shared__ long long imgRow[C::RW];
+  __shared__ size_t rowOff[C::RW];
+  x2_load_twiddles<M>(tw, twr, twM, wr);
+  const int nrows = d.P[1] * d.P[2];
+  const int row0 = blockIdx.x * C::RW;
+  if (threadIdx.x < C::RW) {
+    const int row = row0 + threadIdx.x;
+    long long ir = kOuterRow;
+    if (row < nrows) {
+      const int ly = row % d.P[1] - d.m[1], lz = row / d.P[1] - d.m[2];
+      if ((unsigned)ly < (unsigned)d.D[1] && (unsigned)lz < (unsigned)d.D[2])
+        ir = ((long long)(q_.o[2] + lz) * q_.N[1] + (q_.o[1] + ly)) * q_.N[0] + q_.o[0] - d.m[0];
+    }
+    imgRow[threadIdx.x] = ir;
+    rowOff[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : ~(size_t)0;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < C::RW * C::RS; idx += C::NT) {
+    const int r = idx / C::RS, k = idx - r * C::RS;
+    buf[idx] = imgRow[r] != kOuterRow ? Sin[rowOff[r] + k] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
+  float2* rb = buf + r * C::RS;
+  float2 z[C::A];
+  x2_inverse_half<M>(rb, tw, twr, q, z);
+  x2_park_inverse<M>(rb, q, z);
+  float* fb = reinterpret_cast<float*>(buf);
+  const int x0 = d.m[0], x1 = d.m[0] + d.D[0];
+  for (int idx = threadIdx.x; idx < C::RW * C::PX; idx += C::NT) {
+    const int rr = idx / C::PX, x = idx - rr * C::PX;
+    const long long ir = imgRow[rr];
+    float rv = 1.f;
+    if (ir != kOuterRow && x >= x0 && x < x1) {
+      const float im = q_.img[ir + x];
+      if (im > 0.f) rv = im / fb[rr * 2 * C::RS + x];
+    }
+    fb[rr * 2 * C::RS + x] = rv;
+  }
+  __syncthreads();
+  float2 zf[C::B];
+  x2_forward<M>(rb, tw, q, zf);
+  x2_park_forward<M>(rb, q, zf);
+  x2_post_store<M>(buf, twr, rowOff, Sout);
+}
+
+// ------------------------------------------------------------------ inverse + psi update (output rows only)
+template <int M>
+__global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u, FastDiv divD0,
+                                                            const float2* __restrict__ S,
+                                                            const float2* __restrict__ twM,
+                                                            const float2* __restrict__ wr) {
+  using C = XCfg<M>;
+  __shared__ float2 buf[C::RW * C::RS];
+  __shared__ float2 tw[M];
+  __shared__ float2 twr[M + 1];
+  __shared__ long long psiRow[C::RW];
+  __shared__ size_t rowOff[C::RW];
+  x2_load_twiddles<M>(tw, twr, twM, wr);
+  const int nrows = d.D[1] * d.D[2];
+  const int row0 = blockIdx.x * C::RW;
+  if (threadIdx.x < C::RW) {
+    const int row = row0 + threadIdx.x;
+    long long pr = -1;
+    size_t so = 0;
+    if (row < nrows) {
+      const int ly = row % d.D[1], lz = row / d.D[1];
+      pr = ((long long)(u.o[2] + lz) * u.N[1] + (u.o[1] + ly)) * u.N[0] + u.o[0];
+      so = ((size_t)(d.m[2] + lz) * d.P[1] + (d.m[1] + ly)) * d.HxP;
+    }
+    psiRow[threadIdx.x] = pr;
+    rowOff[threadIdx.x] = so;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < C::RW * C::RS; idx += C::NT) {
+    const int r = idx / C::RS, k = idx - r * C::RS;
+    buf[idx] = psiRow[r] >= 0 ? S[rowOff[r] + k] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
+  float2* rb = buf + r * C::RS;
+  float2 z[C::A];
+  x2_inverse_half<M>(rb, tw, twr, q, z);
+  x2_park_inverse<M>(rb, q, z);
+  const float* fb = reinterpret_cast<const float*>(buf);
+  double sum = 0.0, mx = -1.0;
+  const int DX = d.D[0];
+  const unsigned total = (unsigned)(C::RW * DX);
+  for (unsigned idx = threadIdx.x; idx < total; idx += C::NT) {
+    const unsigned rr = divD0.div(idx);
+    const int lx = (int)(idx - rr * (unsigned)DX);
+    const long long pr = psiRow[rr];
+    if (pr < 0) continue;
+    const size_t gi = (size_t)(pr + lx);
+    const float last = u.psi[gi];
+    const float next = rl_next_value(last, fb[rr * 2 * C::RS + d.m[0] + lx], u.w[gi], u.lambda, u.minValue, u.maxI);
+    u.psi[gi] = next;
+    const float ch = __fsub_rn(next, last);
+    sum += (double)ch;
+    mx = fmax(mx, (double)ch);
+  }
+  block_reduce_stats_nt<C::NT>(sum, mx, u.partial + blockIdx.x);
+}
+
+// ------------------------------------------------------------------ inverse + store of the output region
+template <int M>
+__global__ __launch_bounds__(XCfg<M>::NT) void k_x2_store(Dom d, StoreParams st, FastDiv divD0,
+                                                           const float2* __restrict__ S,
+                                                           const float2* __restrict__ twM,
+                                                           const float2* __restrict__ wr) {
+  using C = XCfg<M>;
+  __shared__ float2 buf[C::RW * C::RS];
+  __shared__ float2 tw[M];
+  __shared__ float2 twr[M + 1];
+  __shared__ size_t rowOff[C::RW];
+  x2_load_twiddles<M>(tw, twr, twM, wr);
+  const int nrows = d.D[1] * d.D[2];
+  const int row0 = blockIdx.x * C::RW;
+  if (threadIdx.x < C::RW) {
+    const int row = row0 + threadIdx.x;
+    size_t so = ~(size_t)0;
+    if (row < nrows) {
+      const int ly = row % d.D[1], lz = row / d.D[1];
+      so = ((size_t)(d.m[2] + lz) * d.P[1] + (d.m[1] + ly)) * d.HxP;
+    }
+    rowOff[threadIdx.x] = so;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < C::RW * C::RS; idx += C::NT) {
+    const int r = idx / C::RS, k = idx - r * C::RS;
+    buf[idx] = rowOff[r] != ~(size_t)0 ? S[rowOff[r] + k] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
+  float2* rb = buf + r * C::RS;
+  float2 z[C::A];
+  x2_inverse_half<M>(rb, tw, twr, q, z);
+  x2_park_inverse<M>(rb, q, z);
+  const float* fb = reinterpret_cast<const float*>(buf);
+  const int DX = d.D[0];
+  const unsigned total = (unsigned)(C::RW * DX);
+  for (unsigned idx = threadIdx.x; idx < total; idx += C::NT) {
+    const unsigned rr = divD0.div(idx);
+    const int lx = (int)(idx - rr * (unsigned)DX);
+    const int row = row0 + (int)rr;
+    if (row < nrows) st.out[(size_t)row * DX + lx] = fb[rr * 2 * C::RS + d.m[0] + lx];
+  }
+}
+
+// ------------------------------------------------------------------ host launchers (two-step when M splits)
+template <int M>
+void launch_x_fwd(const Dom& d, const GatherParams& g, float2* S, const float2* twM, const float2* wr,
+                  hipStream_t s) {
+  if constexpr (x2_ok<M>()) {
+    using C = XCfg<M>;
+    const int rows = d.P[1] * d.P[2];
+    hipLaunchKernelGGL(k_x2_fwd<M>, dim3((rows + C::RW - 1) / C::RW), dim3(C::NT), 0, s, d, g, S, twM, wr);
+  } else {
+    launch_x1_fwd<M>(d, g, S, twM, wr, s);
+  }
+}
+template <int M>
+void launch_x_ratio(const Dom& d, const RatioParams& q, const float2* Sin, float2* Sout, const float2* twM,
+                    const float2* wr, hipStream_t s) {
+  if constexpr (x2_ok<M>()) {
+    using C = XCfg<M>;
+    const int rows = d.P[1] * d.P[2];
+    hipLaunchKernelGGL(k_x2_ratio<M>, dim3((rows + C::RW - 1) / C::RW), dim3(C::NT), 0, s, d, q, Sin, Sout, twM,
+                       wr);
+  } else {
+    launch_x1_ratio<M>(d, q, Sin, Sout, twM, wr, s);
+  }
+}
+template <int M>
+int launch_x_update(const Dom& d, const UpdateParams& u, const float2* S, const float2* twM, const float2* wr,
+                    hipStream_t s) {
+  if constexpr (x2_ok<M>()) {
+    using C = XCfg<M>;
+    const int rows = d.D[1] * d.D[2];
+    const int nb = (rows + C::RW - 1) / C::RW;
+    hipLaunchKernelGGL(k_x2_update<M>, dim3(nb), dim3(C::NT), 0, s, d, u, FastDiv((unsigned)d.D[0]), S, twM, wr);
+    return nb;
+  } else {
+    return launch_x1_update<M>(d, u, S, twM, wr, s);
+  }
+}
+template <int M>
+void launch_x_store(const Dom& d, const StoreParams& st, const float2* S, const float2* twM, const float2* wr,
+                    hipStream_t s) {
+  if constexpr (x2_ok<M>()) {
+    using C = XCfg<M>;
+    const int rows = d.D[1] * d.D[2];
+    hipLaunchKernelGGL(k_x2_store<M>, dim3((rows + C::RW - 1) / C::RW), dim3(C::NT), 0, s, d, st,
+                       FastDiv((unsigned)d.D[0]), S, twM, wr);
+  } else {
+    launch_x1_store<M>(d, st, S, twM, wr, s);
+  }
+}
+
+}  // namespace mvr
