@@ -49,6 +49,10 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-views", type=int, default=1)
+    ap.add_argument("--iterations", type=int, default=0,
+                    help="override the workload's iteration count (short runs under a profiler)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this size (counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this path "
@@ -60,6 +64,19 @@ def main():
     from mvr import synthetic as syn
 
     w = syn.C2 if args.workload == "c2" else syn.C1
+    if args.iterations > 0:
+        import dataclasses
+        w = dataclasses.replace(w, iterations=args.iterations)
+    if args.calibrate_copy_mib > 0:
+        # plain streaming kernels of known size: a fill (writes N bytes) and an elementwise
+        # multiply (reads N, writes N), N = calibrate_copy_mib MiB
+        import torch
+        src = torch.ones(args.calibrate_copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 2.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     t0 = time.time()
     inp = syn.make_inputs(w)
     log(f"inputs generated in {time.time() - t0:.1f}s")

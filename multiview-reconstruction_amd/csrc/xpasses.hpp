@@ -141,7 +141,7 @@ __device__ __forceinline__ void x2_post_store(const float2* buf, const float2* t
   for (int idx = threadIdx.x; idx < C::RW * (M + 1); idx += C::NT) {
     const int r = idx / (M + 1), k = idx - r * (M + 1);
     const size_t o = dstRow[r];
-    if (o == ~(size_t)0) continue;
+    if (o == ~(size_t)0) continue;  // no output row
     const float2* rb = buf + r * C::RS;
     const float2 zk = rb[k == M ? 0 : k];
     const float2 zc = cconj(rb[k == 0 ? 0 : M - k]);
@@ -159,6 +159,38 @@ __device__ __forceinline__ void x2_load_twiddles(float2* tw, float2* twr, const 
   for (int i = threadIdx.x; i <= M; i += C::NT) twr[i] = wr[i];
 }
 
+// ------------------------------------------------------------------ pipelined HBM <-> LDS staging
+// Every loop below has a compile-time trip count and issues all of its global
+// loads before it consumes any of them (lanes without data read a valid dummy
+// address and discard the value), so each wave keeps many loads in flight.
+constexpr size_t kNoRow = ~(size_t)0;
+
+// rows of M+1 complex from S into LDS rows of stride RS; rows with offset kNoRow become zero
+template <int M>
+__device__ __forceinline__ void x2_stage_spectrum(float2* buf, const float2* __restrict__ S, const size_t* rowOff) {
+  using C = XCfg<M>;
+  constexpr int TOT = C::RW * (M + 1);
+  constexpr int IT = (TOT + C::NT - 1) / C::NT;
+  float2 v[IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int idx = threadIdx.x + it * C::NT;
+    const int r = min(idx / (M + 1), C::RW - 1), k = idx - r * (M + 1);
+    const size_t o = rowOff[r];
+    const bool ok = idx < TOT && o != kNoRow;
+    v[it] = S[ok ? o + k : 0];
+    if (!ok) v[it] = make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int idx = threadIdx.x + it * C::NT;
+    if (idx < TOT) {
+      const int r = idx / (M + 1), k = idx - r * (M + 1);
+      buf[r * C::RS + k] = v[it];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ forward with gather
 template <int M>
 __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, float2* __restrict__ S,
@@ -174,9 +206,9 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
   x2_load_twiddles<M>(tw, twr, twM, wr);
   const int nrows = d.P[1] * d.P[2];
   const int row0 = blockIdx.x * C::RW;
+  float* fb = reinterpret_cast<float*>(buf);
   if (g.mode == GATHER_KERNEL) {
     // kernel placement (setup only): per element
-    float* fb = reinterpret_cast<float*>(buf);
     for (int idx = threadIdx.x; idx < C::RW * C::PX; idx += C::NT) {
       const int r = idx / C::PX, x = idx - r * C::PX;
       const int row = row0 + r;
@@ -186,7 +218,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
     }
     if (threadIdx.x < C::RW) {
       const int row = row0 + threadIdx.x;
-      dstRow[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : ~(size_t)0;
+      dstRow[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : kNoRow;
     }
   } else {
     for (int x = threadIdx.x; x < C::PX; x += C::NT) {
@@ -207,16 +239,30 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
           off = ((long long)gz * g.N[1] + gy) * g.N[0];
         }
       }
-      srcRow[threadIdx.x] = off;
-      dstRow[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : ~(size_t)0;
+      srcRow[threadIdx.x] = off;   // row start in src (>= 0), -1: row outside the volume
+      dstRow[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : kNoRow;
     }
     __syncthreads();
-    float* fb = reinterpret_cast<float*>(buf);
-    for (int idx = threadIdx.x; idx < C::RW * C::PX; idx += C::NT) {
-      const int r = idx / C::PX, x = idx - r * C::PX;
+    constexpr int TOT = C::RW * C::PX;
+    constexpr int IT = (TOT + C::NT - 1) / C::NT;
+    float v[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int idx = threadIdx.x + it * C::NT;
+      const int r = min(idx / C::PX, C::RW - 1), x = idx - r * C::PX;
       const long long off = srcRow[r];
-      const int gx = xmap[x];
-      fb[r * 2 * C::RS + x] = (off >= 0 && gx >= 0) ? g.src[off + gx] : g.value;
+      const int gx = xmap[min(x, C::PX - 1)];
+      const bool ok = idx < TOT && off >= 0 && gx >= 0;
+      v[it] = g.src[ok ? off + gx : 0];
+      if (!ok) v[it] = g.value;
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int idx = threadIdx.x + it * C::NT;
+      if (idx < TOT) {
+        const int r = idx / C::PX, x = idx - r * C::PX;
+        fb[r * 2 * C::RS + x] = v[it];
+      }
     }
   }
   __syncthreads();
@@ -242,7 +288,8 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
   __shared__ float2 twr[M + 1];
   // img index of FFT position x is imgRow + x (may be negative: x starts at m0); kOuterRow: ratio row of ones
   __shared__ long long imgRow[C::RW];
-  __shared__ size_t rowOff[C::RW];
+  __shared__ size_t inOff[C::RW];    // spectrum row to read (kNoRow: outer row, not read)
+  __shared__ size_t outOff[C::RW];   // spectrum row to write
   x2_load_twiddles<M>(tw, twr, twM, wr);
   const int nrows = d.P[1] * d.P[2];
   const int row0 = blockIdx.x * C::RW;
@@ -255,13 +302,11 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
         ir = ((long long)(q_.o[2] + lz) * q_.N[1] + (q_.o[1] + ly)) * q_.N[0] + q_.o[0] - d.m[0];
     }
     imgRow[threadIdx.x] = ir;
-    rowOff[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : ~(size_t)0;
+    inOff[threadIdx.x] = ir != kOuterRow ? (size_t)row * d.HxP : kNoRow;
+    outOff[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : kNoRow;
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < C::RW * C::RS; idx += C::NT) {
-    const int r = idx / C::RS, k = idx - r * C::RS;
-    buf[idx] = imgRow[r] != kOuterRow ? Sin[rowOff[r] + k] : make_float2(0.f, 0.f);
-  }
+  x2_stage_spectrum<M>(buf, Sin, inOff);
   __syncthreads();
   const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
   float2* rb = buf + r * C::RS;
@@ -269,22 +314,35 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
   x2_inverse_half<M>(rb, tw, twr, q, z);
   x2_park_inverse<M>(rb, q, z);
   float* fb = reinterpret_cast<float*>(buf);
-  const int x0 = d.m[0], x1 = d.m[0] + d.D[0];
-  for (int idx = threadIdx.x; idx < C::RW * C::PX; idx += C::NT) {
-    const int rr = idx / C::PX, x = idx - rr * C::PX;
-    const long long ir = imgRow[rr];
-    float rv = 1.f;
-    if (ir != kOuterRow && x >= x0 && x < x1) {
-      const float im = q_.img[ir + x];
-      if (im > 0.f) rv = im / fb[rr * 2 * C::RS + x];
+  {
+    const int x0 = d.m[0], x1 = d.m[0] + d.D[0];
+    constexpr int TOT = C::RW * C::PX;
+    constexpr int IT = (TOT + C::NT - 1) / C::NT;
+    float im[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int idx = threadIdx.x + it * C::NT;
+      const int rr = min(idx / C::PX, C::RW - 1), x = idx - rr * C::PX;
+      const long long ir = imgRow[rr];
+      const bool ok = idx < TOT && ir != kOuterRow && x >= x0 && x < x1;
+      im[it] = q_.img[ok ? ir + x : 0];
+      if (!ok) im[it] = 0.f;
     }
-    fb[rr * 2 * C::RS + x] = rv;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int idx = threadIdx.x + it * C::NT;
+      if (idx < TOT) {
+        const int rr = idx / C::PX, x = idx - rr * C::PX;
+        float* p = fb + rr * 2 * C::RS + x;
+        *p = im[it] > 0.f ? im[it] / *p : 1.f;
+      }
+    }
   }
   __syncthreads();
   float2 zf[C::B];
   x2_forward<M>(rb, tw, q, zf);
   x2_park_forward<M>(rb, q, zf);
-  x2_post_store<M>(buf, twr, rowOff, Sout);
+  x2_post_store<M>(buf, twr, outOff, Sout);
 }
 
 // ------------------------------------------------------------------ inverse + psi update (output rows only)
@@ -305,20 +363,17 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
   if (threadIdx.x < C::RW) {
     const int row = row0 + threadIdx.x;
     long long pr = -1;
-    size_t so = 0;
+    size_t so = kNoRow;
     if (row < nrows) {
       const int ly = row % d.D[1], lz = row / d.D[1];
       pr = ((long long)(u.o[2] + lz) * u.N[1] + (u.o[1] + ly)) * u.N[0] + u.o[0];
       so = ((size_t)(d.m[2] + lz) * d.P[1] + (d.m[1] + ly)) * d.HxP;
     }
-    psiRow[threadIdx.x] = pr;
+    psiRow[threadIdx.x] = pr;   // >= 0 for output rows (the output region lies inside the volume)
     rowOff[threadIdx.x] = so;
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < C::RW * C::RS; idx += C::NT) {
-    const int r = idx / C::RS, k = idx - r * C::RS;
-    buf[idx] = psiRow[r] >= 0 ? S[rowOff[r] + k] : make_float2(0.f, 0.f);
-  }
+  x2_stage_spectrum<M>(buf, S, rowOff);
   __syncthreads();
   const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
   float2* rb = buf + r * C::RS;
@@ -329,18 +384,35 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
   double sum = 0.0, mx = -1.0;
   const int DX = d.D[0];
   const unsigned total = (unsigned)(C::RW * DX);
-  for (unsigned idx = threadIdx.x; idx < total; idx += C::NT) {
-    const unsigned rr = divD0.div(idx);
-    const int lx = (int)(idx - rr * (unsigned)DX);
-    const long long pr = psiRow[rr];
-    if (pr < 0) continue;
-    const size_t gi = (size_t)(pr + lx);
-    const float last = u.psi[gi];
-    const float next = rl_next_value(last, fb[rr * 2 * C::RS + d.m[0] + lx], u.w[gi], u.lambda, u.minValue, u.maxI);
-    u.psi[gi] = next;
-    const float ch = __fsub_rn(next, last);
-    sum += (double)ch;
-    mx = fmax(mx, (double)ch);
+  constexpr int U = 4;
+  for (unsigned base = threadIdx.x; base < total; base += U * C::NT) {
+    float last[U], wt[U];
+    size_t gi[U];
+    unsigned rr[U];
+    bool ok[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const unsigned idx = base + j * C::NT;
+      rr[j] = min(divD0.div(idx), (unsigned)C::RW - 1);
+      const int lx = (int)(idx - rr[j] * (unsigned)DX);
+      const long long pr = psiRow[rr[j]];
+      ok[j] = idx < total && pr >= 0;
+      gi[j] = ok[j] ? (size_t)(pr + lx) : 0;
+      last[j] = u.psi[gi[j]];
+      wt[j] = u.w[gi[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      if (!ok[j]) continue;
+      const unsigned idx = base + j * C::NT;
+      const int lx = (int)(idx - rr[j] * (unsigned)DX);
+      const float next = rl_next_value(last[j], fb[rr[j] * 2 * C::RS + d.m[0] + lx], wt[j], u.lambda, u.minValue,
+                                       u.maxI);
+      u.psi[gi[j]] = next;
+      const float ch = __fsub_rn(next, last[j]);
+      sum += (double)ch;
+      mx = fmax(mx, (double)ch);
+    }
   }
   block_reduce_stats_nt<C::NT>(sum, mx, u.partial + blockIdx.x);
 }
@@ -361,7 +433,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_store(Dom d, StoreParams st,
   const int row0 = blockIdx.x * C::RW;
   if (threadIdx.x < C::RW) {
     const int row = row0 + threadIdx.x;
-    size_t so = ~(size_t)0;
+    size_t so = kNoRow;
     if (row < nrows) {
       const int ly = row % d.D[1], lz = row / d.D[1];
       so = ((size_t)(d.m[2] + lz) * d.P[1] + (d.m[1] + ly)) * d.HxP;
@@ -369,10 +441,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_store(Dom d, StoreParams st,
     rowOff[threadIdx.x] = so;
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < C::RW * C::RS; idx += C::NT) {
-    const int r = idx / C::RS, k = idx - r * C::RS;
-    buf[idx] = rowOff[r] != ~(size_t)0 ? S[rowOff[r] + k] : make_float2(0.f, 0.f);
-  }
+  x2_stage_spectrum<M>(buf, S, rowOff);
   __syncthreads();
   const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
   float2* rb = buf + r * C::RS;
