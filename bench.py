@@ -30,11 +30,11 @@ HBM_PEAK_GBS = 8000.0
 
 # algorithmic HBM bytes per FFT-extent voxel for each pass (complex half spectrum = 4 B per real voxel):
 #   x_fwd   : psi gather 4 + spectrum write 4
-#   y_fwd/y_inv: spectrum read 4 + write 4
-#   z_conv  : spectrum read 4 + kernel spectrum read 4 + write 4
+#   z_fwd/z_inv: spectrum read 4 + write 4
+#   y_conv  : spectrum read 4 + kernel spectrum read 4 + write 4
 #   x_ratio : spectrum read 4 + view image 4 + spectrum write 4
 #   x_update: spectrum read 4 + psi read 4 + weight 4 + psi write 4
-PASS_BYTES = {"x_fwd": 8, "y_fwd": 8, "z_conv": 12, "y_inv": 8, "x_ratio": 12, "x_update": 16, "x_store": 8}
+PASS_BYTES = {"x_fwd": 8, "z_fwd": 8, "y_conv": 12, "z_inv": 8, "x_ratio": 12, "x_update": 16, "x_store": 8}
 
 
 def log(*a):

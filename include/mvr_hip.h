@@ -120,7 +120,7 @@ int mvr_decon_reset_psi(mvr_decon* d);
 /* FFT geometry: out[0..2] = FFT lengths, out[3] = half-spectrum row pitch */
 int mvr_decon_get_geometry(mvr_decon* d, int out[4]);
 /* per-pass GPU timing with HIP events recorded around each launch on the library's stream.  mask bit c
- * enables category c (0 x_fwd, 1 y_fwd, 2 z_conv, 3 y_inv, 4 x_ratio, 5 x_update, 6 x_store,
+ * enables category c (0 x_fwd, 1 z_fwd, 2 y_conv, 3 z_inv, 4 x_ratio, 5 x_update, 6 x_store,
  * 7 update_mul, 8 reduce); 0 disables.  Totals are cleared by every call. */
 int mvr_decon_profile(mvr_decon* d, unsigned mask);
 int mvr_decon_pass_times(mvr_decon* d, double msTotal[9], long long launches[9]);

@@ -174,14 +174,14 @@ int Decon::view_seq(int v, int statIdx) {
   timer.begin(stream, PASS_X_FWD);
   plan.xf(d, g, S, plan.twM, plan.wr, stream);
   timer.end(stream);
-  timer.begin(stream, PASS_Y_FWD);
-  plan.sy(gy, 0, S, S, nullptr, 1.f, plan.twY, stream);
+  timer.begin(stream, PASS_Z_FWD);
+  plan.sz(gz, 0, S, S, nullptr, 1.f, plan.twZ, stream);
   timer.end(stream);
-  timer.begin(stream, PASS_Z_CONV);
-  plan.sz(gz, 2, S, S, spec1[v], 1.f, plan.twZ, stream);
+  timer.begin(stream, PASS_Y_CONV);
+  plan.sy(gy, 2, S, S, spec1[v], 1.f, plan.twY, stream);
   timer.end(stream);
-  timer.begin(stream, PASS_Y_INV);
-  plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+  timer.begin(stream, PASS_Z_INV);
+  plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
   timer.end(stream);
   RatioParams q{};
   q.img = img[v];
@@ -189,14 +189,14 @@ int Decon::view_seq(int v, int statIdx) {
   timer.begin(stream, PASS_X_RATIO);
   plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
   timer.end(stream);
-  timer.begin(stream, PASS_Y_FWD);
-  plan.sy(gy, 0, S, S, nullptr, 1.f, plan.twY, stream);
+  timer.begin(stream, PASS_Z_FWD);
+  plan.sz(gz, 0, S, S, nullptr, 1.f, plan.twZ, stream);
   timer.end(stream);
-  timer.begin(stream, PASS_Z_CONV);
-  plan.sz(gz, 2, S, S, spec2[v], 1.f, plan.twZ, stream);
+  timer.begin(stream, PASS_Y_CONV);
+  plan.sy(gy, 2, S, S, spec2[v], 1.f, plan.twY, stream);
   timer.end(stream);
-  timer.begin(stream, PASS_Y_INV);
-  plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+  timer.begin(stream, PASS_Z_INV);
+  plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
   timer.end(stream);
   UpdateParams u{};
   u.psi = psi;
@@ -228,16 +228,16 @@ int Decon::iteration_mul(int statIdx) {
   timer.begin(stream, PASS_X_FWD);
   plan.xf(d, g, Spsi, plan.twM, plan.wr, stream);
   timer.end(stream);
-  timer.begin(stream, PASS_Y_FWD);
-  plan.sy(gy, 0, Spsi, Spsi, nullptr, 1.f, plan.twY, stream);
+  timer.begin(stream, PASS_Z_FWD);
+  plan.sz(gz, 0, Spsi, Spsi, nullptr, 1.f, plan.twZ, stream);
   timer.end(stream);
   std::vector<const float*> t2p(V), wp(V);
   for (int v = 0; v < V; ++v) {
-    timer.begin(stream, PASS_Z_CONV);
-    plan.sz(gz, 2, Spsi, S, spec1[v], 1.f, plan.twZ, stream);
+    timer.begin(stream, PASS_Y_CONV);
+    plan.sy(gy, 2, Spsi, S, spec1[v], 1.f, plan.twY, stream);
     timer.end(stream);
-    timer.begin(stream, PASS_Y_INV);
-    plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+    timer.begin(stream, PASS_Z_INV);
+    plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
     timer.end(stream);
     RatioParams q{};
     q.img = img[v];
@@ -245,14 +245,14 @@ int Decon::iteration_mul(int statIdx) {
     timer.begin(stream, PASS_X_RATIO);
     plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
     timer.end(stream);
-    timer.begin(stream, PASS_Y_FWD);
-    plan.sy(gy, 0, S, S, nullptr, 1.f, plan.twY, stream);
+    timer.begin(stream, PASS_Z_FWD);
+    plan.sz(gz, 0, S, S, nullptr, 1.f, plan.twZ, stream);
     timer.end(stream);
-    timer.begin(stream, PASS_Z_CONV);
-    plan.sz(gz, 2, S, S, spec2[v], 1.f, plan.twZ, stream);
+    timer.begin(stream, PASS_Y_CONV);
+    plan.sy(gy, 2, S, S, spec2[v], 1.f, plan.twY, stream);
     timer.end(stream);
-    timer.begin(stream, PASS_Y_INV);
-    plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+    timer.begin(stream, PASS_Z_INV);
+    plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
     timer.end(stream);
     StoreParams st{T2 + nvox() * v};
     timer.begin(stream, PASS_X_STORE);
@@ -397,16 +397,16 @@ int BlockThread::run_seq(int v, float* psiBlock, const float* imgBlock, const fl
   for (int k = 0; k < 3; ++k) { g.N[k] = B[k]; g.org[k] = -d.m[k]; }
   g.mode = GATHER_MIRROR;
   plan.xf(d, g, S, plan.twM, plan.wr, stream);
-  plan.sy(gy, 0, S, S, nullptr, 1.f, plan.twY, stream);
-  plan.sz(gz, 2, S, S, spec1[v], 1.f, plan.twZ, stream);
-  plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+  plan.sz(gz, 0, S, S, nullptr, 1.f, plan.twZ, stream);
+  plan.sy(gy, 2, S, S, spec1[v], 1.f, plan.twY, stream);
+  plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
   RatioParams q{};
   q.img = imgB;
   for (int k = 0; k < 3; ++k) { q.N[k] = B[k]; q.o[k] = 0; }
   plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
-  plan.sy(gy, 0, S, S, nullptr, 1.f, plan.twY, stream);
-  plan.sz(gz, 2, S, S, spec2[v], 1.f, plan.twZ, stream);
-  plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+  plan.sz(gz, 0, S, S, nullptr, 1.f, plan.twZ, stream);
+  plan.sy(gy, 2, S, S, spec2[v], 1.f, plan.twY, stream);
+  plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
   if (ensure_partials((int)(plan.rows_inner() / 4 + 1))) return -1;
   UpdateParams u{};
   u.psi = psiB;
@@ -454,18 +454,18 @@ int BlockThread::run_mul(int nv, float* psiBlock, const float* const* imgBlocks,
   for (int k = 0; k < 3; ++k) { g.N[k] = B[k]; g.org[k] = -d.m[k]; }
   g.mode = GATHER_MIRROR;
   plan.xf(d, g, Spsi, plan.twM, plan.wr, stream);
-  plan.sy(gy, 0, Spsi, Spsi, nullptr, 1.f, plan.twY, stream);
+  plan.sz(gz, 0, Spsi, Spsi, nullptr, 1.f, plan.twZ, stream);
   std::vector<const float*> t2p(nv), wp(nv);
   for (int v = 0; v < nv; ++v) {
-    plan.sz(gz, 2, Spsi, S, spec1[v], 1.f, plan.twZ, stream);
-    plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+    plan.sy(gy, 2, Spsi, S, spec1[v], 1.f, plan.twY, stream);
+    plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
     RatioParams q{};
     q.img = imgV[v];
     for (int k = 0; k < 3; ++k) { q.N[k] = B[k]; q.o[k] = 0; }
     plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
-    plan.sy(gy, 0, S, S, nullptr, 1.f, plan.twY, stream);
-    plan.sz(gz, 2, S, S, spec2[v], 1.f, plan.twZ, stream);
-    plan.sy(gy, 1, S, S, nullptr, 1.f, plan.twY, stream);
+    plan.sz(gz, 0, S, S, nullptr, 1.f, plan.twZ, stream);
+    plan.sy(gy, 2, S, S, spec2[v], 1.f, plan.twY, stream);
+    plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
     StoreParams st{T2 + nb * v};
     plan.xs(d, st, S, plan.twM, plan.wr, stream);
     t2p[v] = T2 + nb * v;

@@ -65,7 +65,7 @@ enum { PSF_OPT_II = 0, PSF_OPT_I = 1, PSF_EFFICIENT_BAYESIAN = 2, PSF_INDEPENDEN
 enum { SCHEME_SEQ = 0, SCHEME_MUL = 1 };
 
 // per-pass timing categories (resident driver)
-enum Pass { PASS_X_FWD = 0, PASS_Y_FWD, PASS_Z_CONV, PASS_Y_INV, PASS_X_RATIO, PASS_X_UPDATE, PASS_X_STORE,
+enum Pass { PASS_X_FWD = 0, PASS_Z_FWD, PASS_Y_CONV, PASS_Z_INV, PASS_X_RATIO, PASS_X_UPDATE, PASS_X_STORE,
             PASS_UPDATE_MUL, PASS_REDUCE, PASS_COUNT };
 
 struct Timer {

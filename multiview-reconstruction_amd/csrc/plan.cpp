@@ -99,9 +99,9 @@ int Plan::kernel_spectrum(const float* dKernel, float2* spec, hipStream_t s) con
 
 int Plan::convolve(const GatherParams& g, const float2* spec, float2* work, float* out, hipStream_t s) const {
   xf(dom, g, work, twM, wr, s);
-  sy(geom_y(), 0, work, work, nullptr, 1.f, twY, s);
-  sz(geom_z(), 2, work, work, spec, 1.f, twZ, s);
-  sy(geom_y(), 1, work, work, nullptr, 1.f, twY, s);
+  sz(geom_z(), 0, work, work, nullptr, 1.f, twZ, s);
+  sy(geom_y(), 2, work, work, spec, 1.f, twY, s);
+  sz(geom_z(), 1, work, work, nullptr, 1.f, twZ, s);
   StoreParams st{out};
   xs(dom, st, work, twM, wr, s);
   MVR_HIP(hipGetLastError());

@@ -17,7 +17,7 @@ HEADER = os.path.join(os.path.dirname(PKG), "include", "mvr_hip.h")
 
 PSF_TYPES = {"OPTIMIZATION_II": 0, "OPTIMIZATION_I": 1, "EFFICIENT_BAYESIAN": 2, "INDEPENDENT": 3}
 SCHEMES = {"SEQ": 0, "MUL": 1}
-PASSES = ["x_fwd", "y_fwd", "z_conv", "y_inv", "x_ratio", "x_update", "x_store", "update_mul", "reduce"]
+PASSES = ["x_fwd", "z_fwd", "y_conv", "z_inv", "x_ratio", "x_update", "x_store", "update_mul", "reduce"]
 
 _lib = None
 _f32p = ctypes.POINTER(ctypes.c_float)
