@@ -51,6 +51,9 @@ def main():
     ap.add_argument("--cpu-sample-views", type=int, default=1)
     ap.add_argument("--iterations", type=int, default=0,
                     help="override the workload's iteration count (short runs under a profiler)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the time-to-convergence comparison")
+    ap.add_argument("--tol", type=float, default=1e-2)
+    ap.add_argument("--reference-iterations", type=int, default=200)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this size (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -128,6 +131,10 @@ def main():
 
     psi = d.psi()
     finite = bool(np.isfinite(psi).all())
+
+    learned = None
+    if not args.no_learned:
+        learned = time_to_converge(d, w, args.tol, args.reference_iterations)
     d.close()
 
     cpu = None
@@ -159,7 +166,65 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
+    if learned:
+        out["learned_init"] = learned
     print(json.dumps(out))
+
+
+def unet_flops(shape_zyx):
+    """Multiply-adds x 2 of one U-Net inference (3^3 layers at their level's voxel count, padded channels)."""
+    from mvr import hip
+    nz, ny, nx = ((s + 3) // 4 * 4 for s in shape_zyx)
+    v0 = nz * ny * nx
+    level = (0, 0, 1, 1, 2, 2, 1, 1, 0, 0)
+    f = 0
+    for l in range(10):
+        f += 2 * 27 * hip.UNet.LAYERS_CIN[l] * hip.UNet.LAYERS_COUT[l] * v0 / 8 ** level[l]
+    return f + 2 * 16 * v0
+
+
+def time_to_converge(d, w, tol, ref_iterations):
+    """Wall time to rel-L2 <= tol to plain RL run to ref_iterations: plain PsiInitAvgPrecise start vs the
+    learned start (U-Net inference included).  The common set-up (PSFs, spectra, AvgPrecise pass) is
+    outside both timed regions."""
+    from mvr import learned as L
+    d.reset_psi()
+    d.run(ref_iterations)
+    d.set_reference(d.psi())
+    d.reset_psi()
+    rel0_plain = d.rel_l2()
+    t0 = time.perf_counter()
+    it_p, rel_p = d.run_until(tol, ref_iterations)
+    t_plain = time.perf_counter() - t0
+    res = {"tol": tol, "reference": f"plain RL, {ref_iterations} iterations from PsiInitAvgPrecise",
+           "plain": {"iterations": it_p, "seconds": t_plain, "rel_l2_start": rel0_plain, "rel_l2_end": rel_p}}
+    if not os.path.exists(L.DEFAULT_WEIGHTS):
+        res["learned"] = None
+        res["note"] = "no trained weights file"
+        return res
+    net = L.load()
+    d.init_learned(net)  # warm-up: activation buffers
+    best = None
+    for _ in range(2):
+        t0 = time.perf_counter()
+        d.init_learned(net)
+        t_inf = time.perf_counter() - t0
+        rel0 = d.rel_l2()
+        t1 = time.perf_counter()
+        it_l, rel_l = d.run_until(tol, ref_iterations)
+        t_it = time.perf_counter() - t1
+        cur = {"iterations": it_l, "seconds": t_inf + t_it, "inference_seconds": t_inf, "rel_l2_start": rel0,
+               "rel_l2_end": rel_l}
+        if best is None or cur["seconds"] < best["seconds"]:
+            best = cur
+    net.close()
+    fl = unet_flops(w.shape_zyx)
+    best["inference_tflops"] = fl / best["inference_seconds"] / 1e12
+    best["inference_mfma_frac_bf16_dense"] = best["inference_tflops"] / 2500.0
+    res["learned"] = best
+    res["speedup"] = t_plain / best["seconds"]
+    res["weights"] = os.path.relpath(L.DEFAULT_WEIGHTS, REPO)
+    return res
 
 
 if __name__ == "__main__":

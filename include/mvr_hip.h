@@ -126,6 +126,45 @@ int mvr_decon_profile(mvr_decon* d, unsigned mask);
 int mvr_decon_pass_times(mvr_decon* d, double msTotal[9], long long launches[9]);
 void mvr_decon_destroy(mvr_decon* d);
 
+/* ---- convergence checks against a reference volume (FP64 device reductions).  The reference keeps
+ * no iteration-wise distance; its IterationStatistics (deconvolution/DeconvolutionMethods.java:308)
+ * are returned by mvr_decon_get_stats.  The reference volume is typically a long plain RL run. */
+int mvr_decon_set_reference(mvr_decon* d, const float* ref);
+/* *out = ||psi - ref||_2 / ||ref||_2 */
+int mvr_decon_rel_l2(mvr_decon* d, double* out);
+/* run iterations until rel-L2 <= tol (checked after each iteration) or maxIterations are done */
+int mvr_decon_run_until(mvr_decon* d, int maxIterations, double tol, int* iterationsOut, double* relOut);
+
+/* ---- learned initial estimate: a new PsiInit (deconvolution/init/PsiInit.java:32-47) built like
+ * PsiInitFromRAI (deconvolution/init/PsiInitFromRAI.java:65-82): psi comes from the network, max[]
+ * and avg from the AvgPrecise pass.  The network (3-level 3D U-Net, 16/32/64 channels, 3^3
+ * convolutions, ReLU; BF16 on MFMA with FP32 accumulation) and its training have no counterpart in
+ * the reference.  Its inputs are the V weighted views and their mean; its output is psi. */
+typedef struct mvr_unet mvr_unet;
+int mvr_unet_create(mvr_unet** out, int dev, int views, unsigned long long seed);
+/* number of FP32 parameters; get/set use one flat array in layer order (per conv layer the weights
+ * [27][cout][cin] then the bias [cout]; last the 1x1 head: 16 weights and its bias) */
+long long mvr_unet_param_count(mvr_unet* u);
+int mvr_unet_get_params(mvr_unet* u, float* out);
+int mvr_unet_set_params(mvr_unet* u, const float* in);
+/* gradients of the last mvr_unet_train_step, same layout as the parameters */
+int mvr_unet_get_grads(mvr_unet* u, float* out);
+/* copy one training volume (V views, V weights, target psi, dims n) to the device; returns its index */
+int mvr_unet_add_volume(mvr_unet* u, const float* const* imgs, const float* const* weights, const float* target,
+                        const int n[3]);
+int mvr_unet_clear_volumes(mvr_unet* u);
+/* one step on batch patches of patch^3 voxels (volume volIdx[b], origin origins[3b..3b+2] as {x, y, z}):
+ * forward, mean squared error to the normalised target in *loss, backward; Adam step when update != 0 */
+int mvr_unet_train_step(mvr_unet* u, const int* volIdx, const int* origins, int batch, int patch, float lr,
+                        int update, double* loss);
+/* psiOut[n] = max(minValue, network estimate) for the given views and weights */
+int mvr_unet_infer(mvr_unet* u, const float* const* imgs, const float* const* weights, const int n[3],
+                   float minValue, float* psiOut);
+void mvr_unet_destroy(mvr_unet* u);
+/* mvr_decon_init with the network's estimate as psi (and as the psi restored by mvr_decon_reset_psi);
+ * on an initialised handle only psi is replaced (PSFs, spectra, max[] and avg are kept) */
+int mvr_decon_init_learned(mvr_decon* d, mvr_unet* net);
+
 const char* mvr_last_error(void);
 
 #ifdef __cplusplus

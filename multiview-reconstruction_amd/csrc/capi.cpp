@@ -5,11 +5,13 @@
 
 #include "../../include/mvr_hip.h"
 #include "host.hpp"
+#include "unet_host.hpp"
 
 using namespace mvr;
 
 struct mvr_block_thread { BlockThread t; };
 struct mvr_decon { Decon d; };
+struct mvr_unet { UNet n; };
 
 #define NULLCHECK(p)                              \
   do {                                            \
@@ -277,6 +279,133 @@ void mvr_decon_destroy(mvr_decon* d) {
   if (!d) return;
   d->d.destroy();
   delete d;
+}
+
+int mvr_decon_init_learned(mvr_decon* d, mvr_unet* net) {
+  NULLCHECK(d);
+  NULLCHECK(net);
+  if (net->n.V != d->d.V) { set_error("the network was built for a different number of views"); return -1; }
+  if (net->n.dev != d->d.dev) { set_error("the network lives on another device"); return -1; }
+  // PSFs, spectra and the AvgPrecise pass (its max[] and avg are kept, as PsiInitFromRAI does);
+  // a handle that is initialised already keeps them and only its psi is replaced
+  if (!d->d.ready && d->d.init(MVR_PSI_INIT_AVG_PRECISE, nullptr)) return -1;
+  MVR_HIP(hipStreamSynchronize(d->d.stream));
+  if (net->n.infer(d->d.img, d->d.w, d->d.N, d->d.minValue, d->d.psi)) return -1;
+  return d->d.commit_psi0();
+}
+
+int mvr_decon_set_reference(mvr_decon* d, const float* ref) {
+  NULLCHECK(d);
+  NULLCHECK(ref);
+  return d->d.set_reference(ref);
+}
+
+int mvr_decon_rel_l2(mvr_decon* d, double* out) {
+  NULLCHECK(d);
+  NULLCHECK(out);
+  return d->d.rel_l2(out);
+}
+
+int mvr_decon_run_until(mvr_decon* d, int maxIterations, double tol, int* iterationsOut, double* relOut) {
+  NULLCHECK(d);
+  return d->d.run_until(maxIterations, tol, iterationsOut, relOut);
+}
+
+// ---------------------------------------------------------------- learned initial estimate
+int mvr_unet_create(mvr_unet** out, int dev, int views, unsigned long long seed) {
+  NULLCHECK(out);
+  auto* h = new (std::nothrow) mvr_unet();
+  if (!h) { set_error("out of host memory"); return -1; }
+  if (h->n.create(dev, views, seed)) { h->n.destroy(); delete h; return -1; }
+  *out = h;
+  return 0;
+}
+
+long long mvr_unet_param_count(mvr_unet* u) {
+  NULLCHECK(u);
+  return (long long)u->n.param_count();
+}
+
+int mvr_unet_get_params(mvr_unet* u, float* out) {
+  NULLCHECK(u);
+  NULLCHECK(out);
+  return u->n.copy_params(out, nullptr, false);
+}
+
+int mvr_unet_set_params(mvr_unet* u, const float* in) {
+  NULLCHECK(u);
+  NULLCHECK(in);
+  return u->n.copy_params(nullptr, in, false);
+}
+
+int mvr_unet_get_grads(mvr_unet* u, float* out) {
+  NULLCHECK(u);
+  NULLCHECK(out);
+  return u->n.copy_params(out, nullptr, true);
+}
+
+int mvr_unet_add_volume(mvr_unet* u, const float* const* imgs, const float* const* weights, const float* target,
+                        const int n[3]) {
+  NULLCHECK(u);
+  NULLCHECK(imgs);
+  NULLCHECK(weights);
+  NULLCHECK(target);
+  NULLCHECK(n);
+  for (int v = 0; v < u->n.V; ++v)
+    if (!imgs[v] || !weights[v]) { set_error("null view image or weight"); return -1; }
+  if (n[0] < 4 || n[1] < 4 || n[2] < 4) { set_error("training volumes need extents >= 4"); return -1; }
+  return u->n.add_volume(imgs, weights, target, n);
+}
+
+int mvr_unet_clear_volumes(mvr_unet* u) {
+  NULLCHECK(u);
+  return u->n.clear_volumes();
+}
+
+int mvr_unet_train_step(mvr_unet* u, const int* volIdx, const int* origins, int batch, int patch, float lr,
+                        int update, double* loss) {
+  NULLCHECK(u);
+  NULLCHECK(volIdx);
+  NULLCHECK(origins);
+  return u->n.train_step(volIdx, origins, batch, patch, lr, loss, update != 0);
+}
+
+int mvr_unet_infer(mvr_unet* u, const float* const* imgs, const float* const* weights, const int n[3],
+                   float minValue, float* psiOut) {
+  NULLCHECK(u);
+  NULLCHECK(imgs);
+  NULLCHECK(weights);
+  NULLCHECK(n);
+  NULLCHECK(psiOut);
+  UNet& net = u->n;
+  MVR_HIP(hipSetDevice(net.dev));
+  const size_t nv = (size_t)n[0] * n[1] * n[2];
+  // borrowed arrays may be host memory: stage views, weights and psi on the device
+  std::vector<float*> di(net.V, nullptr), dw(net.V, nullptr);
+  float* dpsi = nullptr;
+  int rc = -1;
+  do {
+    bool ok = true;
+    for (int v = 0; v < net.V && ok; ++v) {
+      ok = hipMalloc(&di[v], nv * 4) == hipSuccess && hipMalloc(&dw[v], nv * 4) == hipSuccess &&
+           hipMemcpy(di[v], imgs[v], nv * 4, hipMemcpyDefault) == hipSuccess &&
+           hipMemcpy(dw[v], weights[v], nv * 4, hipMemcpyDefault) == hipSuccess;
+    }
+    if (!ok || hipMalloc(&dpsi, nv * 4) != hipSuccess) { set_error("device allocation or copy failed"); break; }
+    if (net.infer(di, dw, n, minValue, dpsi)) break;
+    if (hipMemcpy(psiOut, dpsi, nv * 4, hipMemcpyDefault) != hipSuccess) { set_error("copy-out failed"); break; }
+    rc = 0;
+  } while (0);
+  for (auto p : di) if (p) (void)hipFree(p);
+  for (auto p : dw) if (p) (void)hipFree(p);
+  if (dpsi) (void)hipFree(dpsi);
+  return rc;
+}
+
+void mvr_unet_destroy(mvr_unet* u) {
+  if (!u) return;
+  u->n.destroy();
+  delete u;
 }
 
 }  // extern "C"

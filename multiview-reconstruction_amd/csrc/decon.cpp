@@ -306,6 +306,52 @@ int Decon::run(int iterations) {
   return 0;
 }
 
+int Decon::commit_psi0() {
+  MVR_CHECK(ready, "init first");
+  MVR_HIP(hipSetDevice(dev));
+  MVR_HIP(hipMemcpyAsync(psi0, psi, nvox() * 4, hipMemcpyDeviceToDevice, stream));
+  MVR_HIP(hipStreamSynchronize(stream));
+  stats.clear();
+  return 0;
+}
+
+int Decon::set_reference(const float* r) {
+  MVR_CHECK(r != nullptr, "null reference volume");
+  MVR_HIP(hipSetDevice(dev));
+  if (!ref) MVR_HIP(hipMalloc(&ref, nvox() * 4));
+  if (!relDev) MVR_HIP(hipMalloc(&relDev, sizeof(double2) * 1025));
+  MVR_HIP(hipMemcpyAsync(ref, r, nvox() * 4, hipMemcpyDefault, stream));
+  MVR_HIP(hipStreamSynchronize(stream));
+  return 0;
+}
+
+// rel-L2 = ||psi - ref||_2 / ||ref||_2, reduced in FP64 on the device
+int Decon::rel_l2(double* out) {
+  MVR_CHECK(ready && ref, "init and set a reference first");
+  MVR_HIP(hipSetDevice(dev));
+  launch_rel_l2(psi, ref, nvox(), relDev + 1, relDev, stream);
+  MVR_HIP(hipGetLastError());
+  double2 h;
+  MVR_HIP(hipMemcpyAsync(&h, relDev, sizeof(double2), hipMemcpyDeviceToHost, stream));
+  MVR_HIP(hipStreamSynchronize(stream));
+  *out = h.y > 0 ? std::sqrt(h.x / h.y) : std::sqrt(h.x);
+  return 0;
+}
+
+// Iterations until rel-L2 to the reference is <= tol (checked after every iteration), at most maxIterations.
+int Decon::run_until(int maxIterations, double tol, int* iterationsOut, double* relOut) {
+  double r = 0;
+  if (rel_l2(&r)) return -1;
+  int it = 0;
+  while (r > tol && it < maxIterations) {
+    if (run(1) || rel_l2(&r)) return -1;
+    ++it;
+  }
+  if (iterationsOut) *iterationsOut = it;
+  if (relOut) *relOut = r;
+  return 0;
+}
+
 void Decon::destroy() {
   (void)hipSetDevice(dev);
   if (stream) (void)hipStreamSynchronize(stream);
@@ -313,10 +359,12 @@ void Decon::destroy() {
   for (auto p : w) if (p) (void)hipFree(p);
   for (auto p : spec1) if (p) (void)hipFree(p);
   for (auto p : spec2) if (p) (void)hipFree(p);
-  for (void* p : {(void*)psi, (void*)psi0, (void*)S, (void*)Spsi, (void*)T2, (void*)partial, (void*)statsDev})
+  for (void* p : {(void*)psi, (void*)psi0, (void*)S, (void*)Spsi, (void*)T2, (void*)partial, (void*)statsDev,
+                  (void*)ref, (void*)relDev})
     if (p) (void)hipFree(p);
   img.clear(); w.clear(); spec1.clear(); spec2.clear();
   psi = psi0 = nullptr; S = Spsi = nullptr; T2 = nullptr; partial = nullptr; statsDev = nullptr;
+  ref = nullptr; relDev = nullptr;
   plan.destroy();
   timer.release();
   if (stream) (void)hipStreamDestroy(stream);

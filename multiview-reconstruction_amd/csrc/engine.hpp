@@ -99,5 +99,6 @@ void launch_update_mul(float* psi, const float* const* T2, const float* const* w
                        const int o[3], const int D[3], double lambda, float minValue, float maxI, double2* partial,
                        int* nPartialOut, hipStream_t s);
 void launch_scale_copy_kernel_spectrum(const float2* in, float2* out, size_t n, float scale, hipStream_t s);
+void launch_rel_l2(const float* a, const float* b, size_t n, double2* partial, double2* out, hipStream_t s);
 
 }  // namespace mvr

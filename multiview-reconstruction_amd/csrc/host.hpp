@@ -113,11 +113,20 @@ struct Decon {
   bool ready = false;
   Timer timer;
 
+  // distance to a reference volume (convergence checks)
+  float* ref = nullptr;
+  double2* relDev = nullptr;
+
   int create(int dev, const int N[3], int V, int psfType, int scheme, float lambda, float minValue, int nT);
   int set_view(int v, const float* img, const float* w, const float* kernel, const int K[3]);
   int init(int psiInit, const float* psiIn);
+  // psi0 := the psi currently resident (after an external initialiser wrote it)
+  int commit_psi0();
   int run(int iterations);
   int reset_psi();
+  int set_reference(const float* r);
+  int rel_l2(double* out);
+  int run_until(int maxIterations, double tol, int* iterationsOut, double* relOut);
   void destroy();
 
   size_t nvox() const { return (size_t)N[0] * N[1] * N[2]; }

@@ -156,6 +156,54 @@ void launch_update_mul(float* psi, const float* const* T2, const float* const* w
   *nPartialOut = nb;
 }
 
+// Distance of psi to a reference volume in FP64: per work-group {sum (a - b)^2, sum b^2},
+// then one work-group sums the partials in a fixed order.
+__global__ __launch_bounds__(kThreads) void k_rel_l2_partials(const float* __restrict__ a, const float* __restrict__ b,
+                                                              size_t n, double2* __restrict__ partial) {
+  __shared__ double sd[kThreads], sr[kThreads];
+  double d = 0.0, r = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double e = (double)a[i] - (double)b[i];
+    d += e * e;
+    r += (double)b[i] * (double)b[i];
+  }
+  sd[threadIdx.x] = d;
+  sr[threadIdx.x] = r;
+  __syncthreads();
+  for (int off = kThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      sd[threadIdx.x] += sd[threadIdx.x + off];
+      sr[threadIdx.x] += sr[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(sd[0], sr[0]);
+}
+
+__global__ void k_sum2(const double2* __restrict__ partial, int n, double2* __restrict__ out) {
+  __shared__ double sd[kThreads], sr[kThreads];
+  double d = 0.0, r = 0.0;
+  for (int i = threadIdx.x; i < n; i += kThreads) { d += partial[i].x; r += partial[i].y; }
+  sd[threadIdx.x] = d;
+  sr[threadIdx.x] = r;
+  __syncthreads();
+  for (int off = kThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      sd[threadIdx.x] += sd[threadIdx.x + off];
+      sr[threadIdx.x] += sr[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = make_double2(sd[0], sr[0]);
+}
+
+// partial must hold 1024 entries; out receives {sum (a - b)^2, sum b^2}
+void launch_rel_l2(const float* a, const float* b, size_t n, double2* partial, double2* out, hipStream_t s) {
+  const int nb = (int)std::max<size_t>(1, std::min<size_t>((n + kThreads - 1) / kThreads, 1024));
+  hipLaunchKernelGGL(k_rel_l2_partials, dim3(nb), dim3(kThreads), 0, s, a, b, n, partial);
+  hipLaunchKernelGGL(k_sum2, dim3(1), dim3(kThreads), 0, s, partial, nb, out);
+}
+
 __global__ void k_scale_copy(const float2* __restrict__ in, float2* __restrict__ out, size_t n, float scale) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     out[i] = cscale(in[i], scale);

@@ -1,0 +1,497 @@
+// unet.hip -- kernels of the learned initial estimate: a 3-level 3D U-Net
+// (16/32/64 channels, 3^3 convolutions, ReLU) in BF16 on MFMA with FP32
+// accumulation, forward and backward.
+//
+// Tensors are channels-last bf16 [n][z][y][x][c] with the channel count padded
+// to a multiple of 16; a tensor view is (pointer, channel stride, channel
+// offset), so a skip concatenation is two producers writing disjoint channel
+// ranges of one buffer.  Convolution weights are [tap][cout][cin] (cin
+// contiguous), master copies FP32, compute copies bf16.
+//
+// MFMA layout (v_mfma_f32_16x16x16_bf16): A is 16(m) x 16(k), lane l holds
+// A[l%16][4(l/16)..4(l/16)+3]; B is 16(k) x 16(n), lane l holds
+// B[4(l/16)..+3][l%16]; D lane l holds D[4(l/16)+r][l%16], r = 0..3.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "unet.hpp"
+
+namespace mvr {
+
+typedef __attribute__((ext_vector_type(4))) short s16x4;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);  // round to nearest even
+}
+
+// ------------------------------------------------------------------ convolution (forward and data gradient)
+// out[v][co] = act( sum_tap sum_ci in[v + tap][ci] * w[tap][co][ci] + bias[co] ), zero padding.
+// Grid: x = ceil(W / 64) * H * D * N (x-segment fastest), y = cout / 16; four waves of 16 voxels each.
+__global__ __launch_bounds__(256) void k_conv_fwd(ConvArgs a) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = lane & 15, g = lane >> 4;
+  const int xb = (a.W + 63) / 64;
+  const int x0 = (blockIdx.x % xb) * 64 + wave * 16;
+  if (x0 >= a.W) return;
+  const int row = blockIdx.x / xb;  // (n * D + z) * H + y
+  const int y = row % a.H, nz = row / a.H;
+  const int z = nz % a.D, n = nz / a.D;
+  const int co0 = blockIdx.y * 16;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  const int nchunk = a.cin / 16;
+  for (int tap = 0; tap < a.taps; ++tap) {
+    int dz = 0, dy = 0, dx = 0;
+    if (a.taps == 27) { dz = tap / 9 - 1; dy = (tap / 3) % 3 - 1; dx = tap % 3 - 1; }
+    const int zz = z + dz, yy = y + dy, xx = x0 + i + dx;
+    const bool ok = (unsigned)zz < (unsigned)a.D && (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W;
+    const size_t vin = ok ? ((((size_t)n * a.D + zz) * a.H + yy) * a.W + xx) : 0;
+    const uint16_t* pin = a.in + vin * a.in_cs + 4 * g;
+    const uint16_t* pw = a.w + ((size_t)tap * a.cout + co0 + i) * a.cin + 4 * g;
+    for (int cb = 0; cb < nchunk; ++cb) {
+      s16x4 av = *reinterpret_cast<const s16x4*>(pin + cb * 16);
+      if (!ok) av = s16x4{0, 0, 0, 0};
+      const s16x4 bv = *reinterpret_cast<const s16x4*>(pw + cb * 16);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+  }
+  const int co = co0 + i;
+  const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int x = x0 + 4 * g + r;
+    if (x >= a.W) continue;
+    float v = acc[r] + b;
+    if (a.relu) v = fmaxf(v, 0.f);
+    const size_t vo = (((size_t)n * a.D + z) * a.H + y) * a.W + x;
+    a.out[vo * a.out_cs + a.out_co + co] = f2bf(v);
+  }
+}
+
+// ------------------------------------------------------------------ weight gradient
+// dw[tap][co][ci] = sum_v dout[v][co] * in[v + tap][ci], accumulated per work-group
+// over a contiguous range of 16-voxel x-segments; partial[blockIdx.x][tap][co][ci].
+// Grid: x = number of partitions, y = 27 (or 1) taps.  The 16 x 16 tiles
+// (co block, ci block) are spread over the four waves.
+__global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = lane & 15, g = lane >> 4;
+  const int tap = blockIdx.y;
+  int dz = 0, dy = 0, dx = 0;
+  if (a.taps == 27) { dz = tap / 9 - 1; dy = (tap / 3) % 3 - 1; dx = tap % 3 - 1; }
+  const int ncb = a.cout / 16, nib = a.cin / 16, ntiles = ncb * nib;
+  // tiles handled by this wave: t = wave, wave + 4, ... (at most kMaxWgradTiles)
+  f32x4 acc[kMaxWgradTiles];
+#pragma unroll
+  for (int t = 0; t < kMaxWgradTiles; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int segsPerRow = (a.W + 15) / 16;
+  const long long nseg = (long long)a.N * a.D * a.H * segsPerRow;
+  const long long s0 = nseg * blockIdx.x / gridDim.x, s1 = nseg * (blockIdx.x + 1) / gridDim.x;
+  for (long long s = s0; s < s1; ++s) {
+    const int sx = (int)(s % segsPerRow);
+    const long long row = s / segsPerRow;
+    const int y = (int)(row % a.H);
+    const long long nz = row / a.H;
+    const int z = (int)(nz % a.D), n = (int)(nz / a.D);
+    const int x0 = sx * 16;
+    // A operand: dout^T, A[m = co][k = v] -> lane holds dout[v = 4g + r][co = i]
+    // B operand: in shifted by the tap, B[k = v][n = ci] -> lane holds in[v = 4g + r][ci = i]
+    const int zz = z + dz, yy = y + dy;
+    const bool rowok = (unsigned)zz < (unsigned)a.D && (unsigned)yy < (unsigned)a.H;
+    for (int t = wave, slot = 0; t < ntiles; t += 4, ++slot) {
+      const int cb = t / nib, ib = t % nib;
+      s16x4 av, bv;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int x = x0 + 4 * g + r;
+        uint16_t dv = 0, iv = 0;
+        if (x < a.W) {
+          const size_t vo = (((size_t)n * a.D + z) * a.H + y) * a.W + x;
+          dv = a.dout[vo * a.dout_cs + a.dout_co + cb * 16 + i];
+          const int xx = x + dx;
+          if (rowok && (unsigned)xx < (unsigned)a.W) {
+            const size_t vi = (((size_t)n * a.D + zz) * a.H + yy) * a.W + xx;
+            iv = a.in[vi * a.in_cs + ib * 16 + i];
+          }
+        }
+        av[r] = (short)dv;
+        bv[r] = (short)iv;
+      }
+#pragma unroll
+      for (int q = 0; q < kMaxWgradTiles; ++q)
+        if (q == slot) acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[q], 0, 0, 0);
+    }
+  }
+  // D lane l holds dw[co = cb*16 + 4g + r][ci = ib*16 + i]
+  float* part = a.partial + ((size_t)blockIdx.x * a.taps + tap) * a.cout * a.cin;
+  for (int t = wave, slot = 0; t < ntiles; t += 4, ++slot) {
+    const int cb = t / nib, ib = t % nib;
+#pragma unroll
+    for (int q = 0; q < kMaxWgradTiles; ++q)
+      if (q == slot)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) part[(size_t)(cb * 16 + 4 * g + r) * a.cin + ib * 16 + i] = acc[q][r];
+  }
+}
+
+// sum of partials: grad[j] = sum_p partial[p][j], j < count
+__global__ void k_sum_partials(const float* __restrict__ partial, int nparts, size_t count, float* __restrict__ grad) {
+  for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += (size_t)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int p = 0; p < nparts; ++p) s += partial[(size_t)p * count + j];
+    grad[j] = s;
+  }
+}
+
+// bias gradient partials: db[co] = sum_v dout[v][co]; partial[blockIdx.x][co]
+__global__ __launch_bounds__(256) void k_bias_grad(const uint16_t* __restrict__ dout, int cs, int co0, int cout,
+                                                   size_t nvox, float* __restrict__ partial) {
+  __shared__ float red[256];
+  const int c = threadIdx.x % cout;
+  const int lanesPerC = 256 / cout;
+  const int sub = threadIdx.x / cout;
+  float s = 0.f;
+  if (sub < lanesPerC) {
+    const size_t v0 = nvox * blockIdx.x / gridDim.x, v1 = nvox * (blockIdx.x + 1) / gridDim.x;
+    for (size_t v = v0 + sub; v < v1; v += lanesPerC) s += bf2f(dout[v * cs + co0 + c]);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < cout) {
+    float t = 0.f;
+    for (int k = 0; k < lanesPerC; ++k) t += red[k * cout + threadIdx.x];
+    partial[(size_t)blockIdx.x * cout + threadIdx.x] = t;
+  }
+}
+
+// ------------------------------------------------------------------ elementwise pieces
+// ReLU backward in place: g[v][c] = post[v][c] > 0 ? g : 0
+__global__ void k_relu_bwd(uint16_t* __restrict__ grad, int gcs, int gco, const uint16_t* __restrict__ post, int pcs,
+                           int pco, int C, size_t nvox) {
+  const size_t total = nvox * C;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t v = e / C;
+    const int c = (int)(e % C);
+    if (bf2f(post[v * pcs + pco + c]) <= 0.f) grad[v * gcs + gco + c] = 0;
+  }
+}
+
+// 2x2x2 average pooling (forward) over channels [0, C) of in -> out
+__global__ void k_pool_fwd(const uint16_t* __restrict__ in, int ics, int ico, uint16_t* __restrict__ out, int ocs,
+                           int C, int N, int D, int H, int W) {  // D, H, W of the output
+  const size_t total = (size_t)N * D * H * W * C;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    size_t v = e / C;
+    const int x = (int)(v % W); v /= W;
+    const int y = (int)(v % H); v /= H;
+    const int z = (int)(v % D);
+    const int n = (int)(v / D);
+    float s = 0.f;
+    for (int k = 0; k < 8; ++k) {
+      const size_t vi = ((((size_t)n * 2 * D + 2 * z + (k >> 2)) * 2 * H + 2 * y + ((k >> 1) & 1)) * 2 * W + 2 * x + (k & 1));
+      s += bf2f(in[vi * ics + ico + c]);
+    }
+    out[(((((size_t)n * D + z) * H + y) * W + x)) * ocs + c] = f2bf(0.125f * s);
+  }
+}
+
+// pooling backward: din[2v + k] (+)= dout[v] / 8
+__global__ void k_pool_bwd(const uint16_t* __restrict__ dout, int ocs, uint16_t* __restrict__ din, int ics, int ico,
+                           int C, int N, int D, int H, int W, int accumulate) {  // D, H, W of the pooled tensor
+  const size_t total = (size_t)N * 8 * D * H * W * C;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    size_t v = e / C;  // fine voxel
+    const int x = (int)(v % (2 * W)); v /= 2 * W;
+    const int y = (int)(v % (2 * H)); v /= 2 * H;
+    const int z = (int)(v % (2 * D));
+    const int n = (int)(v / (2 * D));
+    const float gval = 0.125f * bf2f(dout[((((size_t)n * D + z / 2) * H + y / 2) * W + x / 2) * ocs + c]);
+    const size_t vf = ((((size_t)n * 2 * D + z) * 2 * H + y) * 2 * W + x);
+    uint16_t* p = din + vf * ics + ico + c;
+    *p = f2bf(accumulate ? bf2f(*p) + gval : gval);
+  }
+}
+
+// nearest-neighbour x2 upsampling (forward) into channels [oco, oco + C) of out
+__global__ void k_up_fwd(const uint16_t* __restrict__ in, int ics, uint16_t* __restrict__ out, int ocs, int oco, int C,
+                         int N, int D, int H, int W) {  // D, H, W of the coarse tensor
+  const size_t total = (size_t)N * 8 * D * H * W * C;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    size_t v = e / C;
+    const int x = (int)(v % (2 * W)); v /= 2 * W;
+    const int y = (int)(v % (2 * H)); v /= 2 * H;
+    const int z = (int)(v % (2 * D));
+    const int n = (int)(v / (2 * D));
+    const size_t vc = ((((size_t)n * D + z / 2) * H + y / 2) * W + x / 2);
+    const size_t vf = ((((size_t)n * 2 * D + z) * 2 * H + y) * 2 * W + x);
+    out[vf * ocs + oco + c] = in[vc * ics + c];
+  }
+}
+
+// upsampling backward: dcoarse[v] = sum of the 8 fine gradients
+__global__ void k_up_bwd(const uint16_t* __restrict__ dfine, int fcs, int fco, uint16_t* __restrict__ dcoarse, int ccs,
+                         int C, int N, int D, int H, int W) {  // coarse dims
+  const size_t total = (size_t)N * D * H * W * C;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    size_t v = e / C;
+    const int x = (int)(v % W); v /= W;
+    const int y = (int)(v % H); v /= H;
+    const int z = (int)(v % D);
+    const int n = (int)(v / D);
+    float s = 0.f;
+    for (int k = 0; k < 8; ++k) {
+      const size_t vf = ((((size_t)n * 2 * D + 2 * z + (k >> 2)) * 2 * H + 2 * y + ((k >> 1) & 1)) * 2 * W + 2 * x + (k & 1));
+      s += bf2f(dfine[vf * fcs + fco + c]);
+    }
+    dcoarse[((((size_t)n * D + z) * H + y) * W + x) * ccs + c] = f2bf(s);
+  }
+}
+
+// network input: channels 0..V-1 = w_v * img_v / scale, channel V = their weighted mean
+// sum_v w_v img_v / (scale * sum_v w_v) (0 where no view has weight), rest 0; one sample per
+// volume region [x0, x0 + W) x [y0, ...) x [z0, ...) of volumes of dims N3.  Positions past the
+// volume's last voxel (a region padded up to a multiple of 4) repeat that voxel.
+__global__ void k_pack_input(InputArgs a) {
+  const size_t total = (size_t)a.D * a.H * a.W;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < total; v += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(v % a.W), y = (int)((v / a.W) % a.H), z = (int)(v / ((size_t)a.W * a.H));
+    const int gx = min(a.x0 + x, a.N3[0] - 1), gy = min(a.y0 + y, a.N3[1] - 1), gz = min(a.z0 + z, a.N3[2] - 1);
+    const size_t gi = ((size_t)gz * a.N3[1] + gy) * a.N3[0] + gx;
+    float s = 0.f, sw = 0.f;
+    uint16_t* o = a.out + ((size_t)a.sample * total + v) * 16;
+    for (int c = 0; c < 16; ++c) {
+      float val = 0.f;
+      if (c < a.V) {
+        const float wv = a.w[c][gi];
+        val = a.img[c][gi] * wv * a.inv_scale;
+        s += val;
+        sw += wv;
+      }
+      o[c] = f2bf(val);
+    }
+    const float mean = sw > 0.f ? s / sw : 0.f;
+    o[a.V] = f2bf(mean);
+    if (a.avg_out) a.avg_out[(size_t)a.sample * total + v] = mean;
+  }
+}
+
+// output head: r[v] = sum_c wo[c] d1[v][c] + bo; pred = avg[v] + r[v]
+// training: loss partials of (pred - target)^2 and dr[v] = 2 (pred - target) / count
+// inference: psi[v] = max(minValue, scale * pred)
+__global__ __launch_bounds__(256) void k_head(HeadArgs a) {
+  __shared__ double red[256];
+  double ls = 0.0;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.nvox; v += (size_t)gridDim.x * blockDim.x) {
+    float r = a.bias[0];
+    const uint16_t* p = a.d1 + v * a.d1_cs;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) r += a.wo[c] * bf2f(p[c]);
+    const float pred = a.avg[v] + r;
+    if (a.target) {
+      const float e = pred - a.target[v];
+      ls += (double)e * e;
+      a.dr[v] = 2.f * e / (float)a.nvox;
+    }
+    if (a.psi) {
+      // net dims (W, H, D) may be the output dims (ox, oy, oz) padded up; only inner voxels are written
+      const int x = (int)(v % a.W), y = (int)((v / a.W) % a.H), z = (int)(v / ((size_t)a.W * a.H));
+      if (x < a.ox && y < a.oy && z < a.oz)
+        a.psi[((size_t)z * a.oy + y) * a.ox + x] = fmaxf(a.minValue, a.scale * pred);
+    }
+  }
+  red[threadIdx.x] = ls;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && a.loss_partial) a.loss_partial[blockIdx.x] = red[0];
+}
+
+// head backward: d_d1[v][c] = dr[v] * wo[c]; partials of dwo[c] = sum dr d1[v][c], dbo = sum dr
+__global__ __launch_bounds__(256) void k_head_bwd(HeadArgs a, uint16_t* __restrict__ dd1, float* __restrict__ partial) {
+  __shared__ float red[256][17];
+  float acc[17];
+#pragma unroll
+  for (int c = 0; c < 17; ++c) acc[c] = 0.f;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.nvox; v += (size_t)gridDim.x * blockDim.x) {
+    const float dr = a.dr[v];
+    const uint16_t* p = a.d1 + v * a.d1_cs;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      acc[c] += dr * bf2f(p[c]);
+      dd1[v * 16 + c] = f2bf(dr * a.wo[c]);
+    }
+    acc[16] += dr;
+  }
+#pragma unroll
+  for (int c = 0; c < 17; ++c) red[threadIdx.x][c] = acc[c];
+  __syncthreads();
+  if (threadIdx.x < 17) {
+    float s = 0.f;
+    for (int k = 0; k < 256; ++k) s += red[k][threadIdx.x];
+    partial[(size_t)blockIdx.x * 17 + threadIdx.x] = s;
+  }
+}
+
+// Adam on FP32 master weights; refreshes the bf16 compute copy and, for 3^3 layers,
+// the flipped/transposed copy used by the data gradient (wt[26 - tap][ci][co] = w[tap][co][ci]).
+__global__ void k_adam(AdamArgs a) {
+  for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < a.count; j += (size_t)gridDim.x * blockDim.x) {
+    const float g = a.grad[j];
+    const float m = a.beta1 * a.m[j] + (1.f - a.beta1) * g;
+    const float v = a.beta2 * a.v[j] + (1.f - a.beta2) * g * g;
+    a.m[j] = m;
+    a.v[j] = v;
+    const float mh = m * a.c1, vh = v * a.c2;
+    const float w = a.w[j] - a.lr * mh / (sqrtf(vh) + a.eps);
+    a.w[j] = w;
+    if (a.wbf) a.wbf[j] = f2bf(w);
+    if (a.wt) {
+      const int per = a.cout * a.cin;
+      const int tap = (int)(j / per), rem = (int)(j % per);
+      const int co = rem / a.cin, ci = rem % a.cin;
+      a.wt[((size_t)(a.taps - 1 - tap) * a.cin + ci) * a.cout + co] = f2bf(w);
+    }
+  }
+}
+
+__global__ void k_f32_to_bf16(const float* __restrict__ src, uint16_t* __restrict__ dst, size_t n) {
+  for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (size_t)gridDim.x * blockDim.x)
+    dst[j] = f2bf(src[j]);
+}
+
+__global__ void k_bf16_to_f32(const uint16_t* __restrict__ src, float* __restrict__ dst, size_t n) {
+  for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (size_t)gridDim.x * blockDim.x)
+    dst[j] = bf2f(src[j]);
+}
+
+__global__ void k_zero_u16(uint16_t* __restrict__ p, size_t n) {
+  for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (size_t)gridDim.x * blockDim.x) p[j] = 0;
+}
+
+// normalisation of the network input: partial[b] = {sum w_v img_v, sum w_v} (FP64) over a
+// grid-stride share of the voxels and all views; the scale is the ratio of the totals
+__global__ __launch_bounds__(256) void k_wsum_partials(InputArgs a, size_t n, double2* __restrict__ partial) {
+  __shared__ double red[256], redw[256];
+  double s = 0.0, sw = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    float t = 0.f, tw = 0.f;
+    for (int c = 0; c < a.V; ++c) {
+      t += a.img[c][i] * a.w[c][i];
+      tw += a.w[c][i];
+    }
+    s += (double)t;
+    sw += (double)tw;
+  }
+  red[threadIdx.x] = s;
+  redw[threadIdx.x] = sw;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      red[threadIdx.x] += red[threadIdx.x + off];
+      redw[threadIdx.x] += redw[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(red[0], redw[0]);
+}
+
+// target patch of a training volume, normalised: dst[v] = src[o + v] * invScale
+__global__ void k_gather_target(const float* __restrict__ src, int3 N, int3 o, int P, float invScale,
+                                float* __restrict__ dst) {
+  const size_t total = (size_t)P * P * P;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < total; v += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(v % P), y = (int)((v / P) % P), z = (int)(v / ((size_t)P * P));
+    dst[v] = src[((size_t)(o.z + z) * N.y + (o.y + y)) * N.x + (o.x + x)] * invScale;
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+static inline int grid1(size_t n, int cap = 8192) {
+  const size_t b = (n + 255) / 256;
+  return (int)(b < 1 ? 1 : (b > (size_t)cap ? (size_t)cap : b));
+}
+
+void launch_conv_fwd(const ConvArgs& a, hipStream_t s) {
+  const dim3 grid((unsigned)((a.W + 63) / 64) * a.H * a.D * a.N, a.cout / 16);
+  hipLaunchKernelGGL(k_conv_fwd, grid, dim3(256), 0, s, a);
+}
+void launch_conv_wgrad(const WgradArgs& a, int nparts, hipStream_t s) {
+  hipLaunchKernelGGL(k_conv_wgrad, dim3(nparts, a.taps), dim3(256), 0, s, a);
+}
+void launch_sum_partials(const float* partial, int nparts, size_t count, float* grad, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_partials, dim3(grid1(count, 4096)), dim3(256), 0, s, partial, nparts, count, grad);
+}
+void launch_bias_grad(const uint16_t* dout, int cs, int co0, int cout, size_t nvox, float* partial, int nparts,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_bias_grad, dim3(nparts), dim3(256), 0, s, dout, cs, co0, cout, nvox, partial);
+}
+void launch_relu_bwd(uint16_t* g, int gcs, int gco, const uint16_t* post, int pcs, int pco, int C, size_t nvox,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_relu_bwd, dim3(grid1(nvox * C)), dim3(256), 0, s, g, gcs, gco, post, pcs, pco, C, nvox);
+}
+void launch_pool_fwd(const uint16_t* in, int ics, int ico, uint16_t* out, int ocs, int C, int N, int D, int H, int W,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_pool_fwd, dim3(grid1((size_t)N * D * H * W * C)), dim3(256), 0, s, in, ics, ico, out, ocs, C, N,
+                     D, H, W);
+}
+void launch_pool_bwd(const uint16_t* dout, int ocs, uint16_t* din, int ics, int ico, int C, int N, int D, int H, int W,
+                     int accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(k_pool_bwd, dim3(grid1((size_t)N * 8 * D * H * W * C)), dim3(256), 0, s, dout, ocs, din, ics,
+                     ico, C, N, D, H, W, accumulate);
+}
+void launch_up_fwd(const uint16_t* in, int ics, uint16_t* out, int ocs, int oco, int C, int N, int D, int H, int W,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_up_fwd, dim3(grid1((size_t)N * 8 * D * H * W * C)), dim3(256), 0, s, in, ics, out, ocs, oco, C,
+                     N, D, H, W);
+}
+void launch_up_bwd(const uint16_t* dfine, int fcs, int fco, uint16_t* dcoarse, int ccs, int C, int N, int D, int H,
+                   int W, hipStream_t s) {
+  hipLaunchKernelGGL(k_up_bwd, dim3(grid1((size_t)N * D * H * W * C)), dim3(256), 0, s, dfine, fcs, fco, dcoarse, ccs,
+                     C, N, D, H, W);
+}
+void launch_pack_input(const InputArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_input, dim3(grid1((size_t)a.D * a.H * a.W)), dim3(256), 0, s, a);
+}
+int launch_head(const HeadArgs& a, hipStream_t s) {
+  const int nb = grid1(a.nvox, 1024);
+  hipLaunchKernelGGL(k_head, dim3(nb), dim3(256), 0, s, a);
+  return nb;
+}
+int launch_head_bwd(const HeadArgs& a, uint16_t* dd1, float* partial, hipStream_t s) {
+  const int nb = grid1(a.nvox, 512);
+  hipLaunchKernelGGL(k_head_bwd, dim3(nb), dim3(256), 0, s, a, dd1, partial);
+  return nb;
+}
+void launch_adam(const AdamArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_adam, dim3(grid1(a.count, 2048)), dim3(256), 0, s, a);
+}
+void launch_f32_to_bf16(const float* src, uint16_t* dst, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3(grid1(n)), dim3(256), 0, s, src, dst, n);
+}
+void launch_bf16_to_f32(const uint16_t* src, float* dst, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_bf16_to_f32, dim3(grid1(n)), dim3(256), 0, s, src, dst, n);
+}
+void launch_gather_target(const float* src, const int N[3], const int o[3], int P, float invScale, float* dst,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_target, dim3(grid1((size_t)P * P * P, 2048)), dim3(256), 0, s, src,
+                     make_int3(N[0], N[1], N[2]), make_int3(o[0], o[1], o[2]), P, invScale, dst);
+}
+int launch_wsum_partials(const InputArgs& a, size_t n, double2* partial, hipStream_t s) {
+  const int nb = grid1(n, 1024);
+  hipLaunchKernelGGL(k_wsum_partials, dim3(nb), dim3(256), 0, s, a, n, partial);
+  return nb;
+}
+void launch_zero_u16(uint16_t* p, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u16, dim3(grid1(n)), dim3(256), 0, s, p, n);
+}
+
+}  // namespace mvr

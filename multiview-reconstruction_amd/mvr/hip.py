@@ -78,6 +78,23 @@ def lib():
         L.mvr_decon_pass_times.argtypes = [vp, _f64p, ctypes.POINTER(ctypes.c_longlong)]
         L.mvr_decon_destroy.argtypes = [vp]
         L.mvr_decon_destroy.restype = None
+        L.mvr_decon_set_reference.argtypes = [vp, vp]
+        L.mvr_decon_rel_l2.argtypes = [vp, _f64p]
+        L.mvr_decon_run_until.argtypes = [vp, ctypes.c_int, ctypes.c_double, _i32p, _f64p]
+        L.mvr_decon_init_learned.argtypes = [vp, vp]
+        L.mvr_unet_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong]
+        L.mvr_unet_param_count.argtypes = [vp]
+        L.mvr_unet_param_count.restype = ctypes.c_longlong
+        L.mvr_unet_get_params.argtypes = [vp, vp]
+        L.mvr_unet_set_params.argtypes = [vp, vp]
+        L.mvr_unet_get_grads.argtypes = [vp, vp]
+        L.mvr_unet_add_volume.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), vp, _i32p]
+        L.mvr_unet_clear_volumes.argtypes = [vp]
+        L.mvr_unet_train_step.argtypes = [vp, _i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                          ctypes.c_int, _f64p]
+        L.mvr_unet_infer.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), _i32p, ctypes.c_float, vp]
+        L.mvr_unet_destroy.argtypes = [vp]
+        L.mvr_unet_destroy.restype = None
         _lib = L
     return _lib
 
@@ -266,9 +283,109 @@ class Decon:
         _check(lib().mvr_decon_pass_times(self.h, ms, n))
         return {PASSES[i]: (ms[i], n[i]) for i in range(9)}
 
+    def init_learned(self, net: "UNet"):
+        """PsiInit from the network (PsiInitFromRAI pattern: max[] and avg from the AvgPrecise pass)."""
+        _check(lib().mvr_decon_init_learned(self.h, net.h))
+
+    def set_reference(self, ref):
+        r = _f32(ref)
+        assert r.shape == self.shape
+        _check(lib().mvr_decon_set_reference(self.h, _ptr(r)))
+
+    def rel_l2(self):
+        out = ctypes.c_double()
+        _check(lib().mvr_decon_rel_l2(self.h, ctypes.byref(out)))
+        return out.value
+
+    def run_until(self, tol, max_iterations):
+        """Iterate until rel-L2 to the reference <= tol; returns (iterations, rel-L2)."""
+        it = ctypes.c_int()
+        rel = ctypes.c_double()
+        _check(lib().mvr_decon_run_until(self.h, max_iterations, tol, ctypes.byref(it), ctypes.byref(rel)))
+        return it.value, rel.value
+
     def close(self):
         if self.h:
             lib().mvr_decon_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ptr_array(arrs):
+    return (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+class UNet:
+    """Learned initial estimate: 3-level 3D U-Net (16/32/64 channels) trained and run by the HIP kernels.
+
+    Parameters are one flat FP32 vector: per conv layer the weights [27][cout][cin] then the bias,
+    last the 1x1 head (16 weights, bias).  Volumes are numpy arrays in z, y, x order.
+    """
+
+    LAYERS_CIN = (16, 16, 16, 32, 32, 64, 96, 32, 48, 16)
+    LAYERS_COUT = (16, 16, 32, 32, 64, 64, 32, 32, 16, 16)
+
+    def __init__(self, n_views, seed=0, dev=0):
+        h = ctypes.c_void_p()
+        _check(lib().mvr_unet_create(ctypes.byref(h), dev, n_views, seed))
+        self.h = h
+        self.V = n_views
+        self._keep = []
+
+    def param_count(self):
+        return _check(lib().mvr_unet_param_count(self.h))
+
+    def params(self):
+        out = np.empty(self.param_count(), np.float32)
+        _check(lib().mvr_unet_get_params(self.h, _ptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.param_count()
+        _check(lib().mvr_unet_set_params(self.h, _ptr(p)))
+
+    def grads(self):
+        out = np.empty(self.param_count(), np.float32)
+        _check(lib().mvr_unet_get_grads(self.h, _ptr(out)))
+        return out
+
+    def add_volume(self, imgs, weights, target):
+        imgs = [_f32(a) for a in imgs]
+        weights = [_f32(a) for a in weights]
+        target = _f32(target)
+        assert len(imgs) == self.V and len(weights) == self.V
+        return _check(lib().mvr_unet_add_volume(self.h, _ptr_array(imgs), _ptr_array(weights), _ptr(target),
+                                                _dims(target.shape)))
+
+    def clear_volumes(self):
+        _check(lib().mvr_unet_clear_volumes(self.h))
+
+    def train_step(self, vol_idx, origins_zyx, patch, lr, update=True):
+        """One step on patches; origins are (z, y, x) per patch.  Returns the loss (normalised MSE)."""
+        vi = np.ascontiguousarray(vol_idx, np.int32)
+        o = np.ascontiguousarray(np.asarray(origins_zyx, np.int32)[:, ::-1])  # -> x, y, z
+        loss = ctypes.c_double()
+        _check(lib().mvr_unet_train_step(self.h, vi.ctypes.data_as(_i32p), o.ctypes.data_as(_i32p), len(vi), patch,
+                                         lr, 1 if update else 0, ctypes.byref(loss)))
+        return loss.value
+
+    def infer(self, imgs, weights, min_value=1e-4):
+        imgs = [_f32(a) for a in imgs]
+        weights = [_f32(a) for a in weights]
+        out = np.empty(imgs[0].shape, np.float32)
+        _check(lib().mvr_unet_infer(self.h, _ptr_array(imgs), _ptr_array(weights), _dims(out.shape), min_value,
+                                    _ptr(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().mvr_unet_destroy(self.h)
             self.h = None
 
     def __del__(self):
