@@ -161,8 +161,11 @@ int UNet::ensure_acts(Acts& a, int N_, int D_, int H_, int W_, bool grads) {
 }
 
 void UNet::conv(int l, const uint16_t* in, int in_cs, int in_co, uint16_t* out, int out_cs, int out_co, int N, int D,
-                int H, int W, bool relu) {
+                int H, int W, bool relu, const uint16_t* up, int up_cs, int up_c) {
   ConvArgs c{};
+  c.up_in = up;
+  c.up_cs = up_cs;
+  c.up_c = up ? up_c : 0;
   c.in = in + in_co;
   c.in_cs = in_cs;
   c.cin = kCin[l];
@@ -229,11 +232,21 @@ int UNet::forward(Acts& a, bool training, const float* psiScale, float* psiOut, 
   launch_pool_fwd(a.cat2, 96, 64, a.p2, 32, 32, N, D2, H2, W2, stream);
   conv(4, a.p2, 32, 0, a.a3, 64, 0, N, D2, H2, W2, true);
   conv(5, a.a3, 64, 0, a.b3, 64, 0, N, D2, H2, W2, true);
-  launch_up_fwd(a.b3, 64, a.cat2, 96, 0, 64, N, D2, H2, W2, stream);        // up(b3) -> cat2[0:64)
-  conv(6, a.cat2, 96, 0, a.a4, 32, 0, N, D1, H1, W1, true);
+  // decoder: training materialises the concatenations (the weight gradients read them); inference
+  // reads the upsampled part straight from the coarse tensor inside the convolution's staging
+  if (training) {
+    launch_up_fwd(a.b3, 64, a.cat2, 96, 0, 64, N, D2, H2, W2, stream);      // up(b3) -> cat2[0:64)
+    conv(6, a.cat2, 96, 0, a.a4, 32, 0, N, D1, H1, W1, true);
+  } else {
+    conv(6, a.cat2, 96, 64, a.a4, 32, 0, N, D1, H1, W1, true, a.b3, 64, 64);
+  }
   conv(7, a.a4, 32, 0, a.d2, 32, 0, N, D1, H1, W1, true);
-  launch_up_fwd(a.d2, 32, a.cat1, 48, 0, 32, N, D1, H1, W1, stream);        // up(d2) -> cat1[0:32)
-  conv(8, a.cat1, 48, 0, a.a5, 16, 0, N, D0, H0, W0, true);
+  if (training) {
+    launch_up_fwd(a.d2, 32, a.cat1, 48, 0, 32, N, D1, H1, W1, stream);      // up(d2) -> cat1[0:32)
+    conv(8, a.cat1, 48, 0, a.a5, 16, 0, N, D0, H0, W0, true);
+  } else {
+    conv(8, a.cat1, 48, 32, a.a5, 16, 0, N, D0, H0, W0, true, a.d2, 32, 32);
+  }
   conv(9, a.a5, 16, 0, a.d1, 16, 0, N, D0, H0, W0, true);
   HeadArgs h{};
   h.d1 = a.d1; h.d1_cs = 16;

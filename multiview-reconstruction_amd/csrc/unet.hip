@@ -29,47 +29,7 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 
 // ------------------------------------------------------------------ convolution (forward and data gradient)
-// out[v][co] = act( sum_tap sum_ci in[v + tap][ci] * w[tap][co][ci] + bias[co] ), zero padding.
-// Grid: x = ceil(W / 64) * H * D * N (x-segment fastest), y = cout / 16; four waves of 16 voxels each.
-__global__ __launch_bounds__(256) void k_conv_fwd(ConvArgs a) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = lane & 15, g = lane >> 4;
-  const int xb = (a.W + 63) / 64;
-  const int x0 = (blockIdx.x % xb) * 64 + wave * 16;
-  if (x0 >= a.W) return;
-  const int row = blockIdx.x / xb;  // (n * D + z) * H + y
-  const int y = row % a.H, nz = row / a.H;
-  const int z = nz % a.D, n = nz / a.D;
-  const int co0 = blockIdx.y * 16;
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  const int nchunk = a.cin / 16;
-  for (int tap = 0; tap < a.taps; ++tap) {
-    int dz = 0, dy = 0, dx = 0;
-    if (a.taps == 27) { dz = tap / 9 - 1; dy = (tap / 3) % 3 - 1; dx = tap % 3 - 1; }
-    const int zz = z + dz, yy = y + dy, xx = x0 + i + dx;
-    const bool ok = (unsigned)zz < (unsigned)a.D && (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W;
-    const size_t vin = ok ? ((((size_t)n * a.D + zz) * a.H + yy) * a.W + xx) : 0;
-    const uint16_t* pin = a.in + vin * a.in_cs + 4 * g;
-    const uint16_t* pw = a.w + ((size_t)tap * a.cout + co0 + i) * a.cin + 4 * g;
-    for (int cb = 0; cb < nchunk; ++cb) {
-      s16x4 av = *reinterpret_cast<const s16x4*>(pin + cb * 16);
-      if (!ok) av = s16x4{0, 0, 0, 0};
-      const s16x4 bv = *reinterpret_cast<const s16x4*>(pw + cb * 16);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
-    }
-  }
-  const int co = co0 + i;
-  const float b = a.bias ? a.bias[co] : 0.f;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int x = x0 + 4 * g + r;
-    if (x >= a.W) continue;
-    float v = acc[r] + b;
-    if (a.relu) v = fmaxf(v, 0.f);
-    const size_t vo = (((size_t)n * a.D + z) * a.H + y) * a.W + x;
-    a.out[vo * a.out_cs + a.out_co + co] = f2bf(v);
-  }
-}
+// The 3^3 convolutions run in the LDS-tiled kernel of unet_conv.hip (launch_conv_tiled).
 
 // ------------------------------------------------------------------ weight gradient
 // dw[tap][co][ci] = sum_v dout[v][co] * in[v + tap][ci], accumulated per work-group
@@ -168,90 +128,118 @@ __global__ __launch_bounds__(256) void k_bias_grad(const uint16_t* __restrict__ 
 }
 
 // ------------------------------------------------------------------ elementwise pieces
-// ReLU backward in place: g[v][c] = post[v][c] > 0 ? g : 0
+// All work on units of 8 channels (16 bytes); channel counts, strides and offsets are multiples of 16.
+typedef __attribute__((ext_vector_type(8))) unsigned short u16x8;
+
+__device__ __forceinline__ void unpack8(const u16x8& u, float f[8]) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) f[k] = bf2f(u[k]);
+}
+__device__ __forceinline__ u16x8 pack8(const float f[8]) {
+  u16x8 u;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) u[k] = f2bf(f[k]);
+  return u;
+}
+template <typename T>
+__device__ __forceinline__ const u16x8& at8(const T* p) { return *reinterpret_cast<const u16x8*>(p); }
+__device__ __forceinline__ u16x8& at8w(uint16_t* p) { return *reinterpret_cast<u16x8*>(p); }
+
+// Index spaces: row kernels run on grid (ceil(W * U / 256), H, N * D) of the tensor they iterate
+// over, U = C / 8 units per voxel, so all index arithmetic is 32-bit.
+struct RowIdx {
+  int x, c, y, z, n;
+  bool ok;
+};
+__device__ __forceinline__ RowIdx row_idx(int U, int W, int D) {
+  RowIdx r;
+  const int xu = blockIdx.x * blockDim.x + threadIdx.x;
+  r.ok = xu < W * U;
+  r.x = xu / U;
+  r.c = (xu - r.x * U) * 8;
+  r.y = blockIdx.y;
+  r.z = blockIdx.z % D;
+  r.n = blockIdx.z / D;
+  return r;
+}
+__device__ __forceinline__ size_t vox(int n, int z, int y, int x, int D, int H, int W) {
+  return (((size_t)n * D + z) * H + y) * W + x;
+}
+
+// ReLU backward in place: g[v][c] = post[v][c] > 0 ? g : 0 (C / 8 a power of two: ushift = log2(C / 8))
 __global__ void k_relu_bwd(uint16_t* __restrict__ grad, int gcs, int gco, const uint16_t* __restrict__ post, int pcs,
-                           int pco, int C, size_t nvox) {
-  const size_t total = nvox * C;
+                           int pco, int ushift, size_t nvox) {
+  const size_t total = nvox << ushift;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const size_t v = e / C;
-    const int c = (int)(e % C);
-    if (bf2f(post[v * pcs + pco + c]) <= 0.f) grad[v * gcs + gco + c] = 0;
+    const size_t v = e >> ushift;
+    const int c = (int)(e & ((1u << ushift) - 1)) * 8;
+    const u16x8 pv = at8(post + v * pcs + pco + c);
+    u16x8 gv = at8(grad + v * gcs + gco + c);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (bf2f(pv[k]) <= 0.f) gv[k] = 0;
+    at8w(grad + v * gcs + gco + c) = gv;
   }
 }
 
-// 2x2x2 average pooling (forward) over channels [0, C) of in -> out
+// 2x2x2 average pooling (forward) over channels [0, C) of in -> out; grid over the output (D, H, W)
 __global__ void k_pool_fwd(const uint16_t* __restrict__ in, int ics, int ico, uint16_t* __restrict__ out, int ocs,
-                           int C, int N, int D, int H, int W) {  // D, H, W of the output
-  const size_t total = (size_t)N * D * H * W * C;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(e % C);
-    size_t v = e / C;
-    const int x = (int)(v % W); v /= W;
-    const int y = (int)(v % H); v /= H;
-    const int z = (int)(v % D);
-    const int n = (int)(v / D);
-    float s = 0.f;
-    for (int k = 0; k < 8; ++k) {
-      const size_t vi = ((((size_t)n * 2 * D + 2 * z + (k >> 2)) * 2 * H + 2 * y + ((k >> 1) & 1)) * 2 * W + 2 * x + (k & 1));
-      s += bf2f(in[vi * ics + ico + c]);
-    }
-    out[(((((size_t)n * D + z) * H + y) * W + x)) * ocs + c] = f2bf(0.125f * s);
+                           int C, int D, int H, int W) {
+  const RowIdx r = row_idx(C / 8, W, D);
+  if (!r.ok) return;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const size_t vi = vox(r.n, 2 * r.z + (k >> 2), 2 * r.y + ((k >> 1) & 1), 2 * r.x + (k & 1), 2 * D, 2 * H, 2 * W);
+    float f[8];
+    unpack8(at8(in + vi * ics + ico + r.c), f);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s[q] += f[q];
   }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) s[q] *= 0.125f;
+  at8w(out + vox(r.n, r.z, r.y, r.x, D, H, W) * ocs + r.c) = pack8(s);
 }
 
-// pooling backward: din[2v + k] (+)= dout[v] / 8
+// pooling backward: din[2v + k] (+)= dout[v] / 8; grid over the fine tensor (2D, 2H, 2W)
 __global__ void k_pool_bwd(const uint16_t* __restrict__ dout, int ocs, uint16_t* __restrict__ din, int ics, int ico,
-                           int C, int N, int D, int H, int W, int accumulate) {  // D, H, W of the pooled tensor
-  const size_t total = (size_t)N * 8 * D * H * W * C;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(e % C);
-    size_t v = e / C;  // fine voxel
-    const int x = (int)(v % (2 * W)); v /= 2 * W;
-    const int y = (int)(v % (2 * H)); v /= 2 * H;
-    const int z = (int)(v % (2 * D));
-    const int n = (int)(v / (2 * D));
-    const float gval = 0.125f * bf2f(dout[((((size_t)n * D + z / 2) * H + y / 2) * W + x / 2) * ocs + c]);
-    const size_t vf = ((((size_t)n * 2 * D + z) * 2 * H + y) * 2 * W + x);
-    uint16_t* p = din + vf * ics + ico + c;
-    *p = f2bf(accumulate ? bf2f(*p) + gval : gval);
-  }
+                           int C, int D, int H, int W, int accumulate) {  // D, H, W of the pooled tensor
+  const RowIdx r = row_idx(C / 8, 2 * W, 2 * D);
+  if (!r.ok) return;
+  float gval[8];
+  unpack8(at8(dout + vox(r.n, r.z / 2, r.y / 2, r.x / 2, D, H, W) * ocs + r.c), gval);
+  uint16_t* p = din + vox(r.n, r.z, r.y, r.x, 2 * D, 2 * H, 2 * W) * ics + ico + r.c;
+  float cur[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (accumulate) unpack8(at8(p), cur);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) cur[q] += 0.125f * gval[q];
+  at8w(p) = pack8(cur);
 }
 
-// nearest-neighbour x2 upsampling (forward) into channels [oco, oco + C) of out
+// nearest-neighbour x2 upsampling into channels [oco, oco + C) of out; grid over the fine tensor
 __global__ void k_up_fwd(const uint16_t* __restrict__ in, int ics, uint16_t* __restrict__ out, int ocs, int oco, int C,
-                         int N, int D, int H, int W) {  // D, H, W of the coarse tensor
-  const size_t total = (size_t)N * 8 * D * H * W * C;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(e % C);
-    size_t v = e / C;
-    const int x = (int)(v % (2 * W)); v /= 2 * W;
-    const int y = (int)(v % (2 * H)); v /= 2 * H;
-    const int z = (int)(v % (2 * D));
-    const int n = (int)(v / (2 * D));
-    const size_t vc = ((((size_t)n * D + z / 2) * H + y / 2) * W + x / 2);
-    const size_t vf = ((((size_t)n * 2 * D + z) * 2 * H + y) * 2 * W + x);
-    out[vf * ocs + oco + c] = in[vc * ics + c];
-  }
+                         int D, int H, int W) {  // D, H, W of the coarse tensor
+  const RowIdx r = row_idx(C / 8, 2 * W, 2 * D);
+  if (!r.ok) return;
+  at8w(out + vox(r.n, r.z, r.y, r.x, 2 * D, 2 * H, 2 * W) * ocs + oco + r.c) =
+      at8(in + vox(r.n, r.z / 2, r.y / 2, r.x / 2, D, H, W) * ics + r.c);
 }
 
-// upsampling backward: dcoarse[v] = sum of the 8 fine gradients
+// upsampling backward: dcoarse[v] = sum of the 8 fine gradients; grid over the coarse tensor
 __global__ void k_up_bwd(const uint16_t* __restrict__ dfine, int fcs, int fco, uint16_t* __restrict__ dcoarse, int ccs,
-                         int C, int N, int D, int H, int W) {  // coarse dims
-  const size_t total = (size_t)N * D * H * W * C;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(e % C);
-    size_t v = e / C;
-    const int x = (int)(v % W); v /= W;
-    const int y = (int)(v % H); v /= H;
-    const int z = (int)(v % D);
-    const int n = (int)(v / D);
-    float s = 0.f;
-    for (int k = 0; k < 8; ++k) {
-      const size_t vf = ((((size_t)n * 2 * D + 2 * z + (k >> 2)) * 2 * H + 2 * y + ((k >> 1) & 1)) * 2 * W + 2 * x + (k & 1));
-      s += bf2f(dfine[vf * fcs + fco + c]);
-    }
-    dcoarse[((((size_t)n * D + z) * H + y) * W + x) * ccs + c] = f2bf(s);
+                         int C, int D, int H, int W) {  // coarse dims
+  const RowIdx r = row_idx(C / 8, W, D);
+  if (!r.ok) return;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const size_t vf = vox(r.n, 2 * r.z + (k >> 2), 2 * r.y + ((k >> 1) & 1), 2 * r.x + (k & 1), 2 * D, 2 * H, 2 * W);
+    float f[8];
+    unpack8(at8(dfine + vf * fcs + fco + r.c), f);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s[q] += f[q];
   }
+  at8w(dcoarse + vox(r.n, r.z, r.y, r.x, D, H, W) * ccs + r.c) = pack8(s);
 }
 
 // network input: channels 0..V-1 = w_v * img_v / scale, channel V = their weighted mean
@@ -260,24 +248,32 @@ __global__ void k_up_bwd(const uint16_t* __restrict__ dfine, int fcs, int fco, u
 // volume's last voxel (a region padded up to a multiple of 4) repeat that voxel.
 __global__ void k_pack_input(InputArgs a) {
   const size_t total = (size_t)a.D * a.H * a.W;
-  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < total; v += (size_t)gridDim.x * blockDim.x) {
-    const int x = (int)(v % a.W), y = (int)((v / a.W) % a.H), z = (int)(v / ((size_t)a.W * a.H));
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+  if (x >= a.W) return;
+  {
+    const size_t v = ((size_t)z * a.H + y) * a.W + x;
     const int gx = min(a.x0 + x, a.N3[0] - 1), gy = min(a.y0 + y, a.N3[1] - 1), gz = min(a.z0 + z, a.N3[2] - 1);
     const size_t gi = ((size_t)gz * a.N3[1] + gy) * a.N3[0] + gx;
     float s = 0.f, sw = 0.f;
-    uint16_t* o = a.out + ((size_t)a.sample * total + v) * 16;
-    for (int c = 0; c < 16; ++c) {
-      float val = 0.f;
+    float val[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) val[c] = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {  // InputArgs holds at most 8 views
       if (c < a.V) {
         const float wv = a.w[c][gi];
-        val = a.img[c][gi] * wv * a.inv_scale;
-        s += val;
+        val[c] = a.img[c][gi] * wv * a.inv_scale;
+        s += val[c];
         sw += wv;
       }
-      o[c] = f2bf(val);
     }
     const float mean = sw > 0.f ? s / sw : 0.f;
-    o[a.V] = f2bf(mean);
+#pragma unroll
+    for (int c = 0; c < 16; ++c)
+      if (c == a.V) val[c] = mean;
+    uint16_t* o = a.out + ((size_t)a.sample * total + v) * 16;
+    at8w(o) = pack8(val);
+    at8w(o + 8) = pack8(val + 8);
     if (a.avg_out) a.avg_out[(size_t)a.sample * total + v] = mean;
   }
 }
@@ -421,8 +417,8 @@ static inline int grid1(size_t n, int cap = 8192) {
 }
 
 void launch_conv_fwd(const ConvArgs& a, hipStream_t s) {
-  const dim3 grid((unsigned)((a.W + 63) / 64) * a.H * a.D * a.N, a.cout / 16);
-  hipLaunchKernelGGL(k_conv_fwd, grid, dim3(256), 0, s, a);
+  // every layer of the network has cin, cout and channel strides that are multiples of 16
+  (void)launch_conv_tiled(a, s);
 }
 void launch_conv_wgrad(const WgradArgs& a, int nparts, hipStream_t s) {
   hipLaunchKernelGGL(k_conv_wgrad, dim3(nparts, a.taps), dim3(256), 0, s, a);
@@ -436,30 +432,31 @@ void launch_bias_grad(const uint16_t* dout, int cs, int co0, int cout, size_t nv
 }
 void launch_relu_bwd(uint16_t* g, int gcs, int gco, const uint16_t* post, int pcs, int pco, int C, size_t nvox,
                      hipStream_t s) {
-  hipLaunchKernelGGL(k_relu_bwd, dim3(grid1(nvox * C)), dim3(256), 0, s, g, gcs, gco, post, pcs, pco, C, nvox);
+  const int ushift = C == 16 ? 1 : C == 32 ? 2 : 3;  // C in {16, 32, 64}
+  hipLaunchKernelGGL(k_relu_bwd, dim3(grid1(nvox * C / 8)), dim3(256), 0, s, g, gcs, gco, post, pcs, pco, ushift, nvox);
 }
 void launch_pool_fwd(const uint16_t* in, int ics, int ico, uint16_t* out, int ocs, int C, int N, int D, int H, int W,
                      hipStream_t s) {
-  hipLaunchKernelGGL(k_pool_fwd, dim3(grid1((size_t)N * D * H * W * C)), dim3(256), 0, s, in, ics, ico, out, ocs, C, N,
-                     D, H, W);
+  const dim3 grid((W * (C / 8) + 255) / 256, H, N * D);
+  hipLaunchKernelGGL(k_pool_fwd, grid, dim3(256), 0, s, in, ics, ico, out, ocs, C, D, H, W);
 }
 void launch_pool_bwd(const uint16_t* dout, int ocs, uint16_t* din, int ics, int ico, int C, int N, int D, int H, int W,
                      int accumulate, hipStream_t s) {
-  hipLaunchKernelGGL(k_pool_bwd, dim3(grid1((size_t)N * 8 * D * H * W * C)), dim3(256), 0, s, dout, ocs, din, ics,
-                     ico, C, N, D, H, W, accumulate);
+  const dim3 grid((2 * W * (C / 8) + 255) / 256, 2 * H, N * 2 * D);
+  hipLaunchKernelGGL(k_pool_bwd, grid, dim3(256), 0, s, dout, ocs, din, ics, ico, C, D, H, W, accumulate);
 }
 void launch_up_fwd(const uint16_t* in, int ics, uint16_t* out, int ocs, int oco, int C, int N, int D, int H, int W,
                    hipStream_t s) {
-  hipLaunchKernelGGL(k_up_fwd, dim3(grid1((size_t)N * 8 * D * H * W * C)), dim3(256), 0, s, in, ics, out, ocs, oco, C,
-                     N, D, H, W);
+  const dim3 grid((2 * W * (C / 8) + 255) / 256, 2 * H, N * 2 * D);
+  hipLaunchKernelGGL(k_up_fwd, grid, dim3(256), 0, s, in, ics, out, ocs, oco, C, D, H, W);
 }
 void launch_up_bwd(const uint16_t* dfine, int fcs, int fco, uint16_t* dcoarse, int ccs, int C, int N, int D, int H,
                    int W, hipStream_t s) {
-  hipLaunchKernelGGL(k_up_bwd, dim3(grid1((size_t)N * D * H * W * C)), dim3(256), 0, s, dfine, fcs, fco, dcoarse, ccs,
-                     C, N, D, H, W);
+  const dim3 grid((W * (C / 8) + 255) / 256, H, N * D);
+  hipLaunchKernelGGL(k_up_bwd, grid, dim3(256), 0, s, dfine, fcs, fco, dcoarse, ccs, C, D, H, W);
 }
 void launch_pack_input(const InputArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_pack_input, dim3(grid1((size_t)a.D * a.H * a.W)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_pack_input, dim3((a.W + 255) / 256, a.H, a.D), dim3(256), 0, s, a);
 }
 int launch_head(const HeadArgs& a, hipStream_t s) {
   const int nb = grid1(a.nvox, 1024);

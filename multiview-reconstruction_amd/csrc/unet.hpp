@@ -21,6 +21,10 @@ struct ConvArgs {
   int N, D, H, W;
   int taps;            // 27 (3^3, zero padding 1) or 1
   int relu;
+  // optional fused nearest x2 upsampling: input channels [0, up_c) are read from up_in (channel stride
+  // up_cs) at half the coordinates, channels [up_c, cin) from in (its channel 0 = input channel up_c)
+  const uint16_t* up_in;
+  int up_cs, up_c;
 };
 
 struct WgradArgs {
@@ -75,6 +79,8 @@ struct AdamArgs {
 };
 
 void launch_conv_fwd(const ConvArgs& a, hipStream_t s);
+// LDS-tiled 3^3 convolution (unet_conv.hip); false when the shape is not one it covers
+bool launch_conv_tiled(const ConvArgs& a, hipStream_t s);
 void launch_conv_wgrad(const WgradArgs& a, int nparts, hipStream_t s);
 void launch_sum_partials(const float* partial, int nparts, size_t count, float* grad, hipStream_t s);
 void launch_bias_grad(const uint16_t* dout, int cs, int co0, int cout, size_t nvox, float* partial, int nparts,

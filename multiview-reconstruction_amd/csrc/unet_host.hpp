@@ -71,8 +71,10 @@ struct UNet {
   int ensure_scratch();
   int pack(Acts& a, int sample, const std::vector<float*>& img, const std::vector<float*>& w, const int N3[3],
            const int o[3], float invScale);
+  // up != nullptr: input channels [0, up_c) are nearest x2 upsampled from up (channel stride up_cs), the
+  // rest come from in + in_co
   void conv(int l, const uint16_t* in, int in_cs, int in_co, uint16_t* out, int out_cs, int out_co, int N, int D,
-            int H, int W, bool relu);
+            int H, int W, bool relu, const uint16_t* up = nullptr, int up_cs = 0, int up_c = 0);
   void dgrad(int l, const uint16_t* dout, int dout_cs, int dout_co, uint16_t* din, int din_cs, int din_co, int N,
              int D, int H, int W);
   int wgrad(int l, const uint16_t* in, int in_cs, const uint16_t* dout, int dout_cs, int dout_co, int N, int D, int H,
