@@ -46,7 +46,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--autotune", type=int, default=0, help="autotune FFT lengths over this many candidates per axis")
+    ap.add_argument("--graph", action="store_true", help="replay a graph-captured iteration outside profiled steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-views", type=int, default=1)
     ap.add_argument("--iterations", type=int, default=0,
@@ -66,7 +68,7 @@ def main():
     from mvr import hip
     from mvr import synthetic as syn
 
-    w = syn.C2 if args.workload == "c2" else syn.C1
+    w = {"c1": syn.C1, "c2": syn.C2, "c3": syn.C3}[args.workload]
     if args.iterations > 0:
         import dataclasses
         w = dataclasses.replace(w, iterations=args.iterations)
@@ -94,6 +96,10 @@ def main():
     t0 = time.time()
     d.init()
     log(f"psf init + spectra + psi init in {time.time() - t0:.2f}s, fft grid {d.geometry()}")
+    if args.autotune:
+        P, ms = d.autotune(args.autotune)
+        log(f"autotuned FFT lengths {P}: {ms:.3f} ms per view update")
+    d.use_graph(args.graph)
 
     # warmup; the first warmup step times every pass to find the dominant kernel
     d.profile(None)
@@ -133,7 +139,7 @@ def main():
     finite = bool(np.isfinite(psi).all())
 
     learned = None
-    if not args.no_learned:
+    if not args.no_learned and args.workload == "c2":
         learned = time_to_converge(d, w, args.tol, args.reference_iterations)
     d.close()
 
@@ -144,11 +150,17 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         threads = oracle.set_threads(threads)
         sv = max(1, min(V, args.cpu_sample_views))
-        r = oracle.deconvolve(imgs, ws, ks, psf_type=w.psf_type, lam=w.lam, iterations=1, sample_views=sv)
+        block = None
+        if w.extra and "block_effective" in w.extra:
+            block = tuple(w.extra["block_effective"] + 2 * (k - 1) for k in K[::-1])
+        r = oracle.deconvolve(imgs, ws, ks, psf_type=w.psf_type, lam=w.lam, iterations=1, sample_views=sv,
+                              block_size=block)
         cpu_val = nvox * (sv / V) / r["loop_seconds"]
+        blocks = (f"{w.extra['block_effective']}^3 effective blocks of {block[0]}^3" if block else
+                  f"one {nx + 2 * (K[2] - 1)}x{ny + 2 * (K[1] - 1)}x{nz + 2 * (K[0] - 1)} block")
         cpu = {"value": cpu_val, "unit": "voxel-iters/s", "cores": threads, "kind": "port",
                "sample": f"{sv} of {V} view updates of iteration 1 of the same workload, reference block "
-                         f"structure (one {nx + 2 * (K[2] - 1)}^3 block, padded FFT), oracle restatement "
+                         f"structure ({blocks}, padded FFT), oracle restatement "
                          f"in FP32 on {threads} OpenMP threads; loop time {r['loop_seconds']:.2f}s"}
 
     out = {

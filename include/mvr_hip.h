@@ -126,6 +126,16 @@ int mvr_decon_profile(mvr_decon* d, unsigned mask);
 int mvr_decon_pass_times(mvr_decon* d, double msTotal[9], long long launches[9]);
 void mvr_decon_destroy(mvr_decon* d);
 
+/* ---- batching and scheduling of the resident run (no counterpart in the reference, whose iteration
+ * loop is MultiViewDeconvolution.runIterations, deconvolution/MultiViewDeconvolution.java:144-200) ----
+ * Autotuning: per axis the candidatesPerAxis smallest compiled FFT lengths >= extent + K - 1 are timed on
+ * view updates (coordinate descent over x, y, z); the fastest set replaces the plan and is kept for later
+ * mvr_decon_init calls.  Must follow mvr_decon_init; psi is restored to the initial estimate. */
+int mvr_decon_autotune(mvr_decon* d, int candidatesPerAxis, int chosenXYZ[3], double* msPerUpdate);
+/* enable != 0: each iteration (all views) is captured once as a HIP graph and replayed; the graph is
+ * rebuilt after mvr_decon_init / mvr_decon_autotune.  Pass timing (mvr_decon_profile) runs eagerly. */
+int mvr_decon_use_graph(mvr_decon* d, int enable);
+
 /* ---- convergence checks against a reference volume (FP64 device reductions).  The reference keeps
  * no iteration-wise distance; its IterationStatistics (deconvolution/DeconvolutionMethods.java:308)
  * are returned by mvr_decon_get_stats.  The reference volume is typically a long plain RL run. */

@@ -23,8 +23,9 @@ GEN = os.path.join(BUILD, "gen")
 
 # FFT lengths with compiled kernels.  Strided (y/z) lengths L; the x pass of a
 # length-P axis runs a length-P/2 complex transform, so P/2 is compiled for every P.
-STRIDED = [16, 20, 24, 32, 40, 48, 60, 64, 72, 80, 96, 100, 120, 128, 144, 160, 180, 192, 200, 216, 240, 256, 288,
-           300, 320, 360, 384, 400, 432, 480, 512, 576, 600, 640, 720, 768, 800, 864, 960, 1024, 1152, 1280]
+STRIDED = [16, 20, 24, 32, 40, 48, 60, 64, 72, 80, 96, 100, 120, 128, 144, 160, 180, 192, 200, 216, 240, 256, 270,
+           288, 300, 320, 360, 384, 400, 432, 450, 480, 512, 540, 576, 600, 640, 720, 768, 800, 864, 900, 960, 1024,
+           1080, 1152, 1280]
 HALF = sorted({p // 2 for p in STRIDED})
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -281,6 +281,16 @@ void mvr_decon_destroy(mvr_decon* d) {
   delete d;
 }
 
+int mvr_decon_autotune(mvr_decon* d, int candidatesPerAxis, int chosenXYZ[3], double* msPerUpdate) {
+  NULLCHECK(d);
+  return d->d.autotune(candidatesPerAxis, chosenXYZ, msPerUpdate);
+}
+
+int mvr_decon_use_graph(mvr_decon* d, int enable) {
+  NULLCHECK(d);
+  return d->d.set_graph(enable != 0);
+}
+
 int mvr_decon_init_learned(mvr_decon* d, mvr_unet* net) {
   NULLCHECK(d);
   NULLCHECK(net);

@@ -81,33 +81,8 @@ int Decon::init(int psiInit, const float* psiIn) {
   k1 = kraw;
   if (init_psfs(dev, k1, K, psfType, nT, k2, stream)) return -1;
   // plan over the whole psi volume, mirror-single extension at its faces
-  plan.destroy();
-  if (plan.create(N, K)) return -1;
-  for (auto p : spec1) if (p) (void)hipFree(p);
-  for (auto p : spec2) if (p) (void)hipFree(p);
-  spec1.assign(V, nullptr);
-  spec2.assign(V, nullptr);
-  const size_t nk = (size_t)K[0] * K[1] * K[2];
-  float* dk = nullptr;
-  MVR_HIP(hipMalloc(&dk, nk * 4));
-  for (int v = 0; v < V; ++v) {
-    MVR_HIP(hipMalloc(&spec1[v], plan.ncpx() * 8));
-    MVR_HIP(hipMalloc(&spec2[v], plan.ncpx() * 8));
-    MVR_HIP(hipMemcpyAsync(dk, k1[v].data(), nk * 4, hipMemcpyHostToDevice, stream));
-    if (plan.kernel_spectrum(dk, spec1[v], stream)) return -1;
-    MVR_HIP(hipMemcpyAsync(dk, k2[v].data(), nk * 4, hipMemcpyHostToDevice, stream));
-    if (plan.kernel_spectrum(dk, spec2[v], stream)) return -1;
-    MVR_HIP(hipStreamSynchronize(stream));
-  }
-  (void)hipFree(dk);
-  if (S) (void)hipFree(S);
-  MVR_HIP(hipMalloc(&S, plan.ncpx() * 8));
-  if (scheme == SCHEME_MUL) {
-    if (Spsi) (void)hipFree(Spsi);
-    if (T2) (void)hipFree(T2);
-    MVR_HIP(hipMalloc(&Spsi, plan.ncpx() * 8));
-    MVR_HIP(hipMalloc(&T2, nvox() * 4 * V));
-  }
+  drop_graph();
+  if (build_plan_and_spectra(tunedP[0] ? tunedP : nullptr, V)) return -1;
   // PsiInitAvgPrecise (D/init/PsiInitAvgPrecise.java:52-112)
   {
     const int maxParts = 1024;
@@ -154,6 +129,126 @@ int Decon::init(int psiInit, const float* psiIn) {
   return 0;
 }
 
+// Plan for the whole volume (FFT lengths >= Pmin where given), kernel spectra of the first nviews views,
+// work spectra.
+int Decon::build_plan_and_spectra(const int* Pmin, int nviews) {
+  plan.destroy();
+  if (plan.create(N, K, Pmin)) return -1;
+  for (auto p : spec1) if (p) (void)hipFree(p);
+  for (auto p : spec2) if (p) (void)hipFree(p);
+  spec1.assign(V, nullptr);
+  spec2.assign(V, nullptr);
+  const size_t nk = (size_t)K[0] * K[1] * K[2];
+  float* dk = nullptr;
+  MVR_HIP(hipMalloc(&dk, nk * 4));
+  for (int v = 0; v < nviews; ++v) {
+    MVR_HIP(hipMalloc(&spec1[v], plan.ncpx() * 8));
+    MVR_HIP(hipMalloc(&spec2[v], plan.ncpx() * 8));
+    MVR_HIP(hipMemcpyAsync(dk, k1[v].data(), nk * 4, hipMemcpyHostToDevice, stream));
+    if (plan.kernel_spectrum(dk, spec1[v], stream)) return -1;
+    MVR_HIP(hipMemcpyAsync(dk, k2[v].data(), nk * 4, hipMemcpyHostToDevice, stream));
+    if (plan.kernel_spectrum(dk, spec2[v], stream)) return -1;
+    MVR_HIP(hipStreamSynchronize(stream));
+  }
+  (void)hipFree(dk);
+  if (S) (void)hipFree(S);
+  MVR_HIP(hipMalloc(&S, plan.ncpx() * 8));
+  if (scheme == SCHEME_MUL) {
+    if (Spsi) (void)hipFree(Spsi);
+    if (T2) (void)hipFree(T2);
+    MVR_HIP(hipMalloc(&Spsi, plan.ncpx() * 8));
+    MVR_HIP(hipMalloc(&T2, nvox() * 4 * V));
+  }
+  return 0;
+}
+
+void Decon::drop_graph() {
+  if (graphExec) (void)hipGraphExecDestroy(graphExec);
+  graphExec = nullptr;
+}
+
+int Decon::set_graph(bool on) {
+  useGraph = on;
+  if (!on) drop_graph();
+  return 0;
+}
+
+// Coordinate descent over the compiled FFT lengths of each axis.  A trial rebuilds the plan and view 0's
+// spectra and times view updates (Seq) or iterations (Mul) of view 0 with HIP events; psi is restored
+// from psi0 afterwards.
+int Decon::autotune(int candidates, int chosen[3], double* msOut) {
+  MVR_CHECK(ready, "init first");
+  MVR_CHECK(candidates >= 1 && candidates <= 8, "candidates per axis must be in [1, 8]");
+  MVR_HIP(hipSetDevice(dev));
+  drop_graph();
+  if (ensure_partials((int)std::max<size_t>(2048, (size_t)N[1] * N[2] / 4 + 1))) return -1;
+  double2* st = nullptr;
+  MVR_HIP(hipMalloc(&st, sizeof(double2) * 4));
+  hipEvent_t e0, e1;
+  MVR_HIP(hipEventCreate(&e0));
+  MVR_HIP(hipEventCreate(&e1));
+  int best[3] = {0, 0, 0};
+  plan.destroy();
+  if (plan.create(N, K, nullptr)) return -1;
+  for (int d = 0; d < 3; ++d) best[d] = plan.dom.P[d];
+  double bestMs = 1e30;
+  auto trial = [&](const int P[3], double* ms) -> int {
+    if (build_plan_and_spectra(P, 1)) return -1;
+    const bool timing = timer.on;
+    timer.on = false;
+    int rc = 0;
+    for (int rep = 0; rep < 3 && !rc; ++rep) {  // first repetition warms up
+      if (rep == 1) MVR_HIP(hipEventRecord(e0, stream));
+      rc = scheme == SCHEME_SEQ ? view_seq(0, st) : iteration_mul(st);
+    }
+    timer.on = timing;
+    if (rc) return -1;
+    MVR_HIP(hipEventRecord(e1, stream));
+    MVR_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    MVR_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / 2.0;
+    return 0;
+  };
+  const int Vsave = V;
+  if (scheme == SCHEME_MUL) V = 1;  // time the shared forward transform + one view
+  int rc = 0;
+  if (trial(best, &bestMs)) rc = -1;
+  for (int d = 0; d < 3 && !rc; ++d) {
+    const int need = N[d] + K[d] - 1;
+    int tried = 1;
+    for (const int* p = strided_lengths(); *p && tried < candidates && !rc; ++p) {
+      if (*p <= best[d] || *p < need) continue;
+      if (d == 0) {
+        bool ok = false;
+        for (const int* h = half_lengths(); *h; ++h) ok |= (*h * 2 == *p);
+        if (!ok) continue;
+      }
+      int P[3] = {best[0], best[1], best[2]};
+      P[d] = *p;
+      double ms = 0;
+      if (trial(P, &ms)) { rc = -1; break; }
+      ++tried;
+      if (ms < bestMs) {
+        bestMs = ms;
+        best[d] = *p;
+      }
+    }
+  }
+  V = Vsave;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(st);
+  if (rc) return -1;
+  for (int d = 0; d < 3; ++d) tunedP[d] = best[d];
+  if (build_plan_and_spectra(tunedP, V)) return -1;
+  MVR_HIP(hipMemcpyAsync(psi, psi0, nvox() * 4, hipMemcpyDeviceToDevice, stream));
+  MVR_HIP(hipStreamSynchronize(stream));
+  if (chosen) for (int d = 0; d < 3; ++d) chosen[d] = best[d];
+  if (msOut) *msOut = bestMs;
+  return 0;
+}
+
 int Decon::reset_psi() {
   MVR_CHECK(ready, "init first");
   MVR_HIP(hipSetDevice(dev));
@@ -164,7 +259,7 @@ int Decon::reset_psi() {
 }
 
 // One view of one Seq iteration over the whole psi volume.
-int Decon::view_seq(int v, int statIdx) {
+int Decon::view_seq(int v, double2* statOut) {
   const Dom& d = plan.dom;
   const StridedGeom gy = plan.geom_y(), gz = plan.geom_z();
   GatherParams g{};
@@ -210,14 +305,14 @@ int Decon::view_seq(int v, int statIdx) {
   const int np = plan.xu(d, u, S, plan.twM, plan.wr, stream);
   timer.end(stream);
   timer.begin(stream, PASS_REDUCE);
-  launch_reduce_stats(partial, np, statsDev + statIdx, stream);
+  launch_reduce_stats(partial, np, statOut, stream);
   timer.end(stream);
   MVR_HIP(hipGetLastError());
   return 0;
 }
 
 // One Mul iteration (ComputeBlockMulThreadCPU.runIteration semantics on the whole volume).
-int Decon::iteration_mul(int statIdx) {
+int Decon::iteration_mul(double2* statOut) {
   const Dom& d = plan.dom;
   const StridedGeom gy = plan.geom_y(), gz = plan.geom_z();
   GatherParams g{};
@@ -271,7 +366,7 @@ int Decon::iteration_mul(int statIdx) {
   launch_update_mul(psi, t2p.data(), wp.data(), V, N, o, N, (double)lambda, minValue, (float)m, partial, &np, stream);
   timer.end(stream);
   timer.begin(stream, PASS_REDUCE);
-  launch_reduce_stats(partial, np, statsDev + statIdx, stream);
+  launch_reduce_stats(partial, np, statOut, stream);
   timer.end(stream);
   MVR_HIP(hipGetLastError());
   return 0;
@@ -290,12 +385,45 @@ int Decon::run(int iterations) {
   }
   const int rowsPerWG = 4;  // smallest x-pass row group
   if (ensure_partials((int)std::max<size_t>(2048, plan.rows_inner() / rowsPerWG + 1))) return -1;
-  for (int it = 0; it < iterations; ++it) {
-    if (scheme == SCHEME_SEQ) {
-      for (int v = 0; v < V; ++v)
-        if (view_seq(v, it * V + v)) return -1;
-    } else {
-      if (iteration_mul(it)) return -1;
+  if (useGraph && !timer.on && iterations > 0) {
+    if (!graphStats) MVR_HIP(hipMalloc(&graphStats, sizeof(double2) * V));
+    if (!graphExec) {
+      // capture one iteration (all views): kernel launches only, no allocation or host synchronisation
+      MVR_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      int rc = 0;
+      if (scheme == SCHEME_SEQ) {
+        for (int v = 0; v < V && !rc; ++v) rc = view_seq(v, graphStats + v);
+      } else {
+        rc = iteration_mul(graphStats);
+      }
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(stream, &g);
+      if (rc || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        if (!rc) set_error(std::string("graph capture failed: ") + hipGetErrorString(e));
+        return -1;
+      }
+      const hipError_t ie = hipGraphInstantiate(&graphExec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) {
+        graphExec = nullptr;
+        set_error(std::string("graph instantiation failed: ") + hipGetErrorString(ie));
+        return -1;
+      }
+    }
+    for (int it = 0; it < iterations; ++it) {
+      MVR_HIP(hipGraphLaunch(graphExec, stream));
+      MVR_HIP(hipMemcpyAsync(statsDev + (size_t)it * perIt, graphStats, sizeof(double2) * perIt,
+                             hipMemcpyDeviceToDevice, stream));
+    }
+  } else {
+    for (int it = 0; it < iterations; ++it) {
+      if (scheme == SCHEME_SEQ) {
+        for (int v = 0; v < V; ++v)
+          if (view_seq(v, statsDev + (size_t)it * V + v)) return -1;
+      } else {
+        if (iteration_mul(statsDev + it)) return -1;
+      }
     }
   }
   std::vector<double2> h(std::max(need, 1));
@@ -359,9 +487,11 @@ void Decon::destroy() {
   for (auto p : w) if (p) (void)hipFree(p);
   for (auto p : spec1) if (p) (void)hipFree(p);
   for (auto p : spec2) if (p) (void)hipFree(p);
+  drop_graph();
   for (void* p : {(void*)psi, (void*)psi0, (void*)S, (void*)Spsi, (void*)T2, (void*)partial, (void*)statsDev,
-                  (void*)ref, (void*)relDev})
+                  (void*)ref, (void*)relDev, (void*)graphStats})
     if (p) (void)hipFree(p);
+  graphStats = nullptr;
   img.clear(); w.clear(); spec1.clear(); spec2.clear();
   psi = psi0 = nullptr; S = Spsi = nullptr; T2 = nullptr; partial = nullptr; statsDev = nullptr;
   ref = nullptr; relDev = nullptr;

@@ -117,6 +117,14 @@ struct Decon {
   float* ref = nullptr;
   double2* relDev = nullptr;
 
+  // one iteration captured as a HIP graph and replayed (kernel arguments, including max[] and the
+  // spectra pointers, are baked in: init and autotune drop the graph)
+  bool useGraph = false;
+  hipGraphExec_t graphExec = nullptr;
+  double2* graphStats = nullptr;  // per-view (Seq) or per-iteration (Mul) stats of the replayed iteration
+  // FFT lengths chosen by autotune (0 = the smallest compiled length)
+  int tunedP[3] = {0, 0, 0};
+
   int create(int dev, const int N[3], int V, int psfType, int scheme, float lambda, float minValue, int nT);
   int set_view(int v, const float* img, const float* w, const float* kernel, const int K[3]);
   int init(int psiInit, const float* psiIn);
@@ -127,12 +135,18 @@ struct Decon {
   int set_reference(const float* r);
   int rel_l2(double* out);
   int run_until(int maxIterations, double tol, int* iterationsOut, double* relOut);
+  // FFT length autotuning: per axis, the `candidates` smallest compiled lengths >= D + K - 1 are timed on
+  // view updates (coordinate descent x, y, z); the fastest set is kept and used by later init calls
+  int autotune(int candidates, int chosen[3], double* msPerViewUpdate);
+  int set_graph(bool on);
   void destroy();
 
   size_t nvox() const { return (size_t)N[0] * N[1] * N[2]; }
   int ensure_partials(int n);
-  int view_seq(int v, int statIdx);
-  int iteration_mul(int statIdx);
+  int build_plan_and_spectra(const int* Pmin, int nviews);
+  void drop_graph();
+  int view_seq(int v, double2* statOut);
+  int iteration_mul(double2* statOut);
 };
 
 // ---------------------------------------------------------------------------

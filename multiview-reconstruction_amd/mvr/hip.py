@@ -82,6 +82,8 @@ def lib():
         L.mvr_decon_rel_l2.argtypes = [vp, _f64p]
         L.mvr_decon_run_until.argtypes = [vp, ctypes.c_int, ctypes.c_double, _i32p, _f64p]
         L.mvr_decon_init_learned.argtypes = [vp, vp]
+        L.mvr_decon_autotune.argtypes = [vp, ctypes.c_int, _i32p, _f64p]
+        L.mvr_decon_use_graph.argtypes = [vp, ctypes.c_int]
         L.mvr_unet_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong]
         L.mvr_unet_param_count.argtypes = [vp]
         L.mvr_unet_param_count.restype = ctypes.c_longlong
@@ -282,6 +284,16 @@ class Decon:
         n = (ctypes.c_longlong * 9)()
         _check(lib().mvr_decon_pass_times(self.h, ms, n))
         return {PASSES[i]: (ms[i], n[i]) for i in range(9)}
+
+    def autotune(self, candidates=3):
+        """Times candidate FFT lengths per axis and keeps the fastest; returns ((Px, Py, Pz), ms per update)."""
+        ch = (ctypes.c_int * 3)()
+        ms = ctypes.c_double()
+        _check(lib().mvr_decon_autotune(self.h, candidates, ch, ctypes.byref(ms)))
+        return (ch[0], ch[1], ch[2]), ms.value
+
+    def use_graph(self, enable=True):
+        _check(lib().mvr_decon_use_graph(self.h, 1 if enable else 0))
 
     def init_learned(self, net: "UNet"):
         """PsiInit from the network (PsiInitFromRAI pattern: max[] and avg from the AvgPrecise pass)."""

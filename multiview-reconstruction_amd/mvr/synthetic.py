@@ -53,6 +53,13 @@ C1 = Workload("c1_128cube_3views_eb", (128, 128, 128), 3, 31, 1.5, 4.5, [0, 120,
               "uniform", 0, "EFFICIENT_BAYESIAN", 0.0, 10, 1)
 C2 = Workload("c2_256cube_6views_opt2", (256, 256, 256), 6, 41, 1.8, 5.0, [0, 60, 120, 180, 240, 300], 1500, 2, 6,
               1000.0, "zramp", 20, "OPTIMIZATION_II", 0.0006, 20, 2)
+# Views of uneven extents (x, y, z) placed centred in a common 400 x 352 x 224 frame; each view covers
+# only its box (image 0 and weight 0 outside), weights blend along z at the box faces.  The reference
+# runs it in 128^3 effective blocks (block size 128 + 2(K - 1) = 216).
+C3 = Workload("c3_400x352x224_4views_eb", (224, 352, 400), 4, 45, 1.8, 5.0, [0, 90, 180, 270], 3000, 2, 6, None,
+              "zramp_box", 20, "EFFICIENT_BAYESIAN", 0.0006, 30, 3,
+              extra={"extents_xyz": [(384, 320, 200), (352, 352, 180), (400, 300, 220), (360, 330, 190)],
+                     "block_effective": 128})
 
 
 def small_workload(n=24, views=3, k=7, psf_type="EFFICIENT_BAYESIAN", lam=0.0, iterations=3, weights="uniform",
@@ -168,12 +175,24 @@ def normalize_weights(ws):
     return out
 
 
+def view_boxes(w: Workload):
+    """Per-view covered box ((z0, z1), (y0, y1), (x0, x1)), centred in the frame, or None."""
+    ext = w.extra.get("extents_xyz") if w.extra else None
+    if not ext:
+        return None
+    out = []
+    for ex, ey, ez in ext[:w.n_views]:
+        out.append(tuple(((n - e) // 2, (n - e) // 2 + e) for n, e in zip(w.shape_zyx, (ez, ey, ex))))
+    return out
+
+
 def make_inputs(w: Workload):
     """Returns dict(phantom f64, imgs [V,z,y,x] f32, weights f32, psfs [V,k,k,k] f32)."""
     V = w.n_views
     ph = blob_phantom(w.shape_zyx, w.n_blobs, w.r_min, w.r_max, 1000 + w.seed_index)
     psfs = np.stack([gaussian_psf(w.psf_size, w.sigma_xy, w.sigma_z, a) for a in w.angles_deg[:V]])
     imgs = np.empty((V,) + tuple(w.shape_zyx), np.float32)
+    boxes = view_boxes(w)
     for v in range(V):
         b = convolve_mirror_f64(ph, psfs[v])
         if w.peak is not None:
@@ -181,8 +200,20 @@ def make_inputs(w: Workload):
         rng = np.random.default_rng(2000 + 100 * w.seed_index + v)
         noisy = rng.poisson(np.maximum(b, 0.0)).astype(np.float64)
         imgs[v] = np.maximum(1.0, noisy).astype(np.float32)
+        if boxes is not None:  # uncovered voxels are 0 (outsideValueImg)
+            (z0, z1), (y0, y1), (x0, x1) = boxes[v]
+            cover = np.zeros(w.shape_zyx, bool)
+            cover[z0:z1, y0:y1, x0:x1] = True
+            imgs[v][~cover] = 0.0
     if w.weights == "uniform":
         weights = np.ones_like(imgs)
+    elif w.weights == "zramp_box":
+        raw = np.zeros(imgs.shape, np.float32)
+        for v in range(V):
+            (z0, z1), (y0, y1), (x0, x1) = boxes[v]
+            wz = blend_weight_1d(z1 - z0, w.blend_range)
+            raw[v, z0:z1, y0:y1, x0:x1] = wz[:, None, None]
+        weights = np.ascontiguousarray(normalize_weights(raw))
     else:
         wz = blend_weight_1d(w.shape_zyx[0], w.blend_range)
         raw = np.broadcast_to(wz[None, :, None, None], imgs.shape).astype(np.float32)

@@ -8,11 +8,12 @@ difference from a result difference.
                     oracle's FP32 psi after 3 iterations for 8 variants
                     (4 PSF types, Seq/Mul, lambda 0 / 6e-4, nT 4 / 8), FP64 psi for two of them.
   <workload>.npz  : the full-size workloads (128^3 x 3 views EB 10 it;
-                    256^3 x 6 views Opt-II 20 it): oracle FP32 and FP64 psi
-                    subsampled every 8th (16th) voxel per axis, full-volume sums
-                    and norms, per-view maxima and the average.
+                    256^3 x 6 views Opt-II 20 it; 400x352x224 x 4 uneven views EB
+                    10 it in 128^3 effective blocks): oracle FP32 (and FP64 for
+                    c1/c2) psi subsampled every 8th (16th) voxel per axis,
+                    full-volume sums and norms, per-view maxima and the average.
 
-Run:  python tests/golden/make_golden.py [small] [c1] [c2]
+Run:  python tests/golden/make_golden.py [small] [c1] [c2] [c3]
 """
 import os
 import sys
@@ -62,15 +63,25 @@ def make_small():
     np.savez_compressed(os.path.join(HERE, "small.npz"), **out)
 
 
-def make_full(name):
-    w = {"c1": syn.C1, "c2": syn.C2}[name]
-    step = 8 if name == "c1" else 16
+def make_full(name, precisions=("f32", "f64")):
+    w = {"c1": syn.C1, "c2": syn.C2, "c3": syn.C3}[name]
+    step = {"c1": 8, "c2": 16, "c3": 8}[name]
     inp = syn.make_inputs(w)
     imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
-    out = dict(input_checksums=checksums(imgs, ws, ks), step=np.array([step]), iterations=np.array([w.iterations]))
+    # c3: 10 iterations (the parity bar's count) keep the CPU run near 20 minutes
+    iterations = 10 if name == "c3" else w.iterations
+    out = dict(input_checksums=checksums(imgs, ws, ks), step=np.array([step]), iterations=np.array([iterations]))
+    # c3 runs at the reference's tiling: 128^3 effective blocks, block size 128 + 2(K - 1)
+    block = None
+    if w.extra and "block_effective" in w.extra:
+        block = tuple(w.extra["block_effective"] + 2 * (w.psf_size - 1) for _ in range(3))
+        out["block_size"] = np.array(block)
     for tag, dt in (("f32", np.float32), ("f64", np.float64)):
+        if tag not in precisions:
+            continue
         t = time.time()
-        r = oracle.deconvolve(imgs, ws, ks, psf_type=w.psf_type, lam=w.lam, iterations=w.iterations, dtype=dt)
+        r = oracle.deconvolve(imgs, ws, ks, psf_type=w.psf_type, lam=w.lam, iterations=iterations, dtype=dt,
+                              block_size=block)
         psi = r["psi"].astype(np.float64)
         out[f"sub_{tag}"] = psi[::step, ::step, ::step].copy()
         out[f"sum_{tag}"] = np.array([psi.sum()])
@@ -84,10 +95,12 @@ def make_full(name):
 
 if __name__ == "__main__":
     oracle.build()
-    oracle.set_threads(os.cpu_count() or 1)
+    oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1)
     todo = sys.argv[1:] or ["small", "c1", "c2"]
     if "small" in todo:
         make_small()
     for n in ("c1", "c2"):
         if n in todo:
             make_full(n)
+    if "c3" in todo:
+        make_full("c3", ("f32",))

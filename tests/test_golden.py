@@ -65,7 +65,7 @@ def test_hip_reproduces_small_fixture():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["c1", "c2"])
+@pytest.mark.parametrize("name", ["c1", "c2", "c3"])
 def test_hip_full_size_workload(name):
     path = os.path.join(HERE, f"{name}.npz")
     if not os.path.exists(path):
@@ -73,7 +73,7 @@ def test_hip_full_size_workload(name):
     from mvr import hip
     from mvr import synthetic as syn
     z = np.load(path, allow_pickle=False)
-    w = {"c1": syn.C1, "c2": syn.C2}[name]
+    w = {"c1": syn.C1, "c2": syn.C2, "c3": syn.C3}[name]
     inp = syn.make_inputs(w)
     imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
     cs = np.array([imgs.astype(np.float64).sum(), ws.astype(np.float64).sum(), ks.astype(np.float64).sum()])
@@ -85,15 +85,21 @@ def test_hip_full_size_workload(name):
     d.init()
     mx, avg = d.max_avg()
     np.testing.assert_array_equal(mx, z["max_f32"])
-    d.run(w.iterations)
+    d.run(int(z["iterations"][0]))
     psi = d.psi().astype(np.float64)
     d.close()
     s = int(z["step"][0])
     sub = psi[::s, ::s, ::s]
-    e32, e64 = rel(sub, z["sub_f32"]), rel(sub, z["sub_f64"])
-    print(f"{name}: subsampled rel-L2 vs oracle FP32 {e32:.2e}, vs FP64 {e64:.2e}; "
-          f"oracle FP32 vs FP64 {rel(z['sub_f32'], z['sub_f64']):.2e}")
+    e32 = rel(sub, z["sub_f32"])
+    if "sub_f64" in z.files:
+        e64 = rel(sub, z["sub_f64"])
+        print(f"{name}: subsampled rel-L2 vs oracle FP32 {e32:.2e}, vs FP64 {e64:.2e}; "
+              f"oracle FP32 vs FP64 {rel(z['sub_f32'], z['sub_f64']):.2e}")
+        assert e64 < 1e-4
+    else:
+        # the oracle ran the reference's blocked driver (block_size); the GPU runs one resident domain
+        print(f"{name}: subsampled rel-L2 vs oracle FP32 (blocks {z['block_size']}) {e32:.2e}")
     assert e32 < 1e-4
-    assert e64 < 1e-4
-    assert psi.sum() == pytest.approx(float(z["sum_f64"][0]), rel=1e-5)
-    assert np.linalg.norm(psi) == pytest.approx(float(z["norm_f64"][0]), rel=1e-5)
+    tag = "f64" if "sum_f64" in z.files else "f32"
+    assert psi.sum() == pytest.approx(float(z[f"sum_{tag}"][0]), rel=1e-5)
+    assert np.linalg.norm(psi) == pytest.approx(float(z[f"norm_{tag}"][0]), rel=1e-5)

@@ -35,6 +35,8 @@ def test_device_query():
     ((12, 10, 30), (3, 4, 6), "value", 0.0),
     ((33, 7, 40), (9, 3, 11), "mirror", 0.0),
     ((1, 8, 8), (1, 3, 3), "mirror", 0.0),
+    ((264, 6, 444), (7, 3, 7), "mirror", 0.0),   # FFT lengths 270 (z) and 450 (x: 225-point halves)
+    ((10, 264, 84), (5, 7, 7), "value", 1.0),    # 270 along y
 ])
 def test_fft_convolve_matches_oracle(oracle, shape, kshape, ext, value):
     img = RNG.uniform(0.5, 2.0, shape).astype(np.float32)
@@ -141,6 +143,49 @@ def test_resident_decon_matches_oracle(oracle, psf_type, scheme, lam):
     assert e32 < 1e-4
     assert e64 < 1e-4
     assert d.stats().shape == ((it, 3, 2) if scheme == "SEQ" else (it, 1, 2))
+    d.close()
+
+
+@pytest.mark.parametrize("scheme", ["SEQ", "MUL"])
+def test_graph_replay_matches_eager(scheme):
+    # one captured iteration replayed: the same kernels in the same order, so psi and stats are identical;
+    # the replays are interleaved with eager host work and synchronisations
+    inp = _small_problem(weights="zramp")
+    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+    out = []
+    for graph in (False, True):
+        d = hip.Decon(imgs.shape[1:], 3, "EFFICIENT_BAYESIAN", scheme, 6e-4)
+        for v in range(3):
+            d.set_view(v, imgs[v], ws[v], ks[v])
+        d.init()
+        d.use_graph(graph)
+        d.run(2)
+        mid = d.psi()
+        d.run(3)
+        out.append((mid, d.psi(), d.stats()))
+        d.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+
+
+def test_autotuned_lengths_keep_parity(oracle):
+    inp = _small_problem(weights="zramp")
+    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+    ref = oracle.deconvolve(imgs, ws, ks, psf_type="OPTIMIZATION_II", lam=6e-4, iterations=5)
+    d = hip.Decon(imgs.shape[1:], 3, "OPTIMIZATION_II", "SEQ", 6e-4)
+    for v in range(3):
+        d.set_view(v, imgs[v], ws[v], ks[v])
+    d.init()
+    P, ms = d.autotune(3)
+    need = [n + k - 1 for n, k in zip(imgs.shape[1:][::-1], ks.shape[1:][::-1])]
+    print(f"autotuned FFT lengths {P} (need >= {need}), {ms:.3f} ms per view update")
+    assert all(p >= q for p, q in zip(P, need))
+    assert d.geometry()["P"] == P
+    d.run(5)
+    assert rel(d.psi(), ref["psi"]) < 1e-4
+    d.init()  # a later init keeps the tuned lengths
+    assert d.geometry()["P"] == P
     d.close()
 
 
