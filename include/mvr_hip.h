@@ -33,6 +33,13 @@ extern "C" {
  * PsiInitFromRAI with the precise max (deconvolution/init/PsiInitFromRAI.java:65-82) */
 #define MVR_PSI_INIT_AVG_PRECISE 0
 #define MVR_PSI_INIT_FROM_ARRAY 1
+/* PsiInitAvgApprox (deconvolution/init/PsiInitAvgApprox.java:47-99, PsiInitAvgApproxThread.java:57-85):
+ * avg and max[] from each view's middle x hyperslice */
+#define MVR_PSI_INIT_AVG_APPROX 2
+/* PsiInitBlurredFused (deconvolution/init/PsiInitBlurredFused.java:62-127,
+ * deconvolution/util/FusedNonZeroRandomAccess.java:57-96): weighted fusion of the positive views blurred
+ * with a Gaussian of sigma 5 (mirror-single borders); max[] and avg as AvgPrecise */
+#define MVR_PSI_INIT_BLURRED_FUSED 3
 /* extension modes of mvr_fft_convolve (util/FFTConvolution.java usage in
  * ComputeBlockSeqThreadCPU.java:171-209 and DeconViewPSF.java:152-225) */
 #define MVR_EXT_MIRROR_SINGLE 0

@@ -205,7 +205,8 @@ int mvr_decon_set_view(mvr_decon* d, int v, const float* img, const float* w, co
 
 int mvr_decon_init(mvr_decon* d, int psiInit, const float* psi0) {
   NULLCHECK(d);
-  if (psiInit != MVR_PSI_INIT_AVG_PRECISE && psiInit != MVR_PSI_INIT_FROM_ARRAY) {
+  if (psiInit != MVR_PSI_INIT_AVG_PRECISE && psiInit != MVR_PSI_INIT_FROM_ARRAY && psiInit != MVR_PSI_INIT_AVG_APPROX &&
+      psiInit != MVR_PSI_INIT_BLURRED_FUSED) {
     set_error("unknown psi init");
     return -1;
   }

@@ -115,12 +115,65 @@ int Decon::init(int psiInit, const float* psiIn) {
     }
     avg = s / (double)c;
     if (std::isnan(avg)) avg = 1.0;
+    psiCount = c;
   }
-  if (psiInit == 0) {
+  if (psiInit == PSI_INIT_AVG_PRECISE) {
     launch_fill(psi, nvox(), (float)avg, stream);
-  } else {
+  } else if (psiInit == PSI_INIT_FROM_ARRAY) {
     MVR_CHECK(psiIn != nullptr, "psi init from array needs an array");
     MVR_HIP(hipMemcpyAsync(psi, psiIn, nvox() * 4, hipMemcpyDefault, stream));
+  } else if (psiInit == PSI_INIT_AVG_APPROX) {
+    // PsiInitAvgApprox.runInitialization (D/init/PsiInitAvgApprox.java:47-99): per view the mean and max
+    // of its middle x hyperslice; avg = mean of the view means, max[v] = the slice max
+    double2* part = nullptr;
+    MVR_HIP(hipMalloc(&part, sizeof(double2) * 256));
+    double a = 0.0;
+    for (int v = 0; v < V; ++v) {
+      const int np = launch_psi_approx_slice(img[v], N, part, stream);
+      MVR_HIP(hipGetLastError());
+      std::vector<double2> h(np);
+      MVR_HIP(hipMemcpyAsync(h.data(), part, sizeof(double2) * np, hipMemcpyDeviceToHost, stream));
+      MVR_HIP(hipStreamSynchronize(stream));
+      double sum = 0.0, mx = -1.0e308;
+      for (const double2& p : h) { sum += p.x; mx = std::max(mx, p.y); }
+      a += sum / ((double)N[1] * N[2]);
+      maxI[v] = (float)mx;
+    }
+    (void)hipFree(part);
+    avg = a / (double)V;
+    if (std::isnan(avg)) avg = 1.0;
+    launch_fill(psi, nvox(), (float)avg, stream);
+  } else if (psiInit == PSI_INIT_BLURRED_FUSED) {
+    // PsiInitBlurredFused.runInitialization (D/init/PsiInitBlurredFused.java:62-127): psi = weighted mean
+    // of the positive views, blurred by a Gaussian of sigma 5 with mirror-single borders (separable, the
+    // normalised half kernel of ImgLib2's Gauss3: H = max(2, (int)(3 sigma + 0.5) + 1) taps,
+    // k[x] = exp(-x^2 / (2 sigma^2)) / (2 (0.5 + sum_{x>0} exp(...)))).  max[] and avg are the AvgPrecise ones.
+    MVR_CHECK(psiCount > 0, "none of the views covers the deconvolved area");
+    const double sigma = 5.0;
+    const int H = std::max(2, (int)(3 * sigma + 0.5) + 1);
+    std::vector<double> half(H);
+    half[0] = 1.0;
+    double sum = 0.5;
+    for (int x = 1; x < H; ++x) {
+      half[x] = std::exp(-(double)(x * x) / (2 * sigma * sigma));
+      sum += half[x];
+    }
+    sum *= 2;
+    for (double& k : half) k /= sum;
+    double* dh = nullptr;
+    MVR_HIP(hipMalloc(&dh, sizeof(double) * H));
+    MVR_HIP(hipMemcpyAsync(dh, half.data(), sizeof(double) * H, hipMemcpyHostToDevice, stream));
+    float* tmp = reinterpret_cast<float*>(S);  // the work spectrum holds >= 4 bytes per voxel
+    std::vector<const float*> ip(img.begin(), img.end()), wp(w.begin(), w.end());
+    launch_fused_nonzero(ip.data(), wp.data(), V, nvox(), tmp, stream);
+    launch_sym_conv_axis(tmp, psi, N, 0, dh, H, stream);
+    launch_sym_conv_axis(psi, tmp, N, 1, dh, H, stream);
+    launch_sym_conv_axis(tmp, psi, N, 2, dh, H, stream);
+    MVR_HIP(hipGetLastError());
+    MVR_HIP(hipStreamSynchronize(stream));
+    (void)hipFree(dh);
+  } else {
+    MVR_CHECK(false, "unknown psi init");
   }
   MVR_HIP(hipMemcpyAsync(psi0, psi, nvox() * 4, hipMemcpyDeviceToDevice, stream));
   MVR_HIP(hipStreamSynchronize(stream));

@@ -100,5 +100,10 @@ void launch_update_mul(float* psi, const float* const* T2, const float* const* w
                        int* nPartialOut, hipStream_t s);
 void launch_scale_copy_kernel_spectrum(const float2* in, float2* out, size_t n, float scale, hipStream_t s);
 void launch_rel_l2(const float* a, const float* b, size_t n, double2* partial, double2* out, hipStream_t s);
+int launch_psi_approx_slice(const float* img, const int N[3], double2* partial, hipStream_t s);
+void launch_fused_nonzero(const float* const* imgs, const float* const* ws, int V, size_t n, float* out,
+                          hipStream_t s);
+void launch_sym_conv_axis(const float* in, float* out, const int N[3], int axis, const double* half, int H,
+                          hipStream_t s);
 
 }  // namespace mvr

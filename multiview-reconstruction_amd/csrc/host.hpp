@@ -63,6 +63,7 @@ void mirror_all(float* a, const int dim[3]);
 
 enum { PSF_OPT_II = 0, PSF_OPT_I = 1, PSF_EFFICIENT_BAYESIAN = 2, PSF_INDEPENDENT = 3 };
 enum { SCHEME_SEQ = 0, SCHEME_MUL = 1 };
+enum { PSI_INIT_AVG_PRECISE = 0, PSI_INIT_FROM_ARRAY = 1, PSI_INIT_AVG_APPROX = 2, PSI_INIT_BLURRED_FUSED = 3 };
 
 // per-pass timing categories (resident driver)
 enum Pass { PASS_X_FWD = 0, PASS_Z_FWD, PASS_Y_CONV, PASS_Z_INV, PASS_X_RATIO, PASS_X_UPDATE, PASS_X_STORE,
@@ -110,6 +111,7 @@ struct Decon {
   std::vector<double> stats;
   std::vector<float> maxI;
   double avg = 0;
+  unsigned long long psiCount = 0;  // voxels with a positive view value (AvgPrecise pass)
   bool ready = false;
   Timer timer;
 

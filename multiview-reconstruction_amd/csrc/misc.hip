@@ -96,6 +96,95 @@ int launch_psi_init_partials(const float* const* imgs, int V, size_t n, double* 
   return nb;
 }
 
+// PsiInitAvgApproxThread.call (D/init/PsiInitAvgApproxThread.java:57-85): min/max/sum over the hyperslice
+// x = nx / 2 of one view (all values, zeros included).  The reference visits that slice once per
+// dimension and divides by the tripled count, which is the slice mean.  Partials per work-group:
+// {sum, max}; the host reduces them in a fixed order.
+__global__ __launch_bounds__(kThreads) void k_psi_approx_slice(const float* __restrict__ img, int3 N,
+                                                               double2* __restrict__ partial) {
+  __shared__ double ss[kThreads], sm[kThreads];
+  const int x = N.x / 2;
+  const size_t n = (size_t)N.y * N.z;
+  double sum = 0.0, mx = -1.0e308;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+    const double v = (double)img[i * N.x + x];  // i = z * ny + y
+    sum += v;
+    mx = fmax(mx, v);
+  }
+  ss[threadIdx.x] = sum;
+  sm[threadIdx.x] = mx;
+  __syncthreads();
+  for (int off = kThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      ss[threadIdx.x] += ss[threadIdx.x + off];
+      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(ss[0], sm[0]);
+}
+
+int launch_psi_approx_slice(const float* img, const int N[3], double2* partial, hipStream_t s) {
+  const size_t n = (size_t)N[1] * N[2];
+  const int nb = (int)std::max<size_t>(1, std::min<size_t>((n + kThreads - 1) / kThreads, 256));
+  hipLaunchKernelGGL(k_psi_approx_slice, dim3(nb), dim3(kThreads), 0, s, img, make_int3(N[0], N[1], N[2]), partial);
+  return nb;
+}
+
+// FusedNonZeroRandomAccess.get (D/util/FusedNonZeroRandomAccess.java:57-96): the weighted mean of the
+// positive view values, sum(I w) / sum(w) in double, 0 where no positive view has weight.
+struct ViewPtrs2 { const float* img[kMaxViews]; const float* w[kMaxViews]; };
+__global__ void k_fused_nonzero(ViewPtrs2 vp, int V, size_t n, float* __restrict__ out) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double sI = 0.0, sW = 0.0;
+    for (int v = 0; v < V; ++v) {
+      const double val = (double)vp.img[v][i];
+      if (val > 0.0) {
+        const double wt = (double)vp.w[v][i];
+        sI += val * wt;
+        sW += wt;
+      }
+    }
+    out[i] = sW > 0.0 ? (float)(sI / sW) : 0.f;
+  }
+}
+
+void launch_fused_nonzero(const float* const* imgs, const float* const* ws, int V, size_t n, float* out,
+                          hipStream_t s) {
+  ViewPtrs2 vp{};
+  for (int v = 0; v < V && v < kMaxViews; ++v) { vp.img[v] = imgs[v]; vp.w[v] = ws[v]; }
+  const int nb = (int)std::max<size_t>(1, std::min<size_t>((n + kThreads - 1) / kThreads, 8192));
+  hipLaunchKernelGGL(k_fused_nonzero, dim3(nb), dim3(kThreads), 0, s, vp, V, n, out);
+}
+
+// One axis of a separable symmetric convolution with mirror-single extension (the Gaussian blur of
+// PsiInitBlurredFused, D/init/PsiInitBlurredFused.java:113-114): out[p] = sum_{|j| < H} k[|j|] in[mirror(p + j)]
+// along `axis`, accumulated in double; mirror-single has period 2n - 2 along the axis.
+__global__ void k_sym_conv_axis(const float* __restrict__ in, float* __restrict__ out, int3 N, int axis,
+                                const double* __restrict__ half, int H) {
+  const size_t n = (size_t)N.x * N.y * N.z;
+  const int len = axis == 0 ? N.x : axis == 1 ? N.y : N.z;
+  const size_t stride = axis == 0 ? 1 : axis == 1 ? (size_t)N.x : (size_t)N.x * N.y;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int pos = (int)((i / stride) % (size_t)len);
+    const float* base = in + (i - (size_t)pos * stride);
+    double acc = half[0] * (double)base[(size_t)pos * stride];
+    for (int j = 1; j < H; ++j) {
+      const int a = mirror_index(pos - j, len), b = mirror_index(pos + j, len);
+      acc += half[j] * ((double)base[(size_t)a * stride] + (double)base[(size_t)b * stride]);
+    }
+    out[i] = (float)acc;
+  }
+}
+
+void launch_sym_conv_axis(const float* in, float* out, const int N[3], int axis, const double* half, int H,
+                          hipStream_t s) {
+  const size_t n = (size_t)N[0] * N[1] * N[2];
+  const int nb = (int)std::max<size_t>(1, std::min<size_t>((n + kThreads - 1) / kThreads, 8192));
+  hipLaunchKernelGGL(k_sym_conv_axis, dim3(nb), dim3(kThreads), 0, s, in, out, make_int3(N[0], N[1], N[2]), axis, half,
+                     H);
+}
+
 // Mul-scheme update (DeconvolutionMethods.computeNextValueMul,
 // D/iteration/sequential/DeconvolutionMethods.java:370-419): geometric mean of the
 // per-view integrals (double product, pow 1/V), min(1, sum of weights).

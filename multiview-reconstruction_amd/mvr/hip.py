@@ -227,9 +227,12 @@ class Decon:
         _check(lib().mvr_decon_set_view(self.h, view, ctypes.c_void_p(img_ptr), ctypes.c_void_p(weight_ptr),
                                         _ptr(kernel), _dims(kernel.shape)))
 
-    def init(self, psi0=None):
+    PSI_INITS = {"AVG_PRECISE": 0, "FROM_ARRAY": 1, "AVG_APPROX": 2, "BLURRED_FUSED": 3}
+
+    def init(self, psi0=None, mode="AVG_PRECISE"):
+        """PsiInit: AVG_PRECISE (default), AVG_APPROX, BLURRED_FUSED, or FROM_ARRAY when psi0 is given."""
         if psi0 is None:
-            _check(lib().mvr_decon_init(self.h, 0, None))
+            _check(lib().mvr_decon_init(self.h, self.PSI_INITS[mode], None))
         else:
             p = _f32(psi0)
             _check(lib().mvr_decon_init(self.h, 1, _ptr(p)))

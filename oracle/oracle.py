@@ -21,6 +21,9 @@ init_psfs (4 PSF types)        delta-PSF known answers, scipy.signal.convolve(mo
 next_value / next_value_mul    closed forms + exhaustive special cases (<=0, NaN, lambda=0, w=0/1)
 divide_into_blocks/sort_blocks known-answer tables incl. the reference Block.main inputs
 psi_init_avg_precise           closed form with numpy
+psi_init_avg_approx            closed form (slice mean / max), invariance to voxels off the slice
+psi_init_blurred_fused         constant images, delta response = separable kernel product, kernel
+                               sum 1, borders against numpy reflect padding + direct convolution
 deconvolve (Seq/Mul)           block-independence invariant, numpy/scipy global-form iteration,
                                FP32-vs-FP64 agreement, flux preservation
 =============================  ==============================================================
@@ -231,6 +234,59 @@ def psi_init_avg_precise(imgs: np.ndarray, nT: int):
     mx = np.zeros(V, np.float32)
     avg = lib().oracle_psi_init_avg_precise(imgs.ravel(), V, imgs[0].size, nT, mx)
     return avg, mx
+
+
+def psi_init_avg_approx(imgs):
+    """PsiInitAvgApprox (src/main/java/net/preibisch/mvrecon/process/deconvolution/init/PsiInitAvgApprox.java:47-99,
+    PsiInitAvgApproxThread.java:57-85): per view, min/max/sum over the hyperslice x = nx/2, visited once per
+    dimension (3 passes, the count tripled); avg = mean over views of sum/count; max[v] = (float) slice max.
+    Returns (avg, max[V]).  FP64 numpy."""
+    imgs = np.asarray(imgs, np.float32)
+    V = imgs.shape[0]
+    means = np.zeros(V)
+    mx = np.zeros(V, np.float32)
+    for v in range(V):
+        sl = imgs[v][:, :, imgs.shape[3] // 2].astype(np.float64)
+        total, count = 0.0, 0
+        for _ in range(3):
+            total += float(sl.sum())
+            count += sl.size
+        means[v] = total / count
+        mx[v] = np.float32(sl.max())
+    avg = float(means.sum() / V)
+    return (1.0 if np.isnan(avg) else avg), mx
+
+
+def gauss3_half_kernel(sigma):
+    """Half kernel of ImgLib2's Gauss3 (imglib2-algorithm, version managed by pom-scijava 43.0.0; the
+    published Gauss3.halfkernelsizes / halfkernel): H = max(2, (int)(3 sigma + 0.5) + 1) taps,
+    k[0] = 1, k[x] = exp(-x^2 / (2 sigma^2)), normalised so that the symmetric kernel sums to 1."""
+    H = max(2, int(3 * sigma + 0.5) + 1)
+    k = np.ones(H)
+    for x in range(1, H):
+        k[x] = np.exp(-(x * x) / (2 * sigma * sigma))
+    total = 2 * (0.5 + k[1:].sum())
+    return k / total
+
+
+def psi_init_blurred_fused(imgs, weights, sigma=5.0):
+    """PsiInitBlurredFused (src/main/java/net/preibisch/mvrecon/process/deconvolution/init/PsiInitBlurredFused.java:62-127,
+    deconvolution/util/FusedNonZeroRandomAccess.java:57-96): psi = sum(I w) / sum(w) over views with I > 0
+    (0 without weight), stored as float, then Gauss3 with mirror-single borders along x, y, z.  FP64.
+    Returns psi (float64)."""
+    from scipy import ndimage
+    imgs = np.asarray(imgs, np.float32).astype(np.float64)
+    weights = np.asarray(weights, np.float32).astype(np.float64)
+    pos = imgs > 0
+    sI = np.where(pos, imgs * weights, 0.0).sum(0)
+    sW = np.where(pos, weights, 0.0).sum(0)
+    fused = np.where(sW > 0, sI / np.where(sW > 0, sW, 1.0), 0.0).astype(np.float32).astype(np.float64)
+    half = gauss3_half_kernel(sigma)
+    full = np.concatenate([half[:0:-1], half])
+    out = fused
+    for axis in (2, 1, 0):
+        out = ndimage.correlate1d(out, full, axis=axis, mode="mirror")
+    return out
 
 
 def block_seq(psi_blk, img_blk, w_blk, k1, k2, lam, min_value, max_i, nT):

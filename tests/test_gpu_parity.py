@@ -189,6 +189,47 @@ def test_autotuned_lengths_keep_parity(oracle):
     d.close()
 
 
+def test_psi_init_avg_approx(oracle):
+    inp = _small_problem(weights="zramp")
+    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+    avg, mx = oracle.psi_init_avg_approx(imgs)
+    d = hip.Decon(imgs.shape[1:], 3)
+    for v in range(3):
+        d.set_view(v, imgs[v], ws[v], ks[v])
+    d.init(mode="AVG_APPROX")
+    gmx, gavg = d.max_avg()
+    np.testing.assert_array_equal(gmx, mx)
+    assert gavg == pytest.approx(avg, rel=1e-12)
+    assert np.all(d.psi() == np.float32(avg))
+    ref = oracle.deconvolve(imgs, ws, ks, iterations=3, psi0=d.psi())
+    d.run(3)
+    # the oracle driver takes max[] from AvgPrecise; with lambda 0 max[] does not enter the update
+    assert rel(d.psi(), ref["psi"]) < 1e-4
+    d.close()
+
+
+def test_psi_init_blurred_fused(oracle):
+    inp = _small_problem(weights="zramp")
+    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+    ref = oracle.psi_init_blurred_fused(imgs, ws, 5.0)
+    avg, mx = oracle.psi_init_avg_precise(imgs, 8)
+    d = hip.Decon(imgs.shape[1:], 3, "OPTIMIZATION_II", "SEQ", 6e-4)
+    for v in range(3):
+        d.set_view(v, imgs[v], ws[v], ks[v])
+    d.init(mode="BLURRED_FUSED")
+    psi = d.psi()
+    e = rel(psi, ref)
+    print(f"blurred fused init: rel-L2 vs FP64 oracle {e:.2e}")
+    assert e < 1e-6
+    gmx, gavg = d.max_avg()
+    np.testing.assert_array_equal(gmx, mx)
+    assert gavg == pytest.approx(avg, rel=1e-12)
+    out = oracle.deconvolve(imgs, ws, ks, psf_type="OPTIMIZATION_II", lam=6e-4, iterations=10, psi0=psi)
+    d.run(10)
+    assert rel(d.psi(), out["psi"]) < 1e-4
+    d.close()
+
+
 def test_resident_psi_init_from_array(oracle):
     inp = _small_problem()
     imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]

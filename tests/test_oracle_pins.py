@@ -399,3 +399,66 @@ def test_stats_are_signed_net_change(oracle):
     assert r["stats"].shape == (1, 3, 2)
     assert np.all(r["stats"][0, :, 1] >= -1)
     assert r0["psi"].std() == 0.0
+
+
+# ---------------------------------------------------------------- PsiInitAvgApprox / PsiInitBlurredFused
+def test_avg_approx_closed_form(oracle):
+    rng = np.random.default_rng(31)
+    imgs = rng.uniform(0, 50, (2, 5, 6, 7)).astype(np.float32)
+    imgs[0, :, :, 3] = np.arange(30, dtype=np.float32).reshape(5, 6)  # x = 7 // 2 = 3
+    imgs[1, :, :, 3] = 4.0
+    avg, mx = oracle.psi_init_avg_approx(imgs)
+    assert avg == pytest.approx((14.5 + 4.0) / 2, rel=1e-14)
+    assert mx[0] == 29.0 and mx[1] == 4.0
+    # voxels off the middle slice do not matter
+    imgs2 = imgs.copy()
+    imgs2[:, :, :, :3] = 1e6
+    assert oracle.psi_init_avg_approx(imgs2)[0] == avg
+
+
+def test_gauss3_half_kernel():
+    import oracle as o
+    for sigma in (0.5, 1.0, 2.3, 5.0):
+        k = o.gauss3_half_kernel(sigma)
+        assert len(k) == max(2, int(3 * sigma + 0.5) + 1)
+        assert k[0] + 2 * k[1:].sum() == pytest.approx(1.0, abs=1e-14)
+        assert np.all(np.diff(k) < 0)
+    k = o.gauss3_half_kernel(5.0)
+    assert len(k) == 16 and k[3] / k[0] == pytest.approx(np.exp(-9 / 50), rel=1e-14)
+
+
+def test_blurred_fused_constant_and_delta(oracle):
+    # constant positive views: the fusion and its blur are that constant
+    imgs = np.full((2, 9, 10, 11), 7.0, np.float32)
+    ws = np.full_like(imgs, 0.5)
+    assert np.abs(oracle.psi_init_blurred_fused(imgs, ws, 2.0) - 7.0).max() < 1e-12
+    # one bright voxel far from the borders: the response is the separable product of the kernel
+    imgs = np.zeros((1, 21, 21, 21), np.float32)
+    imgs[0, 10, 10, 10] = 1.0
+    ws = np.ones_like(imgs)
+    out = oracle.psi_init_blurred_fused(imgs, ws, 1.5)
+    k = oracle.gauss3_half_kernel(1.5)
+    for dz, dy, dx in ((0, 0, 0), (1, 0, 0), (2, 3, 1), (0, 4, 4)):
+        assert out[10 + dz, 10 + dy, 10 + dx] == pytest.approx(k[dz] * k[dy] * k[dx], rel=1e-12)
+    # positive-only fusion: a zero view does not pull the weighted mean down
+    imgs = np.stack([np.full((4, 4, 4), 3.0, np.float32), np.zeros((4, 4, 4), np.float32)])
+    assert np.abs(oracle.psi_init_blurred_fused(imgs, np.ones_like(imgs), 1.0) - 3.0).max() < 1e-12
+
+
+def test_blurred_fused_borders_against_reflect_padding(oracle):
+    rng = np.random.default_rng(5)
+    imgs = rng.uniform(0.5, 9.0, (2, 6, 7, 8)).astype(np.float32)
+    ws = rng.uniform(0.1, 1.0, imgs.shape).astype(np.float32)
+    sigma = 1.2
+    got = oracle.psi_init_blurred_fused(imgs, ws, sigma)
+    fused = ((imgs.astype(np.float64) * ws).sum(0) / ws.astype(np.float64).sum(0)).astype(np.float32).astype(np.float64)
+    k = oracle.gauss3_half_kernel(sigma)
+    full = np.concatenate([k[:0:-1], k])
+    H = len(k) - 1
+    out = fused
+    for axis in (2, 1, 0):
+        pad = [(0, 0)] * 3
+        pad[axis] = (H, H)
+        p = np.pad(out, pad, mode="reflect")  # mirror-single: d c b | a b c d
+        out = np.apply_along_axis(lambda r: np.convolve(r, full, mode="valid"), axis, p)
+    assert np.abs(got - out).max() < 1e-12
