@@ -231,6 +231,18 @@ def time_to_converge(d, w, tol, ref_iterations):
     plain = _trajectory(d, 0.0, ref_iterations)
     res = {"tol": tol, "reference": f"plain RL, {ref_iterations} iterations from PsiInitAvgPrecise",
            "plain": {"rel_l2_start": plain0[1], "rel_l2_after_20": plain[19][1]}}
+    # PsiInitBlurredFused, the reference GUI's default initialiser: its fusion + blur time is the init
+    # difference to AvgPrecise (both inits also rebuild PSFs and spectra)
+    t0 = time.perf_counter()
+    d.init()
+    t_avg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    d.init(mode="BLURRED_FUSED")
+    t_blur = max(0.0, time.perf_counter() - t0 - t_avg)
+    blurred0 = (t_blur, d.rel_l2())
+    blurred = _trajectory(d, t_blur, ref_iterations)
+    res["blurred_fused"] = {"init_seconds_over_avg_precise": t_blur, "rel_l2_start": blurred0[1],
+                            "rel_l2_after_20": blurred[19][1]}
     if not os.path.exists(L.DEFAULT_WEIGHTS):
         res["learned"] = None
         res["note"] = "no trained weights file"
@@ -256,8 +268,11 @@ def time_to_converge(d, w, tol, ref_iterations):
     for t in sorted(set((tol,) + TOLERANCES)):
         ip, tp = _first_below(plain, plain0, t)
         il, tl = _first_below(learned, learned0, t)
-        table.append({"tol": t, "plain_iterations": ip, "plain_seconds": tp, "learned_iterations": il,
-                      "learned_seconds": tl, "speedup": (tp / tl) if (tp and tl) else None})
+        ib, tb = _first_below(blurred, blurred0, t)
+        table.append({"tol": t, "plain_iterations": ip, "plain_seconds": tp, "blurred_fused_iterations": ib,
+                      "blurred_fused_seconds": tb, "learned_iterations": il, "learned_seconds": tl,
+                      "speedup": (tp / tl) if (tp and tl) else None,
+                      "speedup_vs_blurred_fused": (tb / tl) if (tb and tl) else None})
     res["by_tolerance"] = table
     head = next(r for r in table if r["tol"] == tol)
     res["plain"].update(iterations=head["plain_iterations"], seconds=head["plain_seconds"])

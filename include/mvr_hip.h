@@ -133,6 +133,26 @@ int mvr_decon_profile(mvr_decon* d, unsigned mask);
 int mvr_decon_pass_times(mvr_decon* d, double msTotal[9], long long launches[9]);
 void mvr_decon_destroy(mvr_decon* d);
 
+/* ---- input preparation (deconvolution/util/ProcessInputImages.java:279-399): each view resampled into the
+ * deconvolution bounding box, its blending weights, and the weights normalised across views.
+ * affine = the view's registration (input pixel -> world) as a 3 x 4 row-major matrix; the bounding box
+ * is given by its world minimum bbMin and size bbXYZ; outputs are bbXYZ volumes. ---- */
+/* TransformView.transformView (fusion/transformed/TransformView.java:59-73, TransformedInputRandomAccess.java:61-84):
+ * outside the strict interior of the input -> outsideValue; inside, linear (interpolation 1) or nearest (0)
+ * interpolation, clamped to >= minValue when hasMinValue (MultiViewDeconvolution.minValueImg = 1) */
+int mvr_transform_view(int dev, const float* img, const int imgXYZ[3], const double affine[12], const long long bbMin[3],
+                       const int bbXYZ[3], int hasMinValue, float minValue, float outsideValue, int interpolation,
+                       float* out);
+/* TransformWeight.transformBlending (fusion/transformed/TransformWeight.java:82-112, 114-134) with
+ * BlendingRealRandomAccess.computeWeight (fusion/transformed/weights/BlendingRealRandomAccess.java:95-127)
+ * on the input interval [0, imgXYZ - 1]; border and blending per dimension (after FusionTools.adjustBlending) */
+int mvr_transform_blending(int dev, const int imgXYZ[3], const float border[3], const float blending[3],
+                           const double affine[12], const long long bbMin[3], const int bbXYZ[3], float* out);
+/* NormalizingRandomAccess.get for all views (deconvolution/normalization/NormalizingRandomAccess.java:76-108,
+ * 183-214): weights[v] (nXYZ each) are replaced in place */
+int mvr_normalize_weights(int dev, float* const* weights, int numViews, const int nXYZ[3], double osemSpeedup,
+                          int additionalSmoothBlending, float maxDiffRange, float scalingRange);
+
 /* ---- batching and scheduling of the resident run (no counterpart in the reference, whose iteration
  * loop is MultiViewDeconvolution.runIterations, deconvolution/MultiViewDeconvolution.java:144-200) ----
  * Autotuning: per axis the candidatesPerAxis smallest compiled FFT lengths >= extent + K - 1 are timed on

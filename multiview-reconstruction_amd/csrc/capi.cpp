@@ -1,10 +1,12 @@
 // capi.cpp -- extern "C" entry points declared in include/mvr_hip.h.
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <vector>
 
 #include "../../include/mvr_hip.h"
 #include "host.hpp"
+#include "prep.hpp"
 #include "unet_host.hpp"
 
 using namespace mvr;
@@ -320,6 +322,126 @@ int mvr_decon_rel_l2(mvr_decon* d, double* out) {
 int mvr_decon_run_until(mvr_decon* d, int maxIterations, double tol, int* iterationsOut, double* relOut) {
   NULLCHECK(d);
   return d->d.run_until(maxIterations, tol, iterationsOut, relOut);
+}
+
+// ---------------------------------------------------------------- input preparation
+namespace {
+
+// inverse of a 3 x 4 affine (input pixel -> world) in double, as AffineTransform3D keeps it
+int invert_affine(const double* f, Affine3& inv) {
+  const double a = f[0], b = f[1], c = f[2], d = f[4], e = f[5], g = f[6], h = f[8], i = f[9], k = f[10];
+  const double det = a * (e * k - g * i) - b * (d * k - g * h) + c * (d * i - e * h);
+  if (!(std::fabs(det) > 0)) { set_error("affine transform is singular"); return -1; }
+  const double r[9] = {(e * k - g * i) / det, (c * i - b * k) / det, (b * g - c * e) / det,
+                       (g * h - d * k) / det, (a * k - c * h) / det, (c * d - a * g) / det,
+                       (d * i - e * h) / det, (b * h - a * i) / det, (a * e - b * d) / det};
+  for (int row = 0; row < 3; ++row) {
+    inv.m[4 * row + 0] = r[3 * row + 0];
+    inv.m[4 * row + 1] = r[3 * row + 1];
+    inv.m[4 * row + 2] = r[3 * row + 2];
+    inv.m[4 * row + 3] = -(r[3 * row + 0] * f[3] + r[3 * row + 1] * f[7] + r[3 * row + 2] * f[11]);
+  }
+  return 0;
+}
+
+// BlendingRealRandomAccess's static table (fusion/transformed/weights/BlendingRealRandomAccess.java:51-57),
+// uploaded once per device
+int ensure_blend_lut(int dev) {
+  static bool done[64] = {false};
+  if (dev < 0 || dev >= 64) { set_error("bad device"); return -1; }
+  if (done[dev]) return 0;
+  std::vector<double> lut(1001, 0.0);
+  for (double x = 0; x <= 1.0001; x = x + 0.001) lut[(int)(x * 1000.0 + 0.5)] = (std::cos((1 - x) * M_PI) + 1) / 2;
+  if (upload_blend_lut(lut.data())) { set_error("blend table upload failed"); return -1; }
+  done[dev] = true;
+  return 0;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+int mvr_transform_view(int dev, const float* img, const int imgXYZ[3], const double affine[12], const long long bbMin[3],
+                       const int bbXYZ[3], int hasMinValue, float minValue, float outsideValue, int interpolation,
+                       float* out) {
+  NULLCHECK(img);
+  NULLCHECK(imgXYZ);
+  NULLCHECK(affine);
+  NULLCHECK(bbMin);
+  NULLCHECK(bbXYZ);
+  NULLCHECK(out);
+  for (int d = 0; d < 3; ++d)
+    if (imgXYZ[d] < 1 || bbXYZ[d] < 1) { set_error("empty image or bounding box"); return -1; }
+  if (interpolation != 0 && interpolation != 1) { set_error("interpolation must be 0 or 1"); return -1; }
+  Affine3 inv;
+  if (invert_affine(affine, inv)) return -1;
+  MVR_HIP(hipSetDevice(dev));
+  const size_t ni = (size_t)imgXYZ[0] * imgXYZ[1] * imgXYZ[2], no = (size_t)bbXYZ[0] * bbXYZ[1] * bbXYZ[2];
+  DevBuf di, dout;
+  MVR_HIP(hipMalloc(&di.p, ni * 4));
+  MVR_HIP(hipMalloc(&dout.p, no * 4));
+  MVR_HIP(hipMemcpy(di.p, img, ni * 4, hipMemcpyDefault));
+  if (launch_transform_view((const float*)di.p, imgXYZ, inv, bbMin, bbXYZ, hasMinValue, minValue, outsideValue,
+                            interpolation, (float*)dout.p, nullptr)) {
+    set_error("transform kernel launch failed");
+    return -1;
+  }
+  MVR_HIP(hipMemcpy(out, dout.p, no * 4, hipMemcpyDefault));
+  return 0;
+}
+
+int mvr_transform_blending(int dev, const int imgXYZ[3], const float border[3], const float blending[3],
+                           const double affine[12], const long long bbMin[3], const int bbXYZ[3], float* out) {
+  NULLCHECK(imgXYZ);
+  NULLCHECK(border);
+  NULLCHECK(blending);
+  NULLCHECK(affine);
+  NULLCHECK(bbMin);
+  NULLCHECK(bbXYZ);
+  NULLCHECK(out);
+  for (int d = 0; d < 3; ++d)
+    if (imgXYZ[d] < 1 || bbXYZ[d] < 1) { set_error("empty image or bounding box"); return -1; }
+  Affine3 inv;
+  if (invert_affine(affine, inv)) return -1;
+  MVR_HIP(hipSetDevice(dev));
+  if (ensure_blend_lut(dev)) return -1;
+  const size_t no = (size_t)bbXYZ[0] * bbXYZ[1] * bbXYZ[2];
+  DevBuf dout;
+  MVR_HIP(hipMalloc(&dout.p, no * 4));
+  if (launch_transform_blending(imgXYZ, border, blending, inv, bbMin, bbXYZ, (float*)dout.p, nullptr)) {
+    set_error("blending kernel launch failed");
+    return -1;
+  }
+  MVR_HIP(hipMemcpy(out, dout.p, no * 4, hipMemcpyDefault));
+  return 0;
+}
+
+int mvr_normalize_weights(int dev, float* const* weights, int numViews, const int nXYZ[3], double osemSpeedup,
+                          int additionalSmoothBlending, float maxDiffRange, float scalingRange) {
+  NULLCHECK(weights);
+  NULLCHECK(nXYZ);
+  if (numViews < 1 || numViews > kPrepMaxViews) { set_error("numViews must be in [1, 32]"); return -1; }
+  for (int v = 0; v < numViews; ++v)
+    if (!weights[v]) { set_error("null weight array"); return -1; }
+  MVR_HIP(hipSetDevice(dev));
+  const size_t n = (size_t)nXYZ[0] * nXYZ[1] * nXYZ[2];
+  std::vector<DevBuf> bufs(numViews);
+  std::vector<float*> dp(numViews);
+  for (int v = 0; v < numViews; ++v) {
+    MVR_HIP(hipMalloc(&bufs[v].p, n * 4));
+    MVR_HIP(hipMemcpy(bufs[v].p, weights[v], n * 4, hipMemcpyDefault));
+    dp[v] = (float*)bufs[v].p;
+  }
+  if (launch_normalize_weights(dp.data(), numViews, n, osemSpeedup, additionalSmoothBlending, maxDiffRange,
+                               scalingRange, nullptr)) {
+    set_error("normalisation kernel launch failed");
+    return -1;
+  }
+  for (int v = 0; v < numViews; ++v) MVR_HIP(hipMemcpy(weights[v], dp[v], n * 4, hipMemcpyDefault));
+  return 0;
 }
 
 // ---------------------------------------------------------------- learned initial estimate

@@ -1,0 +1,160 @@
+// prep.hip -- input preparation for the deconvolution on the GPU: affine resampling of a view into the
+// deconvolution frame, its blending weights, and the normalisation of the weights across views.
+//
+// References (src/main/java/net/preibisch/mvrecon/process/):
+//   fusion/transformed/TransformView.java (transformView, linear or nearest interpolation),
+//   fusion/transformed/TransformedInputRandomAccess.java:61-84 + AbstractTransformedImgRandomAccess.java
+//     (world -> input pixel coordinates by the inverse affine; strictly-inside test; minValue clamp;
+//     outside value), AbstractTransformedIntervalRandomAccess.intersectsLinearInterpolation,
+//   fusion/transformed/TransformWeight.transformBlending + weights/BlendingRealRandomAccess.java:46-130,
+//   deconvolution/normalization/NormalizingRandomAccess.java:76-108, 183-214 (hard / smooth weights, OSEM).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "prep.hpp"
+
+namespace mvr {
+
+namespace {
+
+// world -> input pixel coordinates: t = A s + b (the inverse of the view's registration)
+__device__ __forceinline__ void apply_inv(const Affine3& a, double sx, double sy, double sz, double& tx, double& ty,
+                                          double& tz) {
+  tx = a.m[0] * sx + a.m[1] * sy + a.m[2] * sz + a.m[3];
+  ty = a.m[4] * sx + a.m[5] * sy + a.m[6] * sz + a.m[7];
+  tz = a.m[8] * sx + a.m[9] * sy + a.m[10] * sz + a.m[11];
+}
+
+__global__ void k_transform_view(const float* __restrict__ img, int3 in, Affine3 inv, long3 off, int3 outDim,
+                                 int hasMin, float minValue, float outside, int linear, float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+  if (x >= outDim.x) return;
+  double tx, ty, tz;
+  apply_inv(inv, (double)(x + off.x), (double)(y + off.y), (double)(z + off.z), tx, ty, tz);
+  float v = outside;
+  // strictly inside (0, n - 1) in every dimension
+  if (tx > 0.0 && ty > 0.0 && tz > 0.0 && tx < in.x - 1 && ty < in.y - 1 && tz < in.z - 1) {
+    double val;
+    if (linear) {
+      const int x0 = (int)floor(tx), y0 = (int)floor(ty), z0 = (int)floor(tz);
+      const double fx = tx - x0, fy = ty - y0, fz = tz - z0;
+      const float* p = img + ((size_t)z0 * in.y + y0) * in.x + x0;
+      const size_t sy = in.x, sz = (size_t)in.x * in.y;
+      const double c00 = (1 - fx) * p[0] + fx * p[1];
+      const double c10 = (1 - fx) * p[sy] + fx * p[sy + 1];
+      const double c01 = (1 - fx) * p[sz] + fx * p[sz + 1];
+      const double c11 = (1 - fx) * p[sz + sy] + fx * p[sz + sy + 1];
+      val = (1 - fz) * ((1 - fy) * c00 + fy * c10) + fz * ((1 - fy) * c01 + fy * c11);
+    } else {
+      // nearest neighbour: the closest pixel, ties rounded up (floor(t + 0.5))
+      const int xi = (int)floor(tx + 0.5), yi = (int)floor(ty + 0.5), zi = (int)floor(tz + 0.5);
+      val = img[((size_t)zi * in.y + yi) * in.x + xi];
+    }
+    v = (float)val;
+    if (hasMin) v = fmaxf(minValue, v);
+  }
+  out[((size_t)z * outDim.y + y) * outDim.x + x] = v;
+}
+
+__constant__ double c_blendLut[1001];
+
+// BlendingRealRandomAccess.computeWeight at the (float) input location, interval [0, dimMinus1]
+__global__ void k_transform_blending(int3 dimMinus1, float3 border, float3 blending, Affine3 inv, long3 off,
+                                     int3 outDim, float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+  if (x >= outDim.x) return;
+  double tx, ty, tz;
+  apply_inv(inv, (double)(x + off.x), (double)(y + off.y), (double)(z + off.z), tx, ty, tz);
+  const float l[3] = {(float)tx, (float)ty, (float)tz};
+  const int dm[3] = {dimMinus1.x, dimMinus1.y, dimMinus1.z};
+  const float bo[3] = {border.x, border.y, border.z};
+  const float bl[3] = {blending.x, blending.y, blending.z};
+  float dist[3];
+  float w = 1.f;
+  bool zero = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    dist[d] = fminf(__fsub_rn(l[d], bo[d]), __fsub_rn(__fsub_rn((float)dm[d], l[d]), bo[d]));
+    zero |= dist[d] <= 0.f;
+  }
+  if (!zero) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float rel = __fdiv_rn(dist[d], bl[d]);
+      if (rel < 1.f) w = (float)((double)w * c_blendLut[(int)((double)rel * 1000.0 + 0.5)]);
+    }
+  } else {
+    w = 0.f;
+  }
+  out[((size_t)z * outDim.y + y) * outDim.x + x] = w;
+}
+
+struct WeightPtrs { float* p[kPrepMaxViews]; };
+
+// NormalizingRandomAccess.get for every view at once (in place)
+__global__ void k_normalize_weights(WeightPtrs w, int V, size_t n, double osem, int smooth, float maxDiffRange,
+                                    float scalingRange) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double sumW = 0.0;
+    for (int v = 0; v < V; ++v) sumW += fmin(1.0, (double)w.p[v][i]);
+    for (int v = 0; v < V; ++v) {
+      const float my = (float)fmin(1.0, (double)w.p[v][i]);
+      double val;
+      if (smooth) {
+        if (sumW <= 0) {
+          val = 0.0;
+        } else {
+          const float ideal = (float)((double)my / sumW);
+          const float diff = __fsub_rn(my, ideal);
+          const float yv = fmaxf(0.f, __fmul_rn(__fsub_rn(maxDiffRange, fabsf(diff)), __fdiv_rn(1.0f, maxDiffRange)));
+          const float scale = __fmul_rn(__fmul_rn(yv, my), scalingRange);
+          val = (double)__fsub_rn(fminf(my, ideal), scale);
+        }
+      } else if (sumW > 1) {
+        val = (double)(float)((double)my / sumW);
+      } else {
+        val = (double)my;
+      }
+      w.p[v][i] = (float)fmin(1.0, val * osem);
+    }
+  }
+}
+
+}  // namespace
+
+int launch_transform_view(const float* img, const int inXYZ[3], const Affine3& inv, const long long off[3],
+                          const int outXYZ[3], int hasMin, float minValue, float outside, int linear, float* out,
+                          hipStream_t s) {
+  const dim3 grid((outXYZ[0] + 255) / 256, outXYZ[1], outXYZ[2]);
+  hipLaunchKernelGGL(k_transform_view, grid, dim3(256), 0, s, img, make_int3(inXYZ[0], inXYZ[1], inXYZ[2]), inv,
+                     make_long3(off[0], off[1], off[2]), make_int3(outXYZ[0], outXYZ[1], outXYZ[2]), hasMin, minValue,
+                     outside, linear, out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int upload_blend_lut(const double* lut) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_blendLut), lut, sizeof(double) * 1001) == hipSuccess ? 0 : -1;
+}
+
+int launch_transform_blending(const int inXYZ[3], const float border[3], const float blending[3], const Affine3& inv,
+                              const long long off[3], const int outXYZ[3], float* out, hipStream_t s) {
+  const dim3 grid((outXYZ[0] + 255) / 256, outXYZ[1], outXYZ[2]);
+  hipLaunchKernelGGL(k_transform_blending, grid, dim3(256), 0, s, make_int3(inXYZ[0] - 1, inXYZ[1] - 1, inXYZ[2] - 1),
+                     make_float3(border[0], border[1], border[2]), make_float3(blending[0], blending[1], blending[2]),
+                     inv, make_long3(off[0], off[1], off[2]), make_int3(outXYZ[0], outXYZ[1], outXYZ[2]), out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_normalize_weights(float* const* w, int V, size_t n, double osem, int smooth, float maxDiffRange,
+                             float scalingRange, hipStream_t s) {
+  WeightPtrs p{};
+  for (int v = 0; v < V && v < kPrepMaxViews; ++v) p.p[v] = w[v];
+  const int nb = (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192));
+  hipLaunchKernelGGL(k_normalize_weights, dim3(nb), dim3(256), 0, s, p, V, n, osem, smooth, maxDiffRange,
+                     scalingRange);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace mvr

@@ -84,6 +84,12 @@ def lib():
         L.mvr_decon_init_learned.argtypes = [vp, vp]
         L.mvr_decon_autotune.argtypes = [vp, ctypes.c_int, _i32p, _f64p]
         L.mvr_decon_use_graph.argtypes = [vp, ctypes.c_int]
+        L.mvr_transform_view.argtypes = [ctypes.c_int, vp, _i32p, _f64p, ctypes.POINTER(ctypes.c_longlong), _i32p,
+                                         ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, vp]
+        L.mvr_transform_blending.argtypes = [ctypes.c_int, _i32p, _f32p, _f32p, _f64p,
+                                             ctypes.POINTER(ctypes.c_longlong), _i32p, vp]
+        L.mvr_normalize_weights.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int, _i32p, ctypes.c_double,
+                                            ctypes.c_int, ctypes.c_float, ctypes.c_float]
         L.mvr_unet_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong]
         L.mvr_unet_param_count.argtypes = [vp]
         L.mvr_unet_param_count.restype = ctypes.c_longlong
@@ -408,3 +414,42 @@ class UNet:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- input preparation
+def _affine(a):
+    a = np.ascontiguousarray(np.asarray(a, np.float64).reshape(3, 4))
+    return a.ctypes.data_as(_f64p), a
+
+
+def transform_view(img, affine, bb_min_xyz, bb_shape_zyx, min_value=1.0, has_min=True, outside=0.0,
+                   interpolation=1, dev=0):
+    """TransformView.transformView: the view resampled into the bounding box (z, y, x array).
+    affine: 3 x 4 input pixel (x, y, z) -> world map; bb_min_xyz: world minimum of the box."""
+    img = _f32(img)
+    out = np.empty(tuple(bb_shape_zyx), np.float32)
+    ap, keep = _affine(affine)
+    bb = (ctypes.c_longlong * 3)(*[int(v) for v in bb_min_xyz])
+    _check(lib().mvr_transform_view(dev, _ptr(img), _dims(img.shape), ap, bb, _dims(out.shape), 1 if has_min else 0,
+                                    min_value, outside, interpolation, _ptr(out)))
+    return out
+
+
+def transform_blending(img_shape_zyx, border_xyz, blending_xyz, affine, bb_min_xyz, bb_shape_zyx, dev=0):
+    """TransformWeight.transformBlending: blending weights of a view in the bounding box (z, y, x array)."""
+    out = np.empty(tuple(bb_shape_zyx), np.float32)
+    ap, keep = _affine(affine)
+    bb = (ctypes.c_longlong * 3)(*[int(v) for v in bb_min_xyz])
+    bo = (ctypes.c_float * 3)(*border_xyz)
+    bl = (ctypes.c_float * 3)(*blending_xyz)
+    _check(lib().mvr_transform_blending(dev, _dims(img_shape_zyx), bo, bl, ap, bb, _dims(out.shape), _ptr(out)))
+    return out
+
+
+def normalize_weights(weights, osem_speedup=1.0, smooth=False, max_diff_range=0.1, scaling_range=0.05, dev=0):
+    """NormalizingRandomAccess for all views; returns the normalised weights [V, z, y, x]."""
+    w = np.array(weights, np.float32, copy=True)
+    ptrs = (ctypes.c_void_p * w.shape[0])(*[w[v].ctypes.data for v in range(w.shape[0])])
+    _check(lib().mvr_normalize_weights(dev, ptrs, w.shape[0], _dims(w.shape[1:]), osem_speedup, 1 if smooth else 0,
+                                       max_diff_range, scaling_range))
+    return w

@@ -1,0 +1,140 @@
+"""Input preparation: the numpy restatement (oracle/prep_ref.py) pinned on the CPU, and the HIP kernels
+(affine resampling, blending weights, weight normalisation) checked against it through the C ABI.
+
+Tolerances: resampled values rel-L2 <= 1e-6 (FP64 vs FP32 interpolation; the rounding order of ImgLib2's
+interpolator is third-party and not pinned), identical inside/outside masks; blending weights and
+normalised weights within 1e-6 absolute.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import prep_ref as P
+
+RNG = np.random.default_rng(404)
+
+
+def rot(axis, deg):
+    c, s = math.cos(math.radians(deg)), math.sin(math.radians(deg))
+    m = np.eye(3)
+    i, j = [(1, 2), (0, 2), (0, 1)][axis]
+    m[i, i], m[i, j], m[j, i], m[j, j] = c, -s, s, c
+    return m
+
+
+def affine(A, t):
+    return np.concatenate([np.asarray(A, np.float64), np.asarray(t, np.float64)[:, None]], axis=1)
+
+
+IDENT = affine(np.eye(3), [0, 0, 0])
+
+
+# ---------------------------------------------------------------- pins (CPU)
+def test_identity_transform_keeps_the_strict_interior():
+    img = RNG.uniform(2, 9, (6, 7, 8)).astype(np.float32)
+    out = P.transform_view(img, IDENT, (0, 0, 0), img.shape, min_value=1.0, outside=0.0)
+    np.testing.assert_array_equal(out[1:-1, 1:-1, 1:-1], img[1:-1, 1:-1, 1:-1])
+    # voxels on the input's faces are not strictly inside: outside value
+    assert np.all(out[0] == 0) and np.all(out[:, :, -1] == 0)
+
+
+def test_integer_translation_shifts_the_image():
+    img = RNG.uniform(2, 9, (9, 10, 11)).astype(np.float32)
+    a = affine(np.eye(3), [2, -1, 3])  # input pixel (x, y, z) -> world (x + 2, y - 1, z + 3)
+    out = P.transform_view(img, a, (0, 0, 0), (9, 10, 11), min_value=0.0, has_min=False, outside=-1.0)
+    for z, y, x in [(4, 4, 5), (5, 2, 3), (7, 7, 9)]:
+        assert out[z, y, x] == img[z - 3, y + 1, x - 2]
+    near = P.transform_view(img, a, (0, 0, 0), (9, 10, 11), has_min=False, outside=-1.0, interpolation=0)
+    np.testing.assert_array_equal(near[out != -1.0], out[out != -1.0])
+
+
+def test_trilinear_reproduces_affine_functions():
+    nz, ny, nx = 12, 13, 14
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    f = lambda X, Y, Z: 3.0 + 0.5 * X - 0.25 * Y + 0.125 * Z  # exactly representable on the grid
+    img = f(x, y, z).astype(np.float32)
+    a = affine(rot(2, 17) @ rot(0, 9) @ np.diag([1.1, 0.9, 1.0]), [1.5, -0.7, 0.3])
+    out = P.transform_view(img, a, (0, 0, 0), (nz, ny, nx), has_min=False, outside=np.nan)
+    tx, ty, tz = P._input_coords(a, (0, 0, 0), (nz, ny, nx))
+    ok = ~np.isnan(out)
+    assert ok.sum() > 500
+    np.testing.assert_allclose(out[ok], f(tx, ty, tz)[ok], rtol=1e-6)
+
+
+def test_nearest_rounds_half_up():
+    img = np.arange(4 * 4 * 4, dtype=np.float32).reshape(4, 4, 4)
+    a = affine(np.eye(3), [-0.5, 0, 0])  # world x = input x - 0.5: world 1 -> input 1.5 -> pixel 2
+    out = P.transform_view(img, a, (1, 1, 1), (1, 1, 1), has_min=False, interpolation=0)
+    assert out[0, 0, 0] == img[1, 1, 2]
+
+
+def test_blending_identity_is_product_of_axis_weights():
+    from mvr import synthetic as syn
+    shape = (30, 45, 50)
+    w = P.transform_blending(shape, (0, 0, 0), (20, 10, 10), IDENT, (0, 0, 0), shape)
+    wz = syn.blend_weight_1d(shape[0], 10)
+    wy = syn.blend_weight_1d(shape[1], 10)
+    wx = syn.blend_weight_1d(shape[2], 20)
+    ref = wz[:, None, None] * wy[None, :, None] * wx[None, None, :]
+    np.testing.assert_allclose(w, ref, rtol=1e-6, atol=0)
+    assert w[15, 22, 25] == 1.0 and np.all(w[0] == 0)
+
+
+def test_normalisation_closed_forms():
+    w = np.array([[0.3, 0.8, 1.5, 0.6, 0.55], [0.4, 0.9, 0.2, 0.6, 0.55]], np.float32)[:, None, None, :]
+    hard = P.normalize_weights(w)
+    np.testing.assert_allclose(hard[:, 0, 0, 0], [0.3, 0.4], rtol=1e-7)          # sum <= 1: unchanged
+    np.testing.assert_allclose(hard[:, 0, 0, 1], [0.8 / 1.7, 0.9 / 1.7], rtol=1e-6)
+    np.testing.assert_allclose(hard[:, 0, 0, 2], [1 / 1.2, 0.2 / 1.2], rtol=1e-6)  # 1.5 clipped to 1
+    osem = P.normalize_weights(w, osem_speedup=2.0)
+    np.testing.assert_allclose(osem[:, 0, 0, 0], [0.6, 0.8], rtol=1e-6)
+    assert np.all(osem <= 1.0)
+    sm = P.normalize_weights(w, smooth=True)
+    assert sm[0, 0, 0, 3] == pytest.approx(0.5, abs=1e-7)        # diff 0.1 = maxDiffRange: no scaling
+    assert sm[0, 0, 0, 4] == pytest.approx(0.5 - 0.5 * 0.55 * 0.05, abs=1e-6)
+
+
+# ---------------------------------------------------------------- HIP kernels (GPU)
+@pytest.mark.gpu
+@pytest.mark.parametrize("interp", [1, 0])
+def test_transform_view_matches_oracle(interp):
+    from mvr import hip
+    img = RNG.uniform(0, 50, (40, 36, 44)).astype(np.float32)
+    a = affine(rot(1, 33) @ rot(2, -12) @ np.diag([1.0, 1.0, 2.2]), [10.3, -4.1, 2.7])
+    bb_min, bb_shape = (-5, -12, 0), (70, 48, 60)
+    g = hip.transform_view(img, a, bb_min, bb_shape, min_value=1.0, outside=0.0, interpolation=interp)
+    o = P.transform_view(img, a, bb_min, bb_shape, min_value=1.0, outside=0.0, interpolation=interp)
+    tx, ty, tz = P._input_coords(a, bb_min, bb_shape)
+    # voxels whose input position lies within 1e-9 of a face could go either way in FP64 vs the kernel
+    edge = np.zeros(bb_shape, bool)
+    for t, n in ((tx, 44), (ty, 36), (tz, 40)):
+        edge |= (np.abs(t) < 1e-9) | (np.abs(t - (n - 1)) < 1e-9)
+    inside = (o != 0) & ~edge
+    assert np.array_equal((g != 0) & ~edge, inside)
+    assert inside.sum() > 10000
+    e = float(np.linalg.norm(g[inside] - o[inside]) / np.linalg.norm(o[inside]))
+    print(f"transform_view interp={interp}: rel-L2 {e:.2e}, {inside.sum()} voxels inside")
+    assert e <= 1e-6
+
+
+@pytest.mark.gpu
+def test_transform_blending_matches_oracle():
+    from mvr import hip
+    a = affine(rot(0, 21) @ np.diag([1.0, 1.0, 1.8]), [3.3, -2.0, 5.5])
+    img_shape, bb_min, bb_shape = (30, 50, 60), (-4, -20, 0), (64, 60, 72)
+    g = hip.transform_blending(img_shape, (2.0, 1.0, 0.0), (15.0, 12.0, 8.0), a, bb_min, bb_shape)
+    o = P.transform_blending(img_shape, (2.0, 1.0, 0.0), (15.0, 12.0, 8.0), a, bb_min, bb_shape)
+    assert (o > 0).sum() > 10000
+    assert np.abs(g - o).max() <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("osem,smooth", [(1.0, False), (2.5, False), (1.0, True)])
+def test_normalize_weights_matches_oracle(osem, smooth):
+    from mvr import hip
+    w = RNG.uniform(0, 1.3, (4, 20, 22, 24)).astype(np.float32)
+    w[:, :3] = 0
+    g = hip.normalize_weights(w, osem_speedup=osem, smooth=smooth)
+    o = P.normalize_weights(w, osem_speedup=osem, smooth=smooth)
+    assert np.abs(g - o).max() <= 1e-6
