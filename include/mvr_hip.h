@@ -153,6 +153,13 @@ int mvr_transform_blending(int dev, const int imgXYZ[3], const float border[3], 
 int mvr_normalize_weights(int dev, float* const* weights, int numViews, const int nXYZ[3], double osemSpeedup,
                           int additionalSmoothBlending, float maxDiffRange, float scalingRange);
 
+/* ---- PSF preparation (deconvolution/util/PSFPreparation.java:41-89): PSFExtraction.getTransformedNormalizedPSF
+ * (psf/PSFExtraction.java:182-193, 367-470): min-max normalisation, then resampling by the view's affine into an
+ * odd-sized grid whose centre is the transformed PSF centre (linear interpolation, zero outside).
+ * mvr_transform_psf_size gives the output dims; out holds that many floats. ---- */
+int mvr_transform_psf_size(const int psfXYZ[3], const double affine[12], int outXYZ[3]);
+int mvr_transform_psf(int dev, const float* psf, const int psfXYZ[3], const double affine[12], float* out);
+
 /* ---- batching and scheduling of the resident run (no counterpart in the reference, whose iteration
  * loop is MultiViewDeconvolution.runIterations, deconvolution/MultiViewDeconvolution.java:144-200) ----
  * Autotuning: per axis the candidatesPerAxis smallest compiled FFT lengths >= extent + K - 1 are timed on

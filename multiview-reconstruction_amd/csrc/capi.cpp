@@ -1,4 +1,5 @@
 // capi.cpp -- extern "C" entry points declared in include/mvr_hip.h.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -413,6 +414,69 @@ int mvr_transform_blending(int dev, const int imgXYZ[3], const float border[3], 
   MVR_HIP(hipMalloc(&dout.p, no * 4));
   if (launch_transform_blending(imgXYZ, border, blending, inv, bbMin, bbXYZ, (float*)dout.p, nullptr)) {
     set_error("blending kernel launch failed");
+    return -1;
+  }
+  MVR_HIP(hipMemcpy(out, dout.p, no * 4, hipMemcpyDefault));
+  return 0;
+}
+
+// PSFExtraction.transformPSF geometry (psf/PSFExtraction.java:367-409): the transformed bounds of [0, dim - 1],
+// size rounded to an odd count, offset so that the transformed centre dim / 2 is the new centre
+static void psf_geometry(const int psfXYZ[3], const double* f, int outXYZ[3], double off[3]) {
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int c = 0; c < 8; ++c) {
+    const double p[3] = {(c & 1) ? psfXYZ[0] - 1.0 : 0.0, (c & 2) ? psfXYZ[1] - 1.0 : 0.0, (c & 4) ? psfXYZ[2] - 1.0 : 0.0};
+    for (int d = 0; d < 3; ++d) {
+      const double t = f[4 * d] * p[0] + f[4 * d + 1] * p[1] + f[4 * d + 2] * p[2] + f[4 * d + 3];
+      lo[d] = std::min(lo[d], t);
+      hi[d] = std::max(hi[d], t);
+    }
+  }
+  const double ctr[3] = {(double)(psfXYZ[0] / 2), (double)(psfXYZ[1] / 2), (double)(psfXYZ[2] / 2)};
+  for (int d = 0; d < 3; ++d) {
+    const double tc = f[4 * d] * ctr[0] + f[4 * d + 1] * ctr[1] + f[4 * d + 2] * ctr[2] + f[4 * d + 3];
+    long long n = (long long)(int)(hi[d] - lo[d]) + 1;
+    if (n % 2 == 0) ++n;
+    outXYZ[d] = (int)n;
+    off[d] = tc - (double)(n / 2);
+  }
+}
+
+int mvr_transform_psf_size(const int psfXYZ[3], const double affine[12], int outXYZ[3]) {
+  NULLCHECK(psfXYZ);
+  NULLCHECK(affine);
+  NULLCHECK(outXYZ);
+  double off[3];
+  psf_geometry(psfXYZ, affine, outXYZ, off);
+  return 0;
+}
+
+int mvr_transform_psf(int dev, const float* psf, const int psfXYZ[3], const double affine[12], float* out) {
+  NULLCHECK(psf);
+  NULLCHECK(psfXYZ);
+  NULLCHECK(affine);
+  NULLCHECK(out);
+  for (int d = 0; d < 3; ++d)
+    if (psfXYZ[d] < 1) { set_error("empty PSF"); return -1; }
+  Affine3 inv;
+  if (invert_affine(affine, inv)) return -1;
+  int outXYZ[3];
+  double off[3];
+  psf_geometry(psfXYZ, affine, outXYZ, off);
+  const size_t ni = (size_t)psfXYZ[0] * psfXYZ[1] * psfXYZ[2], no = (size_t)outXYZ[0] * outXYZ[1] * outXYZ[2];
+  // PSFExtraction.normalize (psf/PSFExtraction.java:453-470): (v - min) / (max - min), in double
+  std::vector<float> h(ni);
+  MVR_HIP(hipSetDevice(dev));
+  MVR_HIP(hipMemcpy(h.data(), psf, ni * 4, hipMemcpyDefault));
+  double mn = 1e308, mx = -1e308;
+  for (float v : h) { mn = std::min(mn, (double)v); mx = std::max(mx, (double)v); }
+  for (float& v : h) v = (float)(((double)v - mn) / (mx - mn));
+  DevBuf di, dout;
+  MVR_HIP(hipMalloc(&di.p, ni * 4));
+  MVR_HIP(hipMalloc(&dout.p, no * 4));
+  MVR_HIP(hipMemcpy(di.p, h.data(), ni * 4, hipMemcpyHostToDevice));
+  if (launch_transform_psf((const float*)di.p, psfXYZ, inv, off, outXYZ, (float*)dout.p, nullptr)) {
+    set_error("PSF transform launch failed");
     return -1;
   }
   MVR_HIP(hipMemcpy(out, dout.p, no * 4, hipMemcpyDefault));

@@ -7,7 +7,8 @@
 //     (world -> input pixel coordinates by the inverse affine; strictly-inside test; minValue clamp;
 //     outside value), AbstractTransformedIntervalRandomAccess.intersectsLinearInterpolation,
 //   fusion/transformed/TransformWeight.transformBlending + weights/BlendingRealRandomAccess.java:46-130,
-//   deconvolution/normalization/NormalizingRandomAccess.java:76-108, 183-214 (hard / smooth weights, OSEM).
+//   deconvolution/normalization/NormalizingRandomAccess.java:76-108, 183-214 (hard / smooth weights, OSEM),
+//   psf/PSFExtraction.java:367-451 (transformPSF / transform: odd-sized, centre-preserving PSF resampling).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -91,6 +92,28 @@ __global__ void k_transform_blending(int3 dimMinus1, float3 border, float3 blend
   out[((size_t)z * outDim.y + y) * outDim.x + x] = w;
 }
 
+// PSFExtraction.transform (psf/PSFExtraction.java:411-451): out[p] = linear interpolation of the zero-extended
+// PSF at inverse(p + offset)
+__global__ void k_transform_psf(const float* __restrict__ psf, int3 in, Affine3 inv, double3 off, int3 outDim,
+                                float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+  if (x >= outDim.x) return;
+  double tx, ty, tz;
+  apply_inv(inv, x + off.x, y + off.y, z + off.z, tx, ty, tz);
+  const int x0 = (int)floor(tx), y0 = (int)floor(ty), z0 = (int)floor(tz);
+  const double fx = tx - x0, fy = ty - y0, fz = tz - z0;
+  double val = 0.0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int xi = x0 + (c & 1), yi = y0 + ((c >> 1) & 1), zi = z0 + (c >> 2);
+    if ((unsigned)xi < (unsigned)in.x && (unsigned)yi < (unsigned)in.y && (unsigned)zi < (unsigned)in.z) {
+      const double wgt = ((c & 1) ? fx : 1 - fx) * (((c >> 1) & 1) ? fy : 1 - fy) * ((c >> 2) ? fz : 1 - fz);
+      val += wgt * (double)psf[((size_t)zi * in.y + yi) * in.x + xi];
+    }
+  }
+  out[((size_t)z * outDim.y + y) * outDim.x + x] = (float)val;
+}
+
 struct WeightPtrs { float* p[kPrepMaxViews]; };
 
 // NormalizingRandomAccess.get for every view at once (in place)
@@ -131,6 +154,14 @@ int launch_transform_view(const float* img, const int inXYZ[3], const Affine3& i
   hipLaunchKernelGGL(k_transform_view, grid, dim3(256), 0, s, img, make_int3(inXYZ[0], inXYZ[1], inXYZ[2]), inv,
                      make_long3(off[0], off[1], off[2]), make_int3(outXYZ[0], outXYZ[1], outXYZ[2]), hasMin, minValue,
                      outside, linear, out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_transform_psf(const float* psf, const int inXYZ[3], const Affine3& inv, const double off[3],
+                         const int outXYZ[3], float* out, hipStream_t s) {
+  const dim3 grid((outXYZ[0] + 255) / 256, outXYZ[1], outXYZ[2]);
+  hipLaunchKernelGGL(k_transform_psf, grid, dim3(256), 0, s, psf, make_int3(inXYZ[0], inXYZ[1], inXYZ[2]), inv,
+                     make_double3(off[0], off[1], off[2]), make_int3(outXYZ[0], outXYZ[1], outXYZ[2]), out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

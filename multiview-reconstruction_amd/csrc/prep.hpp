@@ -17,6 +17,8 @@ int launch_transform_view(const float* img, const int inXYZ[3], const Affine3& i
                           const int outXYZ[3], int hasMin, float minValue, float outside, int linear, float* out,
                           hipStream_t s);
 int upload_blend_lut(const double* lut);
+int launch_transform_psf(const float* psf, const int inXYZ[3], const Affine3& inv, const double off[3],
+                         const int outXYZ[3], float* out, hipStream_t s);
 int launch_transform_blending(const int inXYZ[3], const float border[3], const float blending[3], const Affine3& inv,
                               const long long off[3], const int outXYZ[3], float* out, hipStream_t s);
 int launch_normalize_weights(float* const* w, int V, size_t n, double osem, int smooth, float maxDiffRange,

@@ -90,6 +90,8 @@ def lib():
                                              ctypes.POINTER(ctypes.c_longlong), _i32p, vp]
         L.mvr_normalize_weights.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int, _i32p, ctypes.c_double,
                                             ctypes.c_int, ctypes.c_float, ctypes.c_float]
+        L.mvr_transform_psf_size.argtypes = [_i32p, _f64p, _i32p]
+        L.mvr_transform_psf.argtypes = [ctypes.c_int, vp, _i32p, _f64p, vp]
         L.mvr_unet_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong]
         L.mvr_unet_param_count.argtypes = [vp]
         L.mvr_unet_param_count.restype = ctypes.c_longlong
@@ -453,3 +455,14 @@ def normalize_weights(weights, osem_speedup=1.0, smooth=False, max_diff_range=0.
     _check(lib().mvr_normalize_weights(dev, ptrs, w.shape[0], _dims(w.shape[1:]), osem_speedup, 1 if smooth else 0,
                                        max_diff_range, scaling_range))
     return w
+
+
+def transform_psf(psf, affine, dev=0):
+    """PSFExtraction.getTransformedNormalizedPSF: the normalised PSF resampled by the view's affine."""
+    psf = _f32(psf)
+    ap, keep = _affine(affine)
+    o = (ctypes.c_int * 3)()
+    _check(lib().mvr_transform_psf_size(_dims(psf.shape), ap, o))
+    out = np.empty((o[2], o[1], o[0]), np.float32)
+    _check(lib().mvr_transform_psf(dev, _ptr(psf), _dims(psf.shape), ap, _ptr(out)))
+    return out

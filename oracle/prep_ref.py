@@ -8,10 +8,12 @@ Reference (src/main/java/net/preibisch/mvrecon/process/):
                       (ImgLib2 NLinearInterpolator / NearestNeighborInterpolator, third-party: imglib2 8.0.0);
   transform_blending  fusion/transformed/TransformWeight.java:82-134, weights/BlendingRealRandomAccess.java:46-127
                       (float arithmetic, the 1001-entry cosine table);
-  normalize_weights   deconvolution/normalization/NormalizingRandomAccess.java:76-108, 183-214.
+  normalize_weights   deconvolution/normalization/NormalizingRandomAccess.java:76-108, 183-214;
+  transform_psf       psf/PSFExtraction.java:182-193, 367-470.
 Pins (tests/test_prep.py): identity and integer-translation transforms reproduce the input; trilinear
 interpolation reproduces affine functions exactly; the blending weight with the identity transform is the
-product of the per-axis weights; the normalisation's closed forms and special cases.
+product of the per-axis weights; the normalisation's closed forms and special cases; the PSF transform
+with the identity keeps the normalised PSF, a 90 degree rotation permutes its axes exactly.
 """
 from __future__ import annotations
 
@@ -115,3 +117,36 @@ def normalize_weights(weights, osem_speedup=1.0, smooth=False, max_diff_range=0.
             val = np.where(sumw > 1, hard, my.astype(np.float64))
         out[v] = np.minimum(1.0, val * osem_speedup).astype(np.float32)
     return out
+
+
+def transform_psf(psf, affine):
+    """PSFExtraction.getTransformedNormalizedPSF (psf/PSFExtraction.java:182-193, 367-470): min-max normalise,
+    bounds of the transformed interval [0, dim - 1] -> odd size, centre dim / 2 kept at the new centre,
+    linear interpolation of the zero-extended PSF.  FP64."""
+    psf = np.asarray(psf, np.float32).astype(np.float64)
+    psf = ((psf - psf.min()) / (psf.max() - psf.min())).astype(np.float32).astype(np.float64)
+    a = np.asarray(affine, np.float64).reshape(3, 4)
+    nz, ny, nx = psf.shape
+    corners = np.array([[x, y, z] for z in (0, nz - 1) for y in (0, ny - 1) for x in (0, nx - 1)], np.float64)
+    t = corners @ a[:, :3].T + a[:, 3]
+    size = t.max(0) - t.min(0)
+    n = [int(s) + 1 for s in size]
+    n = [k + 1 if k % 2 == 0 else k for k in n]
+    ctr = a[:, :3] @ np.array([nx // 2, ny // 2, nz // 2], np.float64) + a[:, 3]
+    off = [ctr[d] - (n[d] // 2) for d in range(3)]
+    inv = inverse_affine(affine)
+    z, y, x = np.meshgrid(np.arange(n[2]), np.arange(n[1]), np.arange(n[0]), indexing="ij")
+    s = np.stack([x + off[0], y + off[1], z + off[2]], -1).astype(np.float64)
+    q = s @ inv[:, :3].T + inv[:, 3]
+    X, Y, Z = q[..., 0], q[..., 1], q[..., 2]
+    x0, y0, z0 = np.floor(X).astype(int), np.floor(Y).astype(int), np.floor(Z).astype(int)
+    fx, fy, fz = X - x0, Y - y0, Z - z0
+    out = np.zeros(X.shape)
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                xi, yi, zi = x0 + dx, y0 + dy, z0 + dz
+                ok = (xi >= 0) & (xi < nx) & (yi >= 0) & (yi < ny) & (zi >= 0) & (zi < nz)
+                w = (fx if dx else 1 - fx) * (fy if dy else 1 - fy) * (fz if dz else 1 - fz)
+                out[ok] += w[ok] * psf[zi[ok], yi[ok], xi[ok]]
+    return out.astype(np.float32)

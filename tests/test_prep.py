@@ -138,3 +138,66 @@ def test_normalize_weights_matches_oracle(osem, smooth):
     g = hip.normalize_weights(w, osem_speedup=osem, smooth=smooth)
     o = P.normalize_weights(w, osem_speedup=osem, smooth=smooth)
     assert np.abs(g - o).max() <= 1e-6
+
+
+# ---------------------------------------------------------------- PSF preparation
+def _gauss_psf(shape, sig):
+    z, y, x = np.meshgrid(*[np.arange(n) - n // 2 for n in shape], indexing="ij")
+    return np.exp(-0.5 * ((x / sig[0]) ** 2 + (y / sig[1]) ** 2 + (z / sig[2]) ** 2)).astype(np.float32)
+
+
+def test_psf_transform_identity_and_quarter_turn():
+    psf = _gauss_psf((9, 11, 13), (1.0, 2.0, 1.5)) + RNG.uniform(0, 0.01, (9, 11, 13)).astype(np.float32)
+    p64 = psf.astype(np.float64)
+    norm = ((p64 - p64.min()) / (p64.max() - p64.min())).astype(np.float32)
+    out = P.transform_psf(psf, IDENT)
+    assert out.shape == psf.shape
+    np.testing.assert_array_equal(out, norm)
+    # x' = -y, y' = x: +x turns onto +y, a quarter turn in the xy plane (rot90 from axis x towards axis y)
+    q = affine([[0, -1, 0], [1, 0, 0], [0, 0, 1]], [5, 2, 0])
+    out = P.transform_psf(psf, q)
+    assert out.shape == (9, 13, 11)
+    np.testing.assert_array_equal(out, np.rot90(norm, k=1, axes=(2, 1)))
+
+
+def test_psf_average_and_same_size():
+    from mvr import prep
+    a = np.zeros((3, 5, 5), np.float32)
+    a[1, 2, 2] = 4
+    b = np.zeros((3, 3, 7), np.float32)
+    b[1, 1, 3] = 2
+    avg = prep.average_psf([a, b])
+    assert avg.shape == (3, 3, 5) and avg[1, 1, 2] == 3.0 and avg.sum() == 3.0
+    big = prep.make_same_size(np.arange(27, dtype=np.float32).reshape(3, 3, 3) + 1, (5, 5, 5))
+    assert big[2, 2, 2] == 14 and big[0, 0, 0] == 1 and big[1, 1, 1] == 1
+    small = prep.make_same_size(big, (3, 3, 3))
+    np.testing.assert_array_equal(small, np.arange(27, dtype=np.float32).reshape(3, 3, 3) + 1)
+
+
+@pytest.mark.gpu
+def test_transform_psf_matches_oracle():
+    from mvr import hip
+    psf = _gauss_psf((21, 19, 17), (1.5, 1.2, 3.0))
+    a = affine(rot(1, 40) @ rot(2, 15) @ np.diag([1.0, 1.0, 1.7]), [3.0, -1.0, 2.0])
+    g = hip.transform_psf(psf, a)
+    o = P.transform_psf(psf, a)
+    assert g.shape == o.shape and all(s % 2 == 1 for s in g.shape)
+    e = float(np.linalg.norm(g - o) / np.linalg.norm(o))
+    print(f"transform_psf {psf.shape} -> {g.shape}: rel-L2 {e:.2e}")
+    assert e <= 1e-6
+
+
+@pytest.mark.gpu
+def test_prepare_views_pipeline():
+    from mvr import prep
+    phantom = RNG.uniform(5, 50, (40, 44, 48)).astype(np.float32)
+    affines = [affine(rot(1, ang) @ np.diag([1.0, 1.0, 1.4]), [4.0, -3.0, 6.0]) for ang in (0, 35, 70)]
+    box_min, box_shape = (-6, -8, 0), (64, 52, 60)
+    imgs, ws = prep.prepare_views([phantom] * 3, affines, box_min, box_shape, blending_range_xyz=(10, 10, 6),
+                                  blending_border_xyz=(1, 1, 0), osem_speedup=1.0)
+    oi = np.stack([P.transform_view(phantom, a, box_min, box_shape, 1.0, True, 0.0, 1) for a in affines])
+    ow = P.normalize_weights(np.stack([P.transform_blending(phantom.shape, (1, 1, 0), (10, 10, 6), a, box_min,
+                                                            box_shape) for a in affines]))
+    assert float(np.linalg.norm(imgs - oi) / np.linalg.norm(oi)) <= 1e-6
+    assert np.abs(ws - ow).max() <= 1e-6
+    assert ws.sum(0).max() <= 1.0 + 1e-6
