@@ -146,6 +146,29 @@ def test_resident_decon_matches_oracle(oracle, psf_type, scheme, lam):
     d.close()
 
 
+@pytest.mark.parametrize("shape,kshape,psf_type,scheme", [
+    ((20, 22, 18), (6, 8, 4), "EFFICIENT_BAYESIAN", "SEQ"),   # even PSF sizes: the double-swap Mirror
+    ((17, 15, 19), (5, 7, 6), "OPTIMIZATION_I", "MUL"),
+    ((1, 40, 36), (1, 7, 9), "OPTIMIZATION_II", "SEQ"),       # a single z plane (2D data)
+])
+def test_resident_decon_edge_shapes(oracle, shape, kshape, psf_type, scheme):
+    rng = np.random.default_rng(sum(shape) + sum(kshape))
+    V = 3
+    imgs = rng.uniform(1.0, 40.0, (V,) + shape).astype(np.float32)
+    ws = rng.uniform(0.0, 1.0, (V,) + shape).astype(np.float32)
+    ks = rng.uniform(0.01, 1.0, (V,) + kshape).astype(np.float32)
+    ref = oracle.deconvolve(imgs, ws, ks, psf_type=psf_type, scheme=scheme, lam=6e-4, iterations=5)
+    d = hip.Decon(shape, V, psf_type, scheme, 6e-4)
+    for v in range(V):
+        d.set_view(v, imgs[v], ws[v], ks[v])
+    d.init()
+    d.run(5)
+    e = rel(d.psi(), ref["psi"])
+    print(f"{shape} K {kshape} {psf_type} {scheme}: rel-L2 vs oracle {e:.2e}, fft grid {d.geometry()['P']}")
+    assert e < 1e-4
+    d.close()
+
+
 @pytest.mark.parametrize("scheme", ["SEQ", "MUL"])
 def test_graph_replay_matches_eager(scheme):
     # one captured iteration replayed: the same kernels in the same order, so psi and stats are identical;
