@@ -375,7 +375,7 @@ __global__ __launch_bounds__(kThreads) void k_strided(StridedGeom gm, const floa
 // one LDS round trip per transform.
 template <int L>
 __host__ __device__ constexpr int s2_width() {
-  return L <= 320 ? 16 : 8;
+  return L <= 512 ? 16 : 8;  // 16 columns = 128-byte runs (32 was slower at L = 300, 8 slower at L = 400)
 }
 template <int L>
 __host__ __device__ constexpr int s2_threads() {
