@@ -1,0 +1,173 @@
+"""Writing the deconvolved volume: ImageJ TIFF stacks and N5 datasets (host I/O).
+
+Mirrors the exporters the reference's deconvolution plug-in hands its result to
+(src/main/java/net/preibisch/mvrecon/fiji/plugin/Image_Deconvolution.java:290-299 -> ImgExport):
+  * save_tiff_stack: process/export/Save3dTIFF.java:40-110 (one uncompressed 32-bit float TIFF stack per
+    result, ImageJ description with images/slices, unit and voxel spacing; file name <prefix><title>.tif);
+  * save_n5: process/export/ExportN5Api.java (an N5 dataset: attributes.json with dimensions, blockSize,
+    dataType and compression; one file per block at <dataset>/<i>/<j>/<k> holding a big-endian header
+    {mode, number of dimensions, block size} and the big-endian block data, x fastest; raw or gzip).
+Volumes are numpy arrays indexed [z, y, x].  Readers for both formats are included for round trips.
+"""
+from __future__ import annotations
+
+import gzip
+import json
+import os
+import struct
+
+import numpy as np
+
+# ---------------------------------------------------------------- TIFF (ImageJ stack)
+_TAGS = {"width": 256, "length": 257, "bits": 258, "compression": 259, "photometric": 262, "description": 270,
+         "strip_offsets": 273, "samples": 277, "rows_per_strip": 278, "strip_bytes": 279, "x_res": 282,
+         "y_res": 283, "res_unit": 296, "sample_format": 339}
+
+
+def imagej_description(nz, spacing_zyx=(1.0, 1.0, 1.0), unit="pixel", vmin=None, vmax=None):
+    lines = ["ImageJ=1.54f", f"images={nz}", f"slices={nz}", f"unit={unit}", f"spacing={spacing_zyx[0]}",
+             "loop=false"]
+    if vmin is not None and vmax is not None:
+        lines += [f"min={vmin}", f"max={vmax}"]
+    return "\n".join(lines) + "\n\x00"
+
+
+def save_tiff_stack(vol, path, spacing_zyx=(1.0, 1.0, 1.0), unit="pixel"):
+    """Uncompressed big-endian 32-bit float TIFF stack, one IFD per z plane, ImageJ description in the first."""
+    vol = np.ascontiguousarray(vol, np.float32)
+    if vol.ndim == 2:
+        vol = vol[None]
+    nz, ny, nx = vol.shape
+    desc = imagej_description(nz, spacing_zyx, unit, float(vol.min()), float(vol.max())).encode("latin-1")
+    plane = nx * ny * 4
+    data_off = 8
+    desc_off = data_off + nz * plane
+    res_off = desc_off + len(desc)
+    ifd_off = res_off + 16
+    ntags = 13
+    ifd_size = 2 + ntags * 12 + 4
+    with open(path, "wb") as f:
+        f.write(b"MM" + struct.pack(">HI", 42, ifd_off))
+        f.write(vol.astype(">f4").tobytes())
+        f.write(desc)
+        xr = 1.0 / spacing_zyx[2] if spacing_zyx[2] else 1.0
+        yr = 1.0 / spacing_zyx[1] if spacing_zyx[1] else 1.0
+        f.write(struct.pack(">IIII", int(round(xr * 1e6)), 1000000, int(round(yr * 1e6)), 1000000))
+        for z in range(nz):
+            entries = [
+                (_TAGS["width"], 4, 1, nx), (_TAGS["length"], 4, 1, ny), (_TAGS["bits"], 3, 1, 32),
+                (_TAGS["compression"], 3, 1, 1), (_TAGS["photometric"], 3, 1, 1),
+                (_TAGS["description"], 2, len(desc), desc_off) if z == 0 else (_TAGS["description"], 2, 1, 0),
+                (_TAGS["strip_offsets"], 4, 1, data_off + z * plane), (_TAGS["samples"], 3, 1, 1),
+                (_TAGS["rows_per_strip"], 4, 1, ny), (_TAGS["strip_bytes"], 4, 1, plane),
+                (_TAGS["x_res"], 5, 1, res_off), (_TAGS["y_res"], 5, 1, res_off + 8),
+                (_TAGS["sample_format"], 3, 1, 3)]
+            nxt = ifd_off + (z + 1) * ifd_size if z + 1 < nz else 0
+            f.write(struct.pack(">H", ntags))
+            for tag, typ, cnt, val in entries:
+                if typ == 3:  # SHORT, left-justified in the value field
+                    f.write(struct.pack(">HHIHH", tag, typ, cnt, val, 0))
+                else:
+                    f.write(struct.pack(">HHII", tag, typ, cnt, val))
+            f.write(struct.pack(">I", nxt))
+    return path
+
+
+def read_tiff_stack(path):
+    """Reads uncompressed 32-bit float TIFF stacks (either byte order, one strip or several per plane)."""
+    with open(path, "rb") as f:
+        buf = f.read()
+    bo = ">" if buf[:2] == b"MM" else "<"
+    (magic,) = struct.unpack(bo + "H", buf[2:4])
+    assert magic == 42, "not a TIFF file"
+    (off,) = struct.unpack(bo + "I", buf[4:8])
+    planes = []
+    description = None
+    while off:
+        (n,) = struct.unpack(bo + "H", buf[off:off + 2])
+        tags = {}
+        for k in range(n):
+            e = buf[off + 2 + 12 * k: off + 14 + 12 * k]
+            tag, typ, cnt = struct.unpack(bo + "HHI", e[:8])
+            if typ == 3 and cnt == 1:
+                (val,) = struct.unpack(bo + "H", e[8:10])
+            elif typ == 3 and cnt == 2:
+                val = list(struct.unpack(bo + "HH", e[8:12]))
+            else:
+                (val,) = struct.unpack(bo + "I", e[8:12])
+            tags[tag] = (typ, cnt, val)
+        w, h = tags[256][2], tags[257][2]
+        assert tags[258][2] == 32 and tags.get(339, (0, 0, 1))[2] == 3 and tags.get(259, (0, 0, 1))[2] == 1
+        if 270 in tags and description is None and tags[270][1] > 4:
+            t = tags[270]
+            description = buf[t[2]:t[2] + t[1]].rstrip(b"\x00").decode("latin-1")
+        so, sb = tags[273], tags[279]
+        if so[1] == 1:
+            offs, cnts = [so[2]], [sb[2]]
+        else:
+            offs = list(struct.unpack(bo + "I" * so[1], buf[so[2]:so[2] + 4 * so[1]]))
+            cnts = list(struct.unpack(bo + "I" * sb[1], buf[sb[2]:sb[2] + 4 * sb[1]]))
+        raw = b"".join(buf[o:o + c] for o, c in zip(offs, cnts))
+        planes.append(np.frombuffer(raw, dtype=bo + "f4").reshape(h, w))
+        (off,) = struct.unpack(bo + "I", buf[off + 2 + 12 * n: off + 6 + 12 * n])
+    return np.stack(planes).astype(np.float32), description
+
+
+# ---------------------------------------------------------------- N5
+def save_n5(vol, root, dataset, block_size_xyz=(64, 64, 64), compression="raw", attributes=None):
+    """Writes vol [z, y, x] as float32 N5 dataset `dataset` under `root`."""
+    vol = np.ascontiguousarray(vol, np.float32)
+    nz, ny, nx = vol.shape
+    os.makedirs(os.path.join(root, dataset), exist_ok=True)
+    rattr = os.path.join(root, "attributes.json")
+    if not os.path.exists(rattr):
+        with open(rattr, "w") as f:
+            json.dump({"n5": "4.0.0"}, f)
+    comp = {"type": "raw"} if compression == "raw" else {"type": "gzip", "level": -1}
+    attrs = {"dimensions": [nx, ny, nz], "blockSize": list(block_size_xyz), "dataType": "float32",
+             "compression": comp}
+    if attributes:
+        attrs.update(attributes)
+    with open(os.path.join(root, dataset, "attributes.json"), "w") as f:
+        json.dump(attrs, f)
+    bx, by, bz = block_size_xyz
+    for k in range(0, nz, bz):
+        for j in range(0, ny, by):
+            for i in range(0, nx, bx):
+                blk = vol[k:k + bz, j:j + by, i:i + bx]
+                dz, dy, dx = blk.shape
+                header = struct.pack(">HHIII", 0, 3, dx, dy, dz)
+                data = np.ascontiguousarray(blk).astype(">f4").tobytes()
+                if compression != "raw":
+                    data = gzip.compress(data)
+                d = os.path.join(root, dataset, str(i // bx), str(j // by))
+                os.makedirs(d, exist_ok=True)
+                with open(os.path.join(d, str(k // bz)), "wb") as f:
+                    f.write(header + data)
+    return os.path.join(root, dataset)
+
+
+def read_n5(root, dataset):
+    with open(os.path.join(root, dataset, "attributes.json")) as f:
+        attrs = json.load(f)
+    nx, ny, nz = attrs["dimensions"]
+    bx, by, bz = attrs["blockSize"]
+    assert attrs["dataType"] == "float32"
+    gz = attrs["compression"]["type"] == "gzip"
+    out = np.zeros((nz, ny, nx), np.float32)
+    for k in range(0, nz, bz):
+        for j in range(0, ny, by):
+            for i in range(0, nx, bx):
+                p = os.path.join(root, dataset, str(i // bx), str(j // by), str(k // bz))
+                if not os.path.exists(p):
+                    continue
+                with open(p, "rb") as f:
+                    raw = f.read()
+                mode, nd = struct.unpack(">HH", raw[:4])
+                dims = struct.unpack(">" + "I" * nd, raw[4:4 + 4 * nd])
+                body = raw[4 + 4 * nd:]
+                if gz:
+                    body = gzip.decompress(body)
+                blk = np.frombuffer(body, ">f4").reshape(dims[2], dims[1], dims[0])
+                out[k:k + dims[2], j:j + dims[1], i:i + dims[0]] = blk
+    return out, attrs
