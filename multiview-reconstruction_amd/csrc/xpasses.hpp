@@ -53,7 +53,7 @@ struct XCfg {
   static constexpr int T = A;                      // threads per row
   static constexpr int RW = 256 / A;               // rows per work-group
   static constexpr int NT = RW * T;                // threads per work-group
-  static constexpr int RS = M + 1;                 // LDS row stride (complex)
+  static constexpr int RS = M + 5;                 // LDS row stride (complex): M + 1 used, +4 against bank conflicts
   static constexpr int PX = 2 * M;
 };
 
