@@ -25,16 +25,17 @@ struct FastDiv {
 // {sum, max} of a work-group of NT threads (NT <= 256, any multiple of 1)
 template <int NT>
 __device__ __forceinline__ void block_reduce_stats_nt(double s, double mx, double2* out) {
-  __shared__ double rs[256], rm[256];
+  constexpr int P = NT <= 256 ? 256 : NT <= 512 ? 512 : 1024;  // power of two >= NT
+  __shared__ double rs[P], rm[P];
   rs[threadIdx.x] = s;
   rm[threadIdx.x] = mx;
-  for (int i = NT + (int)threadIdx.x; i < 256; i += NT) {
+  for (int i = NT + (int)threadIdx.x; i < P; i += NT) {
     rs[i] = 0.0;
     rm[i] = -1.0;
   }
   __syncthreads();
 #pragma unroll
-  for (int off = 128; off > 0; off >>= 1) {
+  for (int off = P / 2; off > 0; off >>= 1) {
     if ((int)threadIdx.x < off) {
       rs[threadIdx.x] += rs[threadIdx.x + off];
       rm[threadIdx.x] = fmax(rm[threadIdx.x], rm[threadIdx.x + off]);
