@@ -18,6 +18,7 @@
 // lane l holds rows/columns l%16 of A/B, and D[4(l/16)+r][l%16].
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "unet.hpp"
@@ -79,31 +80,44 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a) {
   constexpr int NL = (HV * 2 + 255) / 256;         // halo loads per thread
   constexpr int NW = (28 * WC * 2 + 255) / 256;    // weight loads per thread
   uint4 hbuf[NL], wbuf[NW];
+  // per-thread halo coordinates of unit k: v = (threadIdx.x >> 1) + 128 k, decomposed incrementally
+  const int half = threadIdx.x & 1;
   auto issue = [&](int cb) {
     // the chunk comes from the upsampled source (half coordinates) or from in
     const bool up = cb * 16 < a.up_c;
-    const uint16_t* src = up ? a.up_in + cb * 16 : a.in + (cb * 16 - a.up_c);
-    const int cs = up ? a.up_cs : a.in_cs;
     const int sh = up ? 1 : 0;
     const int sD = a.D >> sh, sH = a.H >> sh, sW = a.W >> sh;
+    const int cs = up ? a.up_cs : a.in_cs;
+    // sample base in 64 bits once; offsets within a sample fit 32 bits (checked at launch)
+    const uint16_t* src = (up ? a.up_in + cb * 16 : a.in + (cb * 16 - a.up_c)) + (size_t)n * sD * sH * sW * cs +
+                          half * 8;
+    int v0 = threadIdx.x >> 1;
+    int hx = v0 % HX, hy = (v0 / HX) % HY, hz = v0 / (HX * HY);
+    constexpr int DX = 128 % HX, DY = (128 / HX) % HY, DZ = 128 / (HX * HY);
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
-      const int e = threadIdx.x + 256 * k;
-      const int v = e >> 1, half = e & 1;
-      const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
+      if (k) {  // advance by 128 halo voxels
+        hx += DX;
+        int cy = hx >= HX;
+        hx -= cy ? HX : 0;
+        hy += DY + cy;
+        int cz = hy >= HY;
+        hy -= cz ? HY : 0;
+        hz += DZ + cz;
+      }
       const int xx = x0 + hx - 1, yy = y0 + hy - 1, zz = z + hz - 1;
-      const bool ok = e < HV * 2 && (unsigned)xx < (unsigned)a.W && (unsigned)yy < (unsigned)a.H &&
-                      (unsigned)zz < (unsigned)a.D;
-      const size_t gv = ok ? (((size_t)n * sD + (zz >> sh)) * sH + (yy >> sh)) * sW + (xx >> sh) : 0;
-      hbuf[k] = *reinterpret_cast<const uint4*>(src + gv * cs + half * 8);
+      const bool ok = (threadIdx.x + 256 * k) < HV * 2 && (unsigned)xx < (unsigned)a.W &&
+                      (unsigned)yy < (unsigned)a.H && (unsigned)zz < (unsigned)a.D;
+      const int gv = ((zz >> sh) * sH + (yy >> sh)) * sW + (xx >> sh);
+      hbuf[k] = *reinterpret_cast<const uint4*>(src + (ok ? gv * cs : 0));
       if (!ok) hbuf[k] = make_uint4(0u, 0u, 0u, 0u);
     }
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
       const int e = threadIdx.x + 256 * k;
-      const int half = e & 1, c = (e >> 1) % WC, tap = (e >> 1) / WC;
+      const int c = (e >> 1) % WC, tap = (e >> 1) / WC;
       const bool ok = tap < 27;
-      wbuf[k] = *reinterpret_cast<const uint4*>(a.w + ((size_t)(ok ? tap : 0) * a.cout + co0 + c) * a.cin + cb * 16 +
+      wbuf[k] = *reinterpret_cast<const uint4*>(a.w + ((ok ? tap : 0) * a.cout + co0 + c) * a.cin + cb * 16 +
                                                half * 8);
       if (!ok) wbuf[k] = make_uint4(0u, 0u, 0u, 0u);
     }
@@ -128,7 +142,8 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a) {
     issue(cb);
     park();
     __syncthreads();
-#pragma unroll 2
+    // full unroll folds the tap arithmetic into constants; single-group layers keep registers for occupancy
+#pragma unroll(NG == 1 ? 2 : 14)
     for (int p = 0; p < 14; ++p) {
       const int tap = 2 * p + hi;                 // 27 -> zero weights; its A reads the centre tap
       const int tv = tap < 27 ? tap : 13;
@@ -148,23 +163,25 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a) {
   }
 
   // epilogue: bias, ReLU, bf16 stores (voxel 4g + r of the segment, channel co0 + t * 16 + i)
+  float bias[NG];
 #pragma unroll
-  for (int t = 0; t < NG; ++t) {
-    const int co = co0 + t * 16 + i;
-    const float bias = a.bias ? a.bias[co] : 0.f;
+  for (int t = 0; t < NG; ++t) bias[t] = a.bias ? a.bias[co0 + t * 16 + i] : 0.f;
+  uint16_t* outPlane = a.out + (((size_t)n * a.D + z) * a.H) * a.W * a.out_cs + a.out_co + co0 + i;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int s = 4 * wave + m;
-      const int y = y0 + s / SEGX;
-      if (y >= a.H) continue;
+  for (int m = 0; m < 4; ++m) {
+    const int s = 4 * wave + m;
+    const int y = y0 + s / SEGX;
+    const int xs = x0 + (s % SEGX) * 16 + 4 * g;
+    if (y >= a.H) continue;
+    uint16_t* orow = outPlane + ((size_t)y * a.W + xs) * a.out_cs;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int x = x0 + (s % SEGX) * 16 + 4 * g + r;
-        if (x >= a.W) continue;
-        float v = acc[m][t][r] + bias;
+    for (int r = 0; r < 4; ++r) {
+      if (xs + r >= a.W) continue;
+#pragma unroll
+      for (int t = 0; t < NG; ++t) {
+        float v = acc[m][t][r] + bias[t];
         if (a.relu) v = fmaxf(v, 0.f);
-        const size_t vo = (((size_t)n * a.D + z) * a.H + y) * a.W + x;
-        a.out[vo * a.out_cs + a.out_co + co] = f2bf_rne(v);
+        orow[r * a.out_cs + t * 16] = f2bf_rne(v);
       }
     }
   }
@@ -190,6 +207,8 @@ void launch_tx(const ConvArgs& a, hipStream_t s) {
 // 3^3 convolutions with cin and cout multiples of 16; the tile width follows W
 bool launch_conv_tiled(const ConvArgs& a, hipStream_t s) {
   if (a.taps != 27 || a.cin % 16 != 0 || a.cout % 16 != 0 || a.in_cs % 16 != 0) return false;
+  // offsets within one sample are 32-bit in the staging
+  if ((long long)a.D * a.H * a.W * std::max(a.in_cs, a.up_cs) >= (1ll << 31)) return false;
   if (a.up_c && (a.up_c % 16 != 0 || a.up_cs % 16 != 0 || !a.up_in || (a.W | a.H | a.D) & 1)) return false;
   if (a.W > 32)
     launch_tx<64>(a, s);
