@@ -68,6 +68,10 @@ int mvr_convolution3d_fft_inplace(float* im, const int* imDimZYX, const float* k
 /* ---- one util.FFTConvolution.convolve (util/FFTConvolution.java:490-506): out = kernel (*) ext(img) ---- */
 int mvr_fft_convolve(int dev, const float* img, const int nXYZ[3], int ext, float extValue, const float* kernel,
                      const int kXYZ[3], float* out);
+/* FFT lengths with compiled kernels, ascending, into out[0..cap); returns how many there are.  A domain of
+ * extent n + k - 1 along an axis runs at the smallest of them >= that extent (even lengths only along x).
+ * The reference picks its padded sizes inside imglib2's FFTMethods; this list has no counterpart there. */
+int mvr_fft_lengths(int* out, int cap);
 
 /* ---- DeconViewPSF.init for all views in list order (deconvolution/DeconViewPSF.java:119-254,
  * deconvolution/DeconViews.java:69-70): kernel1 normalised to sum 1 (AdjustInput.normToSum1 with the
@@ -204,6 +208,12 @@ int mvr_unet_train_step(mvr_unet* u, const int* volIdx, const int* origins, int 
 /* psiOut[n] = max(minValue, network estimate) for the given views and weights */
 int mvr_unet_infer(mvr_unet* u, const float* const* imgs, const float* const* weights, const int n[3],
                    float minValue, float* psiOut);
+/* Inference of volumes above `voxels` (padded to multiples of 4; default and any value <= 0: 2^28) runs in
+ * z slabs that each carry mvr_unet_inference_halo() planes of context on either side; the result equals
+ * the single-window result.  Bounds the network's activation memory beside a resident deconvolution
+ * (about 285 bytes per window voxel).  No counterpart in the reference (its PsiInit has no network). */
+int mvr_unet_set_inference_window(mvr_unet* u, long long voxels);
+int mvr_unet_inference_halo(void);
 void mvr_unet_destroy(mvr_unet* u);
 /* mvr_decon_init with the network's estimate as psi (and as the psi restored by mvr_decon_reset_psi);
  * on an initialised handle only psi is replaced (PSFs, spectra, max[] and avg are kept) */

@@ -69,6 +69,13 @@ int mvr_get_arch_minor(int dev) {
   return p.minor;
 }
 
+int mvr_fft_lengths(int* out, int cap) {
+  int n = 0;
+  for (const int* p = strided_lengths(); *p; ++p, ++n)
+    if (out && n < cap) out[n] = *p;
+  return n;
+}
+
 int mvr_fft_convolve(int dev, const float* img, const int n[3], int ext, float extValue, const float* kernel,
                      const int k[3], float* out) {
   NULLCHECK(img);
@@ -597,6 +604,15 @@ int mvr_unet_infer(mvr_unet* u, const float* const* imgs, const float* const* we
   for (auto p : dw) if (p) (void)hipFree(p);
   if (dpsi) (void)hipFree(dpsi);
   return rc;
+}
+
+int mvr_unet_inference_halo(void) { return kInferHalo; }
+
+int mvr_unet_set_inference_window(mvr_unet* u, long long voxels) {
+  NULLCHECK(u);
+  if (voxels <= 0) voxels = 1ll << 28;
+  u->n.inferWindow = voxels;
+  return 0;
 }
 
 void mvr_unet_destroy(mvr_unet* u) {

@@ -178,7 +178,7 @@ void UNet::conv(int l, const uint16_t* in, int in_cs, int in_co, uint16_t* out, 
   c.N = N; c.D = D; c.H = H; c.W = W;
   c.taps = 27;
   c.relu = relu ? 1 : 0;
-  launch_conv_fwd(c, stream);
+  if (!launch_conv_fwd(c, stream)) convFailed = true;
 }
 
 // data gradient of layer l: din = conv(dout, flipped transposed weights), no bias
@@ -197,7 +197,7 @@ void UNet::dgrad(int l, const uint16_t* dout, int dout_cs, int dout_co, uint16_t
   c.N = N; c.D = D; c.H = H; c.W = W;
   c.taps = 27;
   c.relu = 0;
-  launch_conv_fwd(c, stream);
+  if (!launch_conv_fwd(c, stream)) convFailed = true;
 }
 
 int UNet::wgrad(int l, const uint16_t* in, int in_cs, const uint16_t* dout, int dout_cs, int dout_co, int N, int D,
@@ -224,6 +224,7 @@ int UNet::forward(Acts& a, bool training, const float* psiScale, float* psiOut, 
                   double* loss) {
   const int N = a.N, D0 = a.D, H0 = a.H, W0 = a.W;
   const int D1 = D0 / 2, H1 = H0 / 2, W1 = W0 / 2, D2 = D0 / 4, H2 = H0 / 4, W2 = W0 / 4;
+  convFailed = false;
   conv(0, a.x0, 16, 0, a.a1, 16, 0, N, D0, H0, W0, true);
   conv(1, a.a1, 16, 0, a.cat1, 48, 32, N, D0, H0, W0, true);                  // e1 -> cat1[32:48)
   launch_pool_fwd(a.cat1, 48, 32, a.p1, 16, 16, N, D1, H1, W1, stream);
@@ -248,6 +249,7 @@ int UNet::forward(Acts& a, bool training, const float* psiScale, float* psiOut, 
     conv(8, a.cat1, 48, 32, a.a5, 16, 0, N, D0, H0, W0, true, a.d2, 32, 32);
   }
   conv(9, a.a5, 16, 0, a.d1, 16, 0, N, D0, H0, W0, true);
+  UCHECK(!convFailed, "U-Net convolution refused: three x-y planes of the input exceed 32-bit offsets");
   HeadArgs h{};
   h.d1 = a.d1; h.d1_cs = 16;
   h.wo = head.w; h.bias = head.w + 16;
@@ -262,7 +264,7 @@ int UNet::forward(Acts& a, bool training, const float* psiScale, float* psiOut, 
   }
   h.psi = psiOut;
   h.W = W0; h.H = H0; h.D = D0;
-  if (outDims) { h.ox = outDims[0]; h.oy = outDims[1]; h.oz = outDims[2]; }
+  if (outDims) { h.ox = outDims[0]; h.oy = outDims[1]; h.oz = outDims[2]; h.zlo = outDims[3]; }
   const int nb = launch_head(h, stream);
   MVR_HIP(hipGetLastError());
   if (training && loss) {
@@ -280,6 +282,7 @@ int UNet::backward(Acts& a) {
   const int N = a.N, D0 = a.D, H0 = a.H, W0 = a.W;
   const int D1 = D0 / 2, H1 = H0 / 2, W1 = W0 / 2, D2 = D0 / 4, H2 = H0 / 4, W2 = W0 / 4;
   const size_t v0 = (size_t)N * D0 * H0 * W0, v1 = v0 / 8, v2 = v0 / 64;
+  convFailed = false;
   HeadArgs h{};
   h.d1 = a.d1; h.d1_cs = 16; h.wo = head.w; h.bias = head.w + 16; h.avg = a.avg; h.dr = a.dr; h.nvox = v0;
   const int hb = launch_head_bwd(h, a.g_d1, bpart, stream);
@@ -322,6 +325,7 @@ int UNet::backward(Acts& a) {
   dgrad(1, a.g_cat1, 48, 32, a.g_a1, 16, 0, N, D0, H0, W0);
   launch_relu_bwd(a.g_a1, 16, 0, a.a1, 16, 0, 16, v0, stream);
   wgrad(0, a.x0, 16, a.g_a1, 16, 0, N, D0, H0, W0);
+  UCHECK(!convFailed, "U-Net convolution refused: three x-y planes of the input exceed 32-bit offsets");
   MVR_HIP(hipGetLastError());
   return 0;
 }
@@ -445,10 +449,27 @@ int UNet::infer(const std::vector<float*>& img, const std::vector<float*>& w, co
   if (weighted_mean(img, w, n, &mean)) return -1;
   const float scale = (float)std::max(1e-6, mean);
   auto up4 = [](int v) { return (v + 3) / 4 * 4; };
-  if (ensure_acts(inf, 1, up4(N3[2]), up4(N3[1]), up4(N3[0]), false)) return -1;
-  const int o[3] = {0, 0, 0};
-  pack(inf, 0, img, w, N3, o, 1.f / scale);
-  if (forward(inf, false, &scale, psiDev, N3, minValue, nullptr)) return -1;
+  const int X = up4(N3[0]), Y = up4(N3[1]), Z = up4(N3[2]);
+  const long long plane = (long long)X * Y;
+  // The volume runs as one window when it fits the voxel budget, else as z slabs of `slab` planes, each
+  // with kInferHalo planes of context on either side (clipped at the volume's faces).  Slab starts and
+  // the halo are multiples of 4, so every window keeps the volume's pooling grid, and the halo exceeds
+  // the network's receptive-field radius: the kept planes are the ones a single window computes.
+  int slab = Z;
+  if ((long long)Z * plane > inferWindow) {
+    slab = (int)((inferWindow / plane - 2 * kInferHalo) / 4 * 4);
+    UCHECK(slab >= 4, "inference window budget too small for one x-y plane plus its halo");
+  }
+  for (int zs = 0; zs < N3[2]; zs += slab) {
+    const int ze = std::min(N3[2], zs + slab);
+    const int ws = std::max(0, zs - kInferHalo), we = std::min(N3[2], ze + kInferHalo);
+    if (ensure_acts(inf, 1, up4(we - ws), Y, X, false)) return -1;
+    const int o[3] = {0, 0, ws};
+    pack(inf, 0, img, w, N3, o, 1.f / scale);
+    // psi planes [zs, ze) = window planes [zs - ws, ze - ws)
+    const int od[4] = {N3[0], N3[1], ze - ws, zs - ws};
+    if (forward(inf, false, &scale, psiDev + (size_t)ws * N3[1] * N3[0], od, minValue, nullptr)) return -1;
+  }
   MVR_HIP(hipStreamSynchronize(stream));
   return 0;
 }

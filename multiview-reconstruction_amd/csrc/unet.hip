@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256) void k_head(HeadArgs a) {
     if (a.psi) {
       // net dims (W, H, D) may be the output dims (ox, oy, oz) padded up; only inner voxels are written
       const int x = (int)(v % a.W), y = (int)((v / a.W) % a.H), z = (int)(v / ((size_t)a.W * a.H));
-      if (x < a.ox && y < a.oy && z < a.oz)
+      if (x < a.ox && y < a.oy && z >= a.zlo && z < a.oz)
         a.psi[((size_t)z * a.oy + y) * a.ox + x] = fmaxf(a.minValue, a.scale * pred);
     }
   }
@@ -416,9 +416,9 @@ static inline int grid1(size_t n, int cap = 8192) {
   return (int)(b < 1 ? 1 : (b > (size_t)cap ? (size_t)cap : b));
 }
 
-void launch_conv_fwd(const ConvArgs& a, hipStream_t s) {
+bool launch_conv_fwd(const ConvArgs& a, hipStream_t s) {
   // every layer of the network has cin, cout and channel strides that are multiples of 16
-  (void)launch_conv_tiled(a, s);
+  return launch_conv_tiled(a, s);
 }
 void launch_conv_wgrad(const WgradArgs& a, int nparts, hipStream_t s) {
   hipLaunchKernelGGL(k_conv_wgrad, dim3(nparts, a.taps), dim3(256), 0, s, a);

@@ -63,7 +63,8 @@ struct HeadArgs {
   float scale, minValue;
   size_t nvox;
   int W, H, D;           // net dims of one sample (psi output only)
-  int ox, oy, oz;        // output volume dims, <= net dims
+  int ox, oy, oz;        // output volume dims, <= net dims; psi holds planes [zlo, oz) of them
+  int zlo;
 };
 
 struct AdamArgs {
@@ -78,7 +79,7 @@ struct AdamArgs {
   float lr, beta1, beta2, eps, c1, c2;  // c1 = 1/(1-beta1^t), c2 = 1/(1-beta2^t)
 };
 
-void launch_conv_fwd(const ConvArgs& a, hipStream_t s);
+bool launch_conv_fwd(const ConvArgs& a, hipStream_t s);  // false: shape not supported, nothing launched
 // LDS-tiled 3^3 convolution (unet_conv.hip); false when the shape is not one it covers
 bool launch_conv_tiled(const ConvArgs& a, hipStream_t s);
 void launch_conv_wgrad(const WgradArgs& a, int nparts, hipStream_t s);

@@ -88,9 +88,12 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a) {
     const int sh = up ? 1 : 0;
     const int sD = a.D >> sh, sH = a.H >> sh, sW = a.W >> sh;
     const int cs = up ? a.up_cs : a.in_cs;
-    // sample base in 64 bits once; offsets within a sample fit 32 bits (checked at launch)
-    const uint16_t* src = (up ? a.up_in + cb * 16 : a.in + (cb * 16 - a.up_c)) + (size_t)n * sD * sH * sW * cs +
-                          half * 8;
+    // base of the first source plane the halo touches, in 64 bits; offsets from it span at most three
+    // planes and fit 32 bits (checked at launch).  zb = max(z - 1, 0) >> sh is a valid plane, so the
+    // dummy address of an invalid lane (offset 0) is inside the tensor.
+    const int zb = max(z - 1, 0) >> sh;
+    const uint16_t* src = (up ? a.up_in + cb * 16 : a.in + (cb * 16 - a.up_c)) +
+                          ((size_t)n * sD + zb) * sH * sW * cs + half * 8;
     int v0 = threadIdx.x >> 1;
     int hx = v0 % HX, hy = (v0 / HX) % HY, hz = v0 / (HX * HY);
     constexpr int DX = 128 % HX, DY = (128 / HX) % HY, DZ = 128 / (HX * HY);
@@ -108,7 +111,7 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a) {
       const int xx = x0 + hx - 1, yy = y0 + hy - 1, zz = z + hz - 1;
       const bool ok = (threadIdx.x + 256 * k) < HV * 2 && (unsigned)xx < (unsigned)a.W &&
                       (unsigned)yy < (unsigned)a.H && (unsigned)zz < (unsigned)a.D;
-      const int gv = ((zz >> sh) * sH + (yy >> sh)) * sW + (xx >> sh);
+      const int gv = (((zz >> sh) - zb) * sH + (yy >> sh)) * sW + (xx >> sh);
       hbuf[k] = *reinterpret_cast<const uint4*>(src + (ok ? gv * cs : 0));
       if (!ok) hbuf[k] = make_uint4(0u, 0u, 0u, 0u);
     }
@@ -207,8 +210,8 @@ void launch_tx(const ConvArgs& a, hipStream_t s) {
 // 3^3 convolutions with cin and cout multiples of 16; the tile width follows W
 bool launch_conv_tiled(const ConvArgs& a, hipStream_t s) {
   if (a.taps != 27 || a.cin % 16 != 0 || a.cout % 16 != 0 || a.in_cs % 16 != 0) return false;
-  // offsets within one sample are 32-bit in the staging
-  if ((long long)a.D * a.H * a.W * std::max(a.in_cs, a.up_cs) >= (1ll << 31)) return false;
+  // the staging's offsets from its first halo plane cover three planes in 32 bits
+  if (3ll * a.H * a.W * std::max(a.in_cs, a.up_cs) >= (1ll << 31)) return false;
   if (a.up_c && (a.up_c % 16 != 0 || a.up_cs % 16 != 0 || !a.up_in || (a.W | a.H | a.D) & 1)) return false;
   if (a.W > 32)
     launch_tx<64>(a, s);

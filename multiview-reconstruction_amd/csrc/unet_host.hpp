@@ -17,6 +17,10 @@ namespace mvr {
 constexpr int kLayers = 10;
 constexpr int kWgradParts = 256;
 
+// planes of context on either side of an inference slab: a multiple of 4 above the network's receptive-field
+// radius of 23 planes (4 at full resolution, 8 at half, 8 at quarter resolution, plus the pooling grid's offset)
+constexpr int kInferHalo = 32;
+
 struct UNet {
   struct Param {
     size_t count = 0;
@@ -52,6 +56,8 @@ struct UNet {
   std::vector<TrainVolume> vols;
   float *wpart = nullptr, *bpart = nullptr;
   double* lossPart = nullptr;
+  bool convFailed = false;         // a convolution launch was refused since the last check
+  long long inferWindow = 1ll << 28;  // voxel budget of one inference window (z slab + halos)
 
   int create(int dev, int V, unsigned long long seed);
   void destroy();

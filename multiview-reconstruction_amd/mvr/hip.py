@@ -54,6 +54,7 @@ def lib():
         L.mvr_get_arch_minor.argtypes = [ctypes.c_int]
         L.mvr_convolution3d_fft_inplace.argtypes = [vp, _i32p, vp, _i32p, ctypes.c_int]
         L.mvr_fft_convolve.argtypes = [ctypes.c_int, vp, _i32p, ctypes.c_int, ctypes.c_float, vp, _i32p, vp]
+        L.mvr_fft_lengths.argtypes = [vp, ctypes.c_int]
         L.mvr_init_psfs.argtypes = [ctypes.c_int, vp, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, vp, vp]
         L.mvr_block_thread_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _i32p, ctypes.c_float,
                                               ctypes.c_float, ctypes.c_int]
@@ -103,6 +104,8 @@ def lib():
         L.mvr_unet_train_step.argtypes = [vp, _i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                           ctypes.c_int, _f64p]
         L.mvr_unet_infer.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), _i32p, ctypes.c_float, vp]
+        L.mvr_unet_set_inference_window.argtypes = [vp, ctypes.c_longlong]
+        L.mvr_unet_inference_halo.argtypes = []
         L.mvr_unet_destroy.argtypes = [vp]
         L.mvr_unet_destroy.restype = None
         _lib = L
@@ -135,6 +138,14 @@ def device_name(dev=0) -> str:
     buf = ctypes.create_string_buffer(256)
     _check(lib().mvr_get_device_name(dev, buf, 256))
     return buf.value.decode()
+
+
+def fft_lengths():
+    """FFT lengths with compiled kernels (ascending)."""
+    n = lib().mvr_fft_lengths(None, 0)
+    out = (ctypes.c_int * n)()
+    lib().mvr_fft_lengths(out, n)
+    return list(out)
 
 
 def fft_convolve(img, kernel, ext="mirror", value=0.0, dev=0):
@@ -353,6 +364,11 @@ class UNet:
     LAYERS_CIN = (16, 16, 16, 32, 32, 64, 96, 32, 48, 16)
     LAYERS_COUT = (16, 16, 32, 32, 64, 64, 32, 32, 16, 16)
 
+    @staticmethod
+    def inference_halo():
+        """Planes of context on either side of an inference slab."""
+        return int(lib().mvr_unet_inference_halo())
+
     def __init__(self, n_views, seed=0, dev=0):
         h = ctypes.c_void_p()
         _check(lib().mvr_unet_create(ctypes.byref(h), dev, n_views, seed))
@@ -405,6 +421,11 @@ class UNet:
         _check(lib().mvr_unet_infer(self.h, _ptr_array(imgs), _ptr_array(weights), _dims(out.shape), min_value,
                                     _ptr(out)))
         return out
+
+    def set_inference_window(self, voxels):
+        """Voxel budget of one inference window; larger volumes run in z slabs with inference_halo()
+        planes of context on either side (<= 0 restores the default, 2^28)."""
+        _check(lib().mvr_unet_set_inference_window(self.h, int(voxels)))
 
     def close(self):
         if self.h:

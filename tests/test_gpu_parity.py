@@ -46,6 +46,50 @@ def test_fft_convolve_matches_oracle(oracle, shape, kshape, ext, value):
     assert rel(got, want) < 3e-6
 
 
+def test_every_compiled_fft_length(oracle):
+    # each compiled length L as the FFT length of one axis at a time (extent L - 2, 3-voxel kernel, the other
+    # axes short): the register FFTs of every length, x halves included, against the FP64 oracle
+    lengths = hip.fft_lengths()
+    assert lengths == sorted(lengths) and lengths[-1] >= 1152
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for L in lengths:
+        for axis in range(3):
+            shape = [3, 4, 5]            # z, y, x
+            shape[2 - axis] = L - 2
+            img = rng.uniform(0.5, 2.0, shape).astype(np.float32)
+            k = rng.uniform(0.0, 1.0, (3, 3, 3)).astype(np.float32)
+            got = hip.fft_convolve(img, k, "mirror", 0.0)
+            want = oracle.fft_convolve(img.astype(np.float64), k.astype(np.float64), "mirror", dtype=np.float64)
+            e = rel(got, want)
+            worst = max(worst, e)
+            assert e < 3e-6, (L, axis, e)
+    print(f"{len(lengths)} lengths x 3 axes, worst rel-L2 {worst:.2e}")
+
+
+@pytest.mark.parametrize("shape", [(574, 6, 1150), (6, 1150, 1278)])
+def test_resident_decon_long_axes(oracle, shape):
+    # the x-ratio / x-update passes and the strided passes at the long lengths of a 1024^2 x 512 frame with
+    # 61^3 PSFs (FFT grid 1152 x 1152 x 576), here with short PSFs and thin volumes
+    rng = np.random.default_rng(sum(shape))
+    V = 2
+    imgs = rng.uniform(1.0, 40.0, (V,) + shape).astype(np.float32)
+    ws = rng.uniform(0.0, 1.0, (V,) + shape).astype(np.float32)
+    ks = rng.uniform(0.01, 1.0, (V, 3, 3, 3)).astype(np.float32)
+    ref = oracle.deconvolve(imgs, ws, ks, psf_type="EFFICIENT_BAYESIAN", lam=6e-4, iterations=3)
+    d = hip.Decon(shape, V, "EFFICIENT_BAYESIAN", "SEQ", 6e-4)
+    for v in range(V):
+        d.set_view(v, imgs[v], ws[v], ks[v])
+    d.init()
+    d.run(3)
+    P = d.geometry()["P"]
+    e = rel(d.psi(), ref["psi"])
+    d.close()
+    print(f"{shape}: FFT grid {P}, rel-L2 vs oracle {e:.2e}")
+    assert max(P) >= 1152
+    assert e < 1e-4
+
+
 def test_inplace_convolution_is_mirror_convolution(oracle):
     img = RNG.uniform(0.5, 2.0, (24, 20, 16)).astype(np.float32)
     k = RNG.uniform(0.0, 1.0, (5, 7, 3)).astype(np.float32)

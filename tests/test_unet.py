@@ -275,3 +275,53 @@ def test_decon_init_learned_uses_network_estimate():
     assert abs(d.rel_l2() - 0.5) < 1e-12
     d.close()
     net.close()
+
+
+def test_receptive_field_within_inference_halo():
+    # With every ReLU active (large biases) the network is affine in its input, so the planes an input
+    # plane reaches are its structural receptive field; the inference slabs' halo must exceed it for the
+    # slab-wise result to equal the single-window one.  Four plane alignments cover the pooling grid.
+    from mvr import hip
+    V = 2
+    layers, head = U.unpack(U.random_params(V, seed=5))
+    p = U.pack([(w, b + 50.0) for w, b in layers], head).astype(np.float64)
+    rng = np.random.default_rng(0)
+    D, H, W = 112, 8, 8
+    x0 = np.zeros((1, D, H, W, 16))
+    x0[..., :V + 1] = rng.uniform(0, 1, (1, D, H, W, V + 1))
+    avg = np.zeros((1, D, H, W))
+    base, _ = U.forward(p, x0, avg)
+    radius = 0
+    for k in range(52, 56):
+        x = x0.copy()
+        x[0, k, :, :, :V + 1] += 1.0
+        out, _ = U.forward(p, x, avg)
+        reach = np.nonzero(np.abs(out - base).max(axis=(0, 2, 3)) > 1e-9 * np.abs(base).max())[0]
+        radius = max(radius, k - reach.min(), reach.max() - k)
+    assert radius == 23   # 4 planes at full resolution, 8 at half, 8 at quarter, pooling grid offset
+    halo = hip.UNet.inference_halo()
+    assert halo > radius and halo % 4 == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("slab", [8, 36])
+def test_unet_slab_inference_equals_single_window(slab):
+    # z slabs with their halo reproduce the single-window estimate exactly, also for a depth that is not a
+    # multiple of 4 (the last slab carries the padded planes)
+    from mvr import hip
+    shape, V = (150, 20, 28), 3
+    rng = np.random.default_rng(slab)
+    imgs = [rng.uniform(0.0, 3.0, shape).astype(np.float32) for _ in range(V)]
+    ws = [rng.uniform(0.0, 1.0, shape).astype(np.float32) for _ in range(V)]
+    net = _gpu_net(V, 0)
+    net.set_params(U.random_params(V, seed=13))
+    whole = net.infer(imgs, ws)
+    halo = hip.UNet.inference_halo()
+    net.set_inference_window((slab + 2 * halo) * 20 * 28)
+    parts = net.infer(imgs, ws)
+    net.set_inference_window(0)
+    again = net.infer(imgs, ws)
+    net.close()
+    print(f"slab {slab} + 2 x {halo} planes: max |diff| {np.abs(parts - whole).max():.3e}")
+    np.testing.assert_array_equal(parts, whole)
+    np.testing.assert_array_equal(again, whole)
