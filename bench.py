@@ -46,7 +46,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"])
     ap.add_argument("--autotune", type=int, default=0, help="autotune FFT lengths over this many candidates per axis")
     ap.add_argument("--graph", action="store_true", help="replay a graph-captured iteration outside profiled steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -68,7 +68,7 @@ def main():
     from mvr import hip
     from mvr import synthetic as syn
 
-    w = {"c1": syn.C1, "c2": syn.C2, "c3": syn.C3}[args.workload]
+    w = {"c1": syn.C1, "c2": syn.C2, "c3": syn.C3, "c5": syn.C5}[args.workload]
     if args.iterations > 0:
         import dataclasses
         w = dataclasses.replace(w, iterations=args.iterations)
@@ -82,17 +82,31 @@ def main():
             torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
-    t0 = time.time()
-    inp = syn.make_inputs(w)
-    log(f"inputs generated in {time.time() - t0:.1f}s")
-    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
     V = w.n_views
     nz, ny, nx = w.shape_zyx
     nvox = nx * ny * nz
-
-    d = hip.Decon(w.shape_zyx, V, w.psf_type, "SEQ", w.lam)
-    for v in range(V):
-        d.set_view(v, imgs[v], ws[v], ks[v])
+    t0 = time.time()
+    device_inputs = nvox * V > (1 << 30)   # the host generator would take minutes: generate on the GPU
+    crop0 = None
+    if device_inputs:
+        import torch
+        inp = syn.make_inputs_device(w, "cuda", log)
+        ks = inp["psfs"]
+        d = hip.Decon(w.shape_zyx, V, w.psf_type, "SEQ", w.lam)
+        for v in range(V):
+            d.set_view_device(v, inp["imgs"][v].data_ptr(), inp["weights"][v].data_ptr(), ks[v])
+        if not args.no_cpu_baseline:   # the CPU sample block's inputs (view 0)
+            crop0 = cpu_block_inputs(inp, w, ks.shape[1:])
+        del inp
+        torch.cuda.empty_cache()
+        imgs = ws = None
+    else:
+        inp = syn.make_inputs(w)
+        imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+        d = hip.Decon(w.shape_zyx, V, w.psf_type, "SEQ", w.lam)
+        for v in range(V):
+            d.set_view(v, imgs[v], ws[v], ks[v])
+    log(f"inputs generated and uploaded in {time.time() - t0:.1f}s")
     t0 = time.time()
     d.init()
     log(f"psf init + spectra + psi init in {time.time() - t0:.2f}s, fft grid {d.geometry()}")
@@ -139,12 +153,17 @@ def main():
     finite = bool(np.isfinite(psi).all())
 
     learned = None
-    if not args.no_learned and args.workload == "c2":
-        learned = time_to_converge(d, w, args.tol, args.reference_iterations)
+    if not args.no_learned and args.workload in ("c2", "c5"):
+        learned = time_to_converge(d, w, args.tol, args.reference_iterations, args.workload)
+    if crop0 is not None:
+        crop0["k1"], crop0["k2"] = d.kernels(0, ks.shape[1:])
+        crop0["psi"], crop0["max"] = d.max_avg()[1], d.max_avg()[0][0]
     d.close()
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and crop0 is not None:
+        cpu = cpu_baseline_block(crop0, w, V)
+    elif not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import oracle
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
@@ -183,6 +202,42 @@ def main():
     print(json.dumps(out))
 
 
+def cpu_block_inputs(inp, w, K):
+    """Inputs of the reference's first block of view 0 (effective block at the origin, margin K - 1, the image
+    and weight zero outside the volume): host copies of the covered part."""
+    be = w.extra["block_effective"]
+    m = [k - 1 for k in K]
+    hi = [min(n, be + mk) for n, mk in zip(w.shape_zyx, m)]
+    img = inp["imgs"][0][:hi[0], :hi[1], :hi[2]].cpu().numpy()
+    wt = inp["weights"][0][:hi[0], :hi[1], :hi[2]].cpu().numpy()
+    return {"img": img, "w": wt, "margin": m, "block": [be + 2 * mk for mk in m]}
+
+
+def cpu_baseline_block(c, w, V):
+    """The oracle's ComputeBlockSeqThreadCPU.runIteration on the first block of view 0 in iteration 1 (psi is
+    the AvgPrecise constant there, so its mirror copy is that constant), timed on this host."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import numpy as np
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = oracle.set_threads(threads)
+    B, m = c["block"], c["margin"]
+    img = np.zeros(B, np.float32)
+    wt = np.zeros(B, np.float32)
+    sz = c["img"].shape
+    img[m[0]:m[0] + sz[0], m[1]:m[1] + sz[1], m[2]:m[2] + sz[2]] = c["img"]
+    wt[m[0]:m[0] + sz[0], m[1]:m[1] + sz[1], m[2]:m[2] + sz[2]] = c["w"]
+    psi = np.full(B, np.float32(c["psi"]), np.float32)
+    t0 = time.perf_counter()
+    oracle.block_seq(psi, img, wt, c["k1"], c["k2"], w.lam, 1e-4, float(c["max"]), 8)
+    secs = time.perf_counter() - t0
+    eff = w.extra["block_effective"] ** 3
+    return {"value": eff / V / secs, "unit": "voxel-iters/s", "cores": threads, "kind": "port",
+            "sample": f"one {B[2]}x{B[1]}x{B[0]} block ({w.extra['block_effective']}^3 effective) of view 0 in "
+                      f"iteration 1 of the same workload (1/{V} iteration of {eff} voxels), reference block "
+                      f"structure, oracle restatement in FP32 on {threads} OpenMP threads; {secs:.2f}s"}
+
+
 def unet_flops(shape_zyx):
     """Multiply-adds x 2 of one U-Net inference (3^3 layers at their level's voxel count, padded channels)."""
     from mvr import hip
@@ -218,17 +273,20 @@ def _first_below(traj, start, tol):
 TOLERANCES = (1e-2, 0.05, 0.1, 0.15, 0.2, 0.3)
 
 
-def time_to_converge(d, w, tol, ref_iterations):
+def time_to_converge(d, w, tol, ref_iterations, workload="c2"):
     """Wall time to rel-L2 <= tol to plain RL run to ref_iterations: plain PsiInitAvgPrecise start vs the
     learned start (U-Net inference included).  The common set-up (PSFs, spectra, AvgPrecise pass) is
     outside both timed regions.  Also reported for the looser tolerances of TOLERANCES."""
     from mvr import learned as L
+    weights = L.TRAINING[workload][1]
     d.reset_psi()
     d.run(ref_iterations)
     d.set_reference(d.psi())
+    log(f"reference: {ref_iterations} plain iterations")
     d.reset_psi()
     plain0 = (0.0, d.rel_l2())
     plain = _trajectory(d, 0.0, ref_iterations)
+    log("plain trajectory done")
     res = {"tol": tol, "reference": f"plain RL, {ref_iterations} iterations from PsiInitAvgPrecise",
            "plain": {"rel_l2_start": plain0[1], "rel_l2_after_20": plain[19][1]}}
     # PsiInitBlurredFused, the reference GUI's default initialiser: its fusion + blur time is the init
@@ -241,13 +299,14 @@ def time_to_converge(d, w, tol, ref_iterations):
     t_blur = max(0.0, time.perf_counter() - t0 - t_avg)
     blurred0 = (t_blur, d.rel_l2())
     blurred = _trajectory(d, t_blur, ref_iterations)
+    log("BlurredFused trajectory done")
     res["blurred_fused"] = {"init_seconds_over_avg_precise": t_blur, "rel_l2_start": blurred0[1],
                             "rel_l2_after_20": blurred[19][1]}
-    if not os.path.exists(L.DEFAULT_WEIGHTS):
+    if not os.path.exists(weights):
         res["learned"] = None
         res["note"] = "no trained weights file"
         return res
-    net = L.load()
+    net = L.load(weights)
     d.init_learned(net)  # warm-up: activation buffers
     t_inf = []
     for _ in range(3):
@@ -258,12 +317,13 @@ def time_to_converge(d, w, tol, ref_iterations):
     inf = min(t_inf)
     learned0 = (inf, d.rel_l2())
     learned = _trajectory(d, inf, ref_iterations)
+    log("learned trajectory done")
     best = min(range(len(learned)), key=lambda i: learned[i][1])
     fl = unet_flops(w.shape_zyx)
     res["learned"] = {"inference_seconds": inf, "rel_l2_start": learned0[1], "rel_l2_min": learned[best][1],
                       "rel_l2_min_at_iteration": best + 1, "inference_tflops": fl / inf / 1e12,
                       "inference_mfma_frac_bf16_dense": fl / inf / 1e12 / 2500.0}
-    res["weights"] = os.path.relpath(L.DEFAULT_WEIGHTS, REPO)
+    res["weights"] = os.path.relpath(weights, REPO)
     table = []
     for t in sorted(set((tol,) + TOLERANCES)):
         ip, tp = _first_below(plain, plain0, t)

@@ -119,16 +119,21 @@ __device__ __forceinline__ void rdft(float2* v) {
 struct Split2 {
   int A, B;
 };
-// L = A x B with A >= B, both smooth and <= 32, max(A, B) as small as possible; {0, 0} if none
-__host__ __device__ constexpr Split2 split2(int L) {
+// L = A x B with A >= B, both smooth and <= amax, max(A, B) as small as possible; {0, 0} if none
+__host__ __device__ constexpr Split2 split2_max(int L, int amax) {
   Split2 best{0, 0};
   for (int b = 2; b <= 32; ++b) {
     if (L % b) continue;
     const int a = L / b;
-    if (a < b || a > 32 || !is_smooth(a) || !is_smooth(b)) continue;
+    if (a < b || a > amax || !is_smooth(a) || !is_smooth(b)) continue;
     if (best.A == 0 || a < best.A) best = Split2{a, b};
   }
   return best;
+}
+// factors up to 32; lengths above 1024 (no such split) take A up to 40 (1080 = 36 x 30, 1152 = 36 x 32,
+// 1280 = 40 x 32)
+__host__ __device__ constexpr Split2 split2(int L) {
+  return split2_max(L, 32).A ? split2_max(L, 32) : split2_max(L, 40);
 }
 
 }  // namespace mvr

@@ -375,8 +375,10 @@ __global__ __launch_bounds__(kThreads) void k_strided(StridedGeom gm, const floa
 // one LDS round trip per transform.
 template <int L>
 __host__ __device__ constexpr int s2_width() {
-  return L <= 512 ? 16 : 8;  // 16 columns = 128-byte runs (32 was slower at L = 300, 8 slower at L = 400)
+  // 16 columns = 128-byte runs (32 was slower at L = 300, 8 slower at L = 400); 8 above L = 640 (LDS)
+  return L <= 640 ? 16 : 8;
 }
+constexpr bool kStridedXcdOrder = true;
 template <int L>
 __host__ __device__ constexpr int s2_threads() {
   return s2_width<L>() * split2(L).A;
@@ -395,10 +397,17 @@ __global__ __launch_bounds__(s2_threads<L>()) void k_strided2(StridedGeom gm, co
   __shared__ float2 tw[L];
   for (int i = threadIdx.x; i < L; i += NT) tw[i] = twg[i];
   const int w = threadIdx.x % W, q = threadIdx.x / W;
-  const int col0 = blockIdx.x * W;
+  // Work-groups are dealt to the 8 XCDs in turn; the tile order is remapped so that each XCD sweeps a
+  // contiguous range of (column tile, plane) tiles: neighbouring column tiles, which share cache lines
+  // when a tile row is shorter than a line, meet in one L2.
+  unsigned tid = blockIdx.y * gridDim.x + blockIdx.x;
+  const unsigned ntile = gridDim.x * gridDim.y;
+  if (kStridedXcdOrder && ntile % 8 == 0) tid = (tid % 8) * (ntile / 8) + tid / 8;
+  const int plane = (int)(tid / gridDim.x);
+  const int col0 = (int)(tid % gridDim.x) * W;
   const bool colOk = w < gm.ncols - col0;
-  const float2* src = in + (size_t)blockIdx.y * gm.planeStride + col0 + w;
-  float2* dst = out + (size_t)blockIdx.y * gm.planeStride + col0 + w;
+  const float2* src = in + (size_t)plane * gm.planeStride + col0 + w;
+  float2* dst = out + (size_t)plane * gm.planeStride + col0 + w;
   const size_t ls = gm.lineStride;
   float2 v[A];
   if constexpr (MODE != 1) {
@@ -434,7 +443,7 @@ __global__ __launch_bounds__(s2_threads<L>()) void k_strided2(StridedGeom gm, co
       return;
     } else {
       // spectrum product, then inverse step 1 over k2 in registers
-      const float2* sp = spec + (size_t)blockIdx.y * gm.planeStride + col0 + w;
+      const float2* sp = spec + (size_t)plane * gm.planeStride + col0 + w;
 #pragma unroll
       for (int k2 = 0; k2 < B; ++k2) {
         const float2 kv = colOk ? sp[(size_t)(q + A * k2) * ls] : make_float2(0.f, 0.f);

@@ -123,6 +123,7 @@ int UNet::copy_params(float* hostOut, const float* hostIn, bool grads) {
 // ------------------------------------------------------------------ activations
 int UNet::Acts::alloc(int N_, int D_, int H_, int W_, bool grads) {
   N = N_; D = D_; H = H_; W = W_;
+  Dcap = D_;
   const size_t v0 = (size_t)N * D * H * W, v1 = v0 / 8, v2 = v0 / 64;
   struct B { uint16_t** p; uint16_t** gp; size_t n; };
   const B bufs[] = {{&x0, nullptr, v0 * 16},   {&a1, &g_a1, v0 * 16},  {&cat1, &g_cat1, v0 * 48},
@@ -149,13 +150,16 @@ void UNet::Acts::release() {
     if (*p) { (void)hipFree(*p); *p = nullptr; }
   for (float** p : {&avg, &target, &dr})
     if (*p) { (void)hipFree(*p); *p = nullptr; }
-  N = D = H = W = 0;
+  N = D = H = W = Dcap = 0;
 }
 
 int UNet::ensure_acts(Acts& a, int N_, int D_, int H_, int W_, bool grads) {
   UCHECK(D_ % 4 == 0 && H_ % 4 == 0 && W_ % 4 == 0 && D_ >= 4 && H_ >= 4 && W_ >= 4,
          "network input dims must be multiples of 4");
-  if (a.N == N_ && a.D == D_ && a.H == H_ && a.W == W_ && (!grads || a.g_a1)) return 0;
+  if (a.N == N_ && a.Dcap >= D_ && a.H == H_ && a.W == W_ && (!grads || a.g_a1)) {
+    a.D = D_;  // the buffers hold Dcap planes per sample; the tensors use the first D
+    return 0;
+  }
   a.release();
   return a.alloc(N_, D_, H_, W_, grads);
 }
@@ -460,6 +464,10 @@ int UNet::infer(const std::vector<float*>& img, const std::vector<float*>& w, co
     slab = (int)((inferWindow / plane - 2 * kInferHalo) / 4 * 4);
     UCHECK(slab >= 4, "inference window budget too small for one x-y plane plus its halo");
   }
+  int maxDepth = 0;
+  for (int zs = 0; zs < N3[2]; zs += slab)
+    maxDepth = std::max(maxDepth, up4(std::min(N3[2], zs + slab + kInferHalo) - std::max(0, zs - kInferHalo)));
+  if (ensure_acts(inf, 1, maxDepth, Y, X, false)) return -1;  // one allocation for every window
   for (int zs = 0; zs < N3[2]; zs += slab) {
     const int ze = std::min(N3[2], zs + slab);
     const int ws = std::max(0, zs - kInferHalo), we = std::min(N3[2], ze + kInferHalo);

@@ -31,6 +31,7 @@ struct UNet {
   };
   struct Acts {
     int N = 0, D = 0, H = 0, W = 0;
+    int Dcap = 0;  // depth the buffers were allocated for (D <= Dcap)
     uint16_t *x0 = nullptr, *a1 = nullptr, *cat1 = nullptr, *p1 = nullptr, *a2 = nullptr, *cat2 = nullptr,
              *p2 = nullptr, *a3 = nullptr, *b3 = nullptr, *a4 = nullptr, *d2 = nullptr, *a5 = nullptr,
              *d1 = nullptr;

@@ -56,6 +56,11 @@ struct XCfg {
   static constexpr int NT = RW * T;                // threads per work-group
   static constexpr int RS = M + 5;                 // LDS row stride (complex): M + 1 used, +4 against bank conflicts
   static constexpr int PX = 2 * M;
+  // x_update: twiddle tables in LDS unless leaving them in global memory (read through L1) fits one more
+  // work-group per CU into the 160 KB of LDS
+  static constexpr int LDS_ROWS = RW * RS * 8 + RW * 16;   // the update kernel's rows and row tables
+  static constexpr int LDS_TW = (2 * M + 1) * 8;
+  static constexpr bool TWL = (160 * 1024) / (LDS_ROWS + LDS_TW) >= (160 * 1024) / LDS_ROWS;
 };
 
 template <int M>
@@ -354,11 +359,15 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
                                                             const float2* __restrict__ wr) {
   using C = XCfg<M>;
   __shared__ float2 buf[C::RW * C::RS];
-  __shared__ float2 tw[M];
-  __shared__ float2 twr[M + 1];
+  // the update pass only runs inverse transforms: its twiddles stay in global memory when that fits one
+  // more work-group per CU (the forward passes read theirs more often and keep them in LDS)
+  __shared__ float2 twS[C::TWL ? M : 1];
+  __shared__ float2 twrS[C::TWL ? M + 1 : 1];
+  const float2* tw = C::TWL ? twS : twM;
+  const float2* twr = C::TWL ? twrS : wr;
   __shared__ long long psiRow[C::RW];
   __shared__ size_t rowOff[C::RW];
-  x2_load_twiddles<M>(tw, twr, twM, wr);
+  if (C::TWL) x2_load_twiddles<M>(twS, twrS, twM, wr);
   const int nrows = d.D[1] * d.D[2];
   const int row0 = blockIdx.x * C::RW;
   if (threadIdx.x < C::RW) {

@@ -12,9 +12,13 @@ Optimization-II workload with seeds disjoint from the benchmark's, views blurred
 RL run to 200 iterations from PsiInitAvgPrecise by the HIP engine.  Training draws a fixed seeded set
 of 64^3 patches and runs Adam on the GPU through the library's own forward and backward kernels.
 
+The 1024^2 x 512 / 8-view Efficient-Bayesian workload has its own network (8 view channels), trained the
+same way on 256^3 volumes with that workload's phantom density, PSFs, noise and weights.
+
 Command line (writes the weights file)::
 
-    python -m mvr.learned --out mvr/weights/unet_c2_6views.npz
+    python -m mvr.learned --workload c2 --out mvr/weights/unet_c2_6views.npz
+    python -m mvr.learned --workload c5 --out mvr/weights/unet_c5_8views.npz
 """
 from __future__ import annotations
 
@@ -33,6 +37,9 @@ from . import synthetic as syn
 
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
 DEFAULT_WEIGHTS = os.path.join(WEIGHTS_DIR, "unet_c2_6views.npz")
+# training workload and weights file per benchmark workload
+TRAINING = {"c2": (syn.C2, DEFAULT_WEIGHTS),
+            "c5": (syn.C5_TRAIN, os.path.join(WEIGHTS_DIR, "unet_c5_8views.npz"))}
 TRAIN_SEEDS = tuple(range(100, 108))   # the benchmark uses seed index 2, the held-out check 90
 HELD_OUT_SEED = 90
 
@@ -142,7 +149,8 @@ def meta(path=DEFAULT_WEIGHTS) -> dict:
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--workload", default="c2", choices=sorted(TRAINING))
+    ap.add_argument("--out", default=None, help="weights file (default: the workload's file under mvr/weights)")
     ap.add_argument("--volumes", type=int, default=len(TRAIN_SEEDS))
     ap.add_argument("--patches", type=int, default=2000)
     ap.add_argument("--patch", type=int, default=64)
@@ -153,15 +161,17 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args(argv)
     seeds = tuple(range(TRAIN_SEEDS[0], TRAIN_SEEDS[0] + args.volumes))
-    net, hist = train(syn.C2, seeds, args.patches, args.patch, args.batch, args.epochs, args.lr,
+    w, out = TRAINING[args.workload]
+    out = args.out or out
+    net, hist = train(w, seeds, args.patches, args.patch, args.batch, args.epochs, args.lr,
                       args.target_iterations, args.seed)
-    m = dict(workload=syn.C2.name, seeds=list(seeds), patches=args.patches, patch=args.patch, batch=args.batch,
+    m = dict(workload=w.name, seeds=list(seeds), patches=args.patches, patch=args.patch, batch=args.batch,
              epochs=args.epochs, lr=args.lr, target_iterations=args.target_iterations, seed=args.seed,
              final_loss=float(np.mean(hist["loss"][-50:])), first_loss=float(np.mean(hist["loss"][:50])),
              median_step_ms=float(np.median(hist["step_ms"])), train_seconds=hist["train_seconds"],
              data_seconds=hist["data_seconds"])
-    save(net, args.out, m)
-    _log("saved", args.out, json.dumps(m))
+    save(net, out, m)
+    _log("saved", out, json.dumps(m))
     net.close()
 
 

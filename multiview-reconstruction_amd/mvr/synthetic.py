@@ -62,6 +62,18 @@ C3 = Workload("c3_400x352x224_4views_eb", (224, 352, 400), 4, 45, 1.8, 5.0, [0, 
                      "block_effective": 128})
 
 
+# Large combined run: 8 views of a 1024 x 1024 x 512 frame, 61^3 PSFs, Efficient-Bayesian, 40 iterations;
+# the reference runs it in 512^3 effective blocks (block size 512 + 2(K - 1) = 632).
+C5 = Workload("c5_1024x1024x512_8views_eb", (512, 1024, 1024), 8, 61, 2.0, 6.0, [0, 45, 90, 135, 180, 225, 270, 315],
+              50000, 1, 8, 2000.0, "zramp", 20, "EFFICIENT_BAYESIAN", 0.0006, 40, 5,
+              extra={"block_effective": 512})
+# Training volumes of the 8-view learned initial estimate: C5's phantom density, PSFs, noise and weights
+# on 256^3 (50000 blobs per 1024^2 x 512 voxels -> 1562 per 256^3).
+C5_TRAIN = Workload("c5_train_256cube_8views_eb", (256, 256, 256), 8, 61, 2.0, 6.0,
+                    [0, 45, 90, 135, 180, 225, 270, 315], 1562, 1, 8, 2000.0, "zramp", 20, "EFFICIENT_BAYESIAN",
+                    0.0006, 40, 6)
+
+
 def small_workload(n=24, views=3, k=7, psf_type="EFFICIENT_BAYESIAN", lam=0.0, iterations=3, weights="uniform",
                    seed_index=7, n_blobs=12, peak=None, shape=None):
     shape = shape or (n, n, n)
@@ -219,3 +231,123 @@ def make_inputs(w: Workload):
         raw = np.broadcast_to(wz[None, :, None, None], imgs.shape).astype(np.float32)
         weights = normalize_weights(raw)
     return dict(phantom=ph, imgs=imgs, weights=np.ascontiguousarray(weights), psfs=psfs)
+
+
+# ---------------------------------------------------------------- device-side generator (large workloads)
+def _blob_params(shape_zyx, n_blobs, r_min, r_max, seed):
+    """The blob centres, radii and amplitudes blob_phantom draws (same generator, same order)."""
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = shape_zyx
+    cz = rng.uniform(0, nz, n_blobs)
+    cy = rng.uniform(0, ny, n_blobs)
+    cx = rng.uniform(0, nx, n_blobs)
+    r = rng.uniform(r_min, r_max, n_blobs)
+    amp = rng.uniform(50.0, 500.0, n_blobs)
+    return cz, cy, cx, r, amp
+
+
+def blob_phantom_torch(shape_zyx, n_blobs, r_min, r_max, seed, background=10.0, device="cuda",
+                       chunk_voxels=1 << 26):
+    """blob_phantom as an FP64 torch tensor on `device`: the same blobs (each voxel within 4 sigma of a centre
+    gets amp exp(-d^2 / 2 sigma^2)), splatted in groups of equal box size; sums differ from the host loop
+    only in FP64 summation order."""
+    import torch
+    cz, cy, cx, r, amp = _blob_params(shape_zyx, n_blobs, r_min, r_max, seed)
+    nz, ny, nx = shape_zyx
+    vol = torch.full(tuple(shape_zyx), float(background), dtype=torch.float64, device=device)
+    flat = vol.view(-1)
+    ext = 4.0 * r
+    half = np.ceil(ext).astype(np.int64) + 1          # offsets -half..half around floor(centre) cover +-ext
+    dims = torch.tensor([nz, ny, nx], device=device)
+    for h in np.unique(half):
+        sel = np.nonzero(half == h)[0]
+        side = 2 * int(h) + 1
+        per = max(1, chunk_voxels // side ** 3)
+        o = torch.arange(-int(h), int(h) + 1, device=device, dtype=torch.int64)
+        for a in range(0, len(sel), per):
+            ids = sel[a:a + per]
+            c = torch.as_tensor(np.stack([cz[ids], cy[ids], cx[ids]], 1), device=device)       # [b, 3]
+            p = torch.floor(c).to(torch.int64)[:, :, None] + o[None, None, :]                   # [b, 3, side]
+            dd = (p.to(torch.float64) - c[:, :, None]) ** 2
+            inside = (p >= 0) & (p < dims[None, :, None])
+            d2 = dd[:, 0, :, None, None] + dd[:, 1, None, :, None] + dd[:, 2, None, None, :]
+            rr = torch.as_tensor(r[ids], device=device)[:, None, None, None]
+            e2 = torch.as_tensor(ext[ids] ** 2, device=device)[:, None, None, None]
+            g = torch.as_tensor(amp[ids], device=device)[:, None, None, None] * torch.exp(-d2 / (2.0 * rr * rr))
+            mask = (d2 <= e2) & inside[:, 0, :, None, None] & inside[:, 1, None, :, None] & inside[:, 2, None, None, :]
+            idx = (p[:, 0, :, None, None] * ny + p[:, 1, None, :, None]) * nx + p[:, 2, None, None, :]
+            flat.index_put_((idx[mask],), g[mask], accumulate=True)
+    return vol
+
+
+def blur_mirror_torch(vol, psf, lengths=None):
+    """convolve_mirror_f64 on a torch FP64 tensor (any device): mirror-single padding, circular convolution of
+    the given lengths (>= the padded extent per axis), 'valid' crop."""
+    import torch
+    import torch.nn.functional as F
+    k = psf.shape
+    c = [s // 2 for s in k]
+    pad = F.pad(vol[None, None], (c[2], k[2] - 1 - c[2], c[1], k[1] - 1 - c[1], c[0], k[0] - 1 - c[0]),
+                mode="reflect")[0, 0]
+    L = lengths or tuple(pad.shape)
+    f = torch.fft.rfftn(pad, s=L)
+    del pad
+    kt = torch.as_tensor(np.asarray(psf, np.float64), device=vol.device)
+    f *= torch.fft.rfftn(kt, s=L)
+    full = torch.fft.irfftn(f, s=L)
+    del f
+    return full[k[0] - 1:k[0] - 1 + vol.shape[0], k[1] - 1:k[1] - 1 + vol.shape[1],
+                k[2] - 1:k[2] - 1 + vol.shape[2]].contiguous()
+
+
+def _smooth_length(n):
+    """Smallest 2^a 3^b >= n (FFT lengths with stock kernels in every FFT library)."""
+    best = 1 << (int(n) - 1).bit_length()
+    p3 = 1
+    while p3 < 2 * n:
+        m = p3
+        while m < n:
+            m *= 2
+        best = min(best, m)
+        p3 *= 3
+    return best
+
+
+def make_inputs_device(w: Workload, device="cuda", log=None):
+    """make_inputs for volumes too large for the host generator, on `device` (torch): the same phantom, PSFs,
+    FP64 mirror blur, peak scaling and clamp; the Poisson noise comes from torch's generator seeded
+    2000 + 100 c + v.  Returns dict(imgs=[V float32 tensors], weights=[V tensors, one shared z profile when
+    the views share it], psfs=[V, k, k, k] numpy)."""
+    import torch
+    V = w.n_views
+    psfs = np.stack([gaussian_psf(w.psf_size, w.sigma_xy, w.sigma_z, a) for a in w.angles_deg[:V]])
+    ph = blob_phantom_torch(w.shape_zyx, w.n_blobs, w.r_min, w.r_max, 1000 + w.seed_index, device=device)
+    if log:
+        log(f"phantom {tuple(w.shape_zyx)} with {w.n_blobs} blobs on {device}")
+    L = tuple(_smooth_length(n + w.psf_size - 1) for n in w.shape_zyx)
+    imgs = []
+    for v in range(V):
+        b = blur_mirror_torch(ph, psfs[v], L)
+        if w.peak is not None:
+            b *= w.peak / float(b.max())
+        g = torch.Generator(device=device)
+        g.manual_seed(2000 + 100 * w.seed_index + v)
+        noisy = torch.poisson(b.clamp_min_(0.0), generator=g)
+        del b
+        imgs.append(noisy.clamp_min_(1.0).to(torch.float32))
+        del noisy
+        if log:
+            log(f"view {v + 1}/{V} blurred and noised")
+    del ph
+    if w.weights == "uniform":
+        wt = torch.ones(tuple(w.shape_zyx), dtype=torch.float32, device=device)
+        weights = [wt] * V
+    elif w.weights == "zramp":
+        wz = blend_weight_1d(w.shape_zyx[0], w.blend_range)
+        prof = normalize_weights(np.broadcast_to(wz[None, :, None, None], (V, len(wz), 1, 1)).astype(np.float32))
+        assert all(np.array_equal(prof[0], prof[v]) for v in range(V))
+        wt = torch.as_tensor(prof[0], device=device).expand(tuple(w.shape_zyx)).contiguous()
+        weights = [wt] * V
+    else:
+        raise ValueError(f"device generator: weights '{w.weights}' not supported")
+    return dict(imgs=imgs, weights=weights, psfs=psfs)
