@@ -189,6 +189,7 @@ int mvr_decon_run_until(mvr_decon* d, int maxIterations, double tol, int* iterat
  * convolutions, ReLU; BF16 on MFMA with FP32 accumulation) and its training have no counterpart in
  * the reference.  Its inputs are the V weighted views and their mean; its output is psi. */
 typedef struct mvr_unet mvr_unet;
+/* views: 1..8 */
 int mvr_unet_create(mvr_unet** out, int dev, int views, unsigned long long seed);
 /* number of FP32 parameters; get/set use one flat array in layer order (per conv layer the weights
  * [27][cout][cin] then the bias [cout]; last the 1x1 head: 16 weights and its bias) */

@@ -24,7 +24,7 @@ size_t UNet::wcount(int l) const { return (size_t)27 * kCout[l] * kCin[l]; }
 
 int UNet::create(int dev_, int V_, unsigned long long seed) {
   dev = dev_;
-  UCHECK(V_ >= 1 && V_ <= 7, "the network takes 1..7 views");
+  UCHECK(V_ >= 1 && V_ <= kNetMaxViews, "the network takes 1..8 views");
   V = V_;
   MVR_HIP(hipSetDevice(dev));
   MVR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));

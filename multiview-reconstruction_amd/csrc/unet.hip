@@ -259,7 +259,7 @@ __global__ void k_pack_input(InputArgs a) {
 #pragma unroll
     for (int c = 0; c < 16; ++c) val[c] = 0.f;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {  // InputArgs holds at most 8 views
+    for (int c = 0; c < kNetMaxViews; ++c) {  // unrolled over the packer's view slots
       if (c < a.V) {
         const float wv = a.w[c][gi];
         val[c] = a.img[c][gi] * wv * a.inv_scale;

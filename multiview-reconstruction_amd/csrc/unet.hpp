@@ -37,9 +37,13 @@ struct WgradArgs {
   float* partial;        // [parts][taps][cout][cin]
 };
 
+// views the network takes: V view channels + their weighted mean fit the 16 input channels; the input
+// packer holds up to 8 view pointers
+constexpr int kNetMaxViews = 8;
+
 struct InputArgs {
-  const float* img[8];
-  const float* w[8];
+  const float* img[kNetMaxViews];
+  const float* w[kNetMaxViews];
   int V;
   int N3[3];             // dims of the source volumes (x, y, z)
   int x0, y0, z0;        // origin of the sample region

@@ -144,7 +144,7 @@ def test_unet_param_roundtrip():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("shape,V", [((16, 16, 16), 2), ((20, 24, 12), 6), ((18, 17, 15), 3), ((8, 12, 72), 2),
-                                     ((12, 8, 40), 3)])
+                                     ((12, 8, 40), 3), ((16, 12, 20), 8)])
 def test_unet_forward_matches_oracle(shape, V):
     rng = np.random.default_rng(sum(shape) + V)
     imgs = [rng.uniform(0.0, 3.0, shape).astype(np.float32) for _ in range(V)]
