@@ -152,6 +152,17 @@ int mvr_transform_view(int dev, const float* img, const int imgXYZ[3], const dou
  * on the input interval [0, imgXYZ - 1]; border and blending per dimension (after FusionTools.adjustBlending) */
 int mvr_transform_blending(int dev, const int imgXYZ[3], const float border[3], const float blending[3],
                            const double affine[12], const long long bbMin[3], const int bbXYZ[3], float* out);
+/* ProcessInputImages.fuseGroups for one group of numViews views (deconvolution/util/ProcessInputImages.java:
+ * 279-399), in one pass without per-view intermediates: view j (imgXYZ[3j..], registration affines[12j..]) is
+ * intensity-adjusted (fusion/intensityadjust/IntensityAdjuster.java:43-54: (float)(v * m00 + m01) with
+ * intensityAdjust[2j..2j+1]; NULL = none) and resampled like mvr_transform_view (min 1, outside 0, linear);
+ * imgOut = FusedRandomAccess AVG (fusion/transformed/FusedRandomAccess.java:67-88) weighted by the view's fusion
+ * blending, weightOut = CombineWeightsSumRandomAccess (weightcombination/CombineWeightsSumRandomAccess.java:40-50)
+ * of the views' deconvolution blending.  fusionBlending / deconBlending hold per view {border xyz, range xyz}
+ * (after FusionTools.adjustBlending); NULL = the constant weight 1 of the reference's no-blending branch. */
+int mvr_fuse_group(int dev, int numViews, const float* const* imgs, const int* imgXYZ, const double* affines,
+                   const double* intensityAdjust, const float* fusionBlending, const float* deconBlending,
+                   const long long bbMin[3], const int bbXYZ[3], float* imgOut, float* weightOut);
 /* NormalizingRandomAccess.get for all views (deconvolution/normalization/NormalizingRandomAccess.java:76-108,
  * 183-214): weights[v] (nXYZ each) are replaced in place */
 int mvr_normalize_weights(int dev, float* const* weights, int numViews, const int nXYZ[3], double osemSpeedup,

@@ -367,6 +367,9 @@ int ensure_blend_lut(int dev) {
 
 struct DevBuf {
   void* p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { if (p) (void)hipFree(p); }
 };
 
@@ -424,6 +427,64 @@ int mvr_transform_blending(int dev, const int imgXYZ[3], const float border[3], 
     return -1;
   }
   MVR_HIP(hipMemcpy(out, dout.p, no * 4, hipMemcpyDefault));
+  return 0;
+}
+
+int mvr_fuse_group(int dev, int numViews, const float* const* imgs, const int* imgXYZ, const double* affines,
+                   const double* intensityAdjust, const float* fusionBlending, const float* deconBlending,
+                   const long long bbMin[3], const int bbXYZ[3], float* imgOut, float* weightOut) {
+  NULLCHECK(imgs);
+  NULLCHECK(imgXYZ);
+  NULLCHECK(affines);
+  NULLCHECK(bbMin);
+  NULLCHECK(bbXYZ);
+  NULLCHECK(imgOut);
+  NULLCHECK(weightOut);
+  if (numViews < 1) { set_error("a group needs at least one view"); return -1; }
+  for (int d = 0; d < 3; ++d)
+    if (bbXYZ[d] < 1) { set_error("empty bounding box"); return -1; }
+  std::vector<FuseView> fv(numViews);
+  for (int j = 0; j < numViews; ++j) {
+    NULLCHECK(imgs[j]);
+    FuseView& v = fv[j];
+    for (int d = 0; d < 3; ++d) {
+      v.in[d] = imgXYZ[3 * j + d];
+      if (v.in[d] < 1) { set_error("empty view image"); return -1; }
+    }
+    if (invert_affine(affines + 12 * j, v.inv)) return -1;
+    v.adjust = intensityAdjust ? 1 : 0;
+    v.adjA = intensityAdjust ? intensityAdjust[2 * j] : 1.0;
+    v.adjB = intensityAdjust ? intensityAdjust[2 * j + 1] : 0.0;
+    v.fusionBlend = fusionBlending ? 1 : 0;
+    v.deconBlend = deconBlending ? 1 : 0;
+    for (int d = 0; d < 3; ++d) {
+      v.fBorder[d] = fusionBlending ? fusionBlending[6 * j + d] : 0.f;
+      v.fRange[d] = fusionBlending ? fusionBlending[6 * j + 3 + d] : 0.f;
+      v.dBorder[d] = deconBlending ? deconBlending[6 * j + d] : 0.f;
+      v.dRange[d] = deconBlending ? deconBlending[6 * j + 3 + d] : 0.f;
+    }
+  }
+  MVR_HIP(hipSetDevice(dev));
+  if (ensure_blend_lut(dev)) return -1;
+  std::vector<DevBuf> dimg(numViews);
+  for (int j = 0; j < numViews; ++j) {
+    const size_t ni = (size_t)fv[j].in[0] * fv[j].in[1] * fv[j].in[2];
+    MVR_HIP(hipMalloc(&dimg[j].p, ni * 4));
+    MVR_HIP(hipMemcpy(dimg[j].p, imgs[j], ni * 4, hipMemcpyDefault));
+    fv[j].img = (const float*)dimg[j].p;
+  }
+  const size_t no = (size_t)bbXYZ[0] * bbXYZ[1] * bbXYZ[2];
+  DevBuf dviews, dout, dw;
+  MVR_HIP(hipMalloc(&dviews.p, sizeof(FuseView) * numViews));
+  MVR_HIP(hipMemcpy(dviews.p, fv.data(), sizeof(FuseView) * numViews, hipMemcpyHostToDevice));
+  MVR_HIP(hipMalloc(&dout.p, no * 4));
+  MVR_HIP(hipMalloc(&dw.p, no * 4));
+  if (launch_fuse_group((const FuseView*)dviews.p, numViews, bbMin, bbXYZ, (float*)dout.p, (float*)dw.p, nullptr)) {
+    set_error("group fusion kernel launch failed");
+    return -1;
+  }
+  MVR_HIP(hipMemcpy(imgOut, dout.p, no * 4, hipMemcpyDefault));
+  MVR_HIP(hipMemcpy(weightOut, dw.p, no * 4, hipMemcpyDefault));
   return 0;
 }
 

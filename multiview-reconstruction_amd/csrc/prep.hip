@@ -28,50 +28,53 @@ __device__ __forceinline__ void apply_inv(const Affine3& a, double sx, double sy
   tz = a.m[8] * sx + a.m[9] * sy + a.m[10] * sz + a.m[11];
 }
 
+// input pixel value, optionally intensity-adjusted (IntensityAdjuster.convert: (float)(v * m00 + m01))
+__device__ __forceinline__ double pix(const float* p, bool adjust, double a, double b) {
+  return adjust ? (double)(float)((double)*p * a + b) : (double)*p;
+}
+
+// TransformedInputRandomAccess.get at input location t: strictly inside (0, n - 1) in every dimension ->
+// linear or nearest interpolation clamped to >= minValue; otherwise the outside value
+__device__ __forceinline__ float sample_view(const float* __restrict__ img, int nx, int ny, int nz, double tx,
+                                             double ty, double tz, int hasMin, float minValue, float outside,
+                                             int linear, bool adjust = false, double a = 1.0, double b = 0.0) {
+  if (!(tx > 0.0 && ty > 0.0 && tz > 0.0 && tx < nx - 1 && ty < ny - 1 && tz < nz - 1)) return outside;
+  double val;
+  if (linear) {
+    const int x0 = (int)floor(tx), y0 = (int)floor(ty), z0 = (int)floor(tz);
+    const double fx = tx - x0, fy = ty - y0, fz = tz - z0;
+    const float* p = img + ((size_t)z0 * ny + y0) * nx + x0;
+    const size_t sy = nx, sz = (size_t)nx * ny;
+    const double c00 = (1 - fx) * pix(p, adjust, a, b) + fx * pix(p + 1, adjust, a, b);
+    const double c10 = (1 - fx) * pix(p + sy, adjust, a, b) + fx * pix(p + sy + 1, adjust, a, b);
+    const double c01 = (1 - fx) * pix(p + sz, adjust, a, b) + fx * pix(p + sz + 1, adjust, a, b);
+    const double c11 = (1 - fx) * pix(p + sz + sy, adjust, a, b) + fx * pix(p + sz + sy + 1, adjust, a, b);
+    val = (1 - fz) * ((1 - fy) * c00 + fy * c10) + fz * ((1 - fy) * c01 + fy * c11);
+  } else {
+    // nearest neighbour: the closest pixel, ties rounded up (floor(t + 0.5))
+    const int xi = (int)floor(tx + 0.5), yi = (int)floor(ty + 0.5), zi = (int)floor(tz + 0.5);
+    val = pix(img + ((size_t)zi * ny + yi) * nx + xi, adjust, a, b);
+  }
+  float v = (float)val;
+  if (hasMin) v = fmaxf(minValue, v);
+  return v;
+}
+
 __global__ void k_transform_view(const float* __restrict__ img, int3 in, Affine3 inv, long3 off, int3 outDim,
                                  int hasMin, float minValue, float outside, int linear, float* __restrict__ out) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
   if (x >= outDim.x) return;
   double tx, ty, tz;
   apply_inv(inv, (double)(x + off.x), (double)(y + off.y), (double)(z + off.z), tx, ty, tz);
-  float v = outside;
-  // strictly inside (0, n - 1) in every dimension
-  if (tx > 0.0 && ty > 0.0 && tz > 0.0 && tx < in.x - 1 && ty < in.y - 1 && tz < in.z - 1) {
-    double val;
-    if (linear) {
-      const int x0 = (int)floor(tx), y0 = (int)floor(ty), z0 = (int)floor(tz);
-      const double fx = tx - x0, fy = ty - y0, fz = tz - z0;
-      const float* p = img + ((size_t)z0 * in.y + y0) * in.x + x0;
-      const size_t sy = in.x, sz = (size_t)in.x * in.y;
-      const double c00 = (1 - fx) * p[0] + fx * p[1];
-      const double c10 = (1 - fx) * p[sy] + fx * p[sy + 1];
-      const double c01 = (1 - fx) * p[sz] + fx * p[sz + 1];
-      const double c11 = (1 - fx) * p[sz + sy] + fx * p[sz + sy + 1];
-      val = (1 - fz) * ((1 - fy) * c00 + fy * c10) + fz * ((1 - fy) * c01 + fy * c11);
-    } else {
-      // nearest neighbour: the closest pixel, ties rounded up (floor(t + 0.5))
-      const int xi = (int)floor(tx + 0.5), yi = (int)floor(ty + 0.5), zi = (int)floor(tz + 0.5);
-      val = img[((size_t)zi * in.y + yi) * in.x + xi];
-    }
-    v = (float)val;
-    if (hasMin) v = fmaxf(minValue, v);
-  }
-  out[((size_t)z * outDim.y + y) * outDim.x + x] = v;
+  out[((size_t)z * outDim.y + y) * outDim.x + x] =
+      sample_view(img, in.x, in.y, in.z, tx, ty, tz, hasMin, minValue, outside, linear);
 }
 
 __constant__ double c_blendLut[1001];
 
-// BlendingRealRandomAccess.computeWeight at the (float) input location, interval [0, dimMinus1]
-__global__ void k_transform_blending(int3 dimMinus1, float3 border, float3 blending, Affine3 inv, long3 off,
-                                     int3 outDim, float* __restrict__ out) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
-  if (x >= outDim.x) return;
-  double tx, ty, tz;
-  apply_inv(inv, (double)(x + off.x), (double)(y + off.y), (double)(z + off.z), tx, ty, tz);
-  const float l[3] = {(float)tx, (float)ty, (float)tz};
-  const int dm[3] = {dimMinus1.x, dimMinus1.y, dimMinus1.z};
-  const float bo[3] = {border.x, border.y, border.z};
-  const float bl[3] = {blending.x, blending.y, blending.z};
+// BlendingRealRandomAccess.computeWeight at the (float) input location l, interval [0, dm]
+__device__ __forceinline__ float blend_weight(const float l[3], const int dm[3], const float bo[3],
+                                              const float bl[3]) {
   float dist[3];
   float w = 1.f;
   bool zero = false;
@@ -89,7 +92,50 @@ __global__ void k_transform_blending(int3 dimMinus1, float3 border, float3 blend
   } else {
     w = 0.f;
   }
-  out[((size_t)z * outDim.y + y) * outDim.x + x] = w;
+  return w;
+}
+
+__global__ void k_transform_blending(int3 dimMinus1, float3 border, float3 blending, Affine3 inv, long3 off,
+                                     int3 outDim, float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+  if (x >= outDim.x) return;
+  double tx, ty, tz;
+  apply_inv(inv, (double)(x + off.x), (double)(y + off.y), (double)(z + off.z), tx, ty, tz);
+  const float l[3] = {(float)tx, (float)ty, (float)tz};
+  const int dm[3] = {dimMinus1.x, dimMinus1.y, dimMinus1.z};
+  const float bo[3] = {border.x, border.y, border.z};
+  const float bl[3] = {blending.x, blending.y, blending.z};
+  out[((size_t)z * outDim.y + y) * outDim.x + x] = blend_weight(l, dm, bo, bl);
+}
+
+// ProcessInputImages.fuseGroups (deconvolution/util/ProcessInputImages.java:309-396) for one group, fused in
+// one pass: each view is resampled (TransformView, minValueImg 1, outside 0, linear) after its intensity
+// adjustment; img = FusedRandomAccess AVG (fusion/transformed/FusedRandomAccess.java:67-88: views whose fusion
+// weight is 0 are skipped, sum I w / sum w in double, 0 when no weight) and weight = CombineWeightsSum
+// (weightcombination/CombineWeightsSumRandomAccess.java:40-50: (float) of the double sum of decon weights)
+__global__ void k_fuse_group(const FuseView* __restrict__ views, int V, long3 off, int3 outDim,
+                             float* __restrict__ img, float* __restrict__ weight) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+  if (x >= outDim.x) return;
+  const double sx = (double)(x + off.x), sy = (double)(y + off.y), sz = (double)(z + off.z);
+  double sumI = 0.0, sumW = 0.0, sumD = 0.0;
+  for (int j = 0; j < V; ++j) {
+    const FuseView& v = views[j];
+    double tx, ty, tz;
+    apply_inv(v.inv, sx, sy, sz, tx, ty, tz);
+    const float l[3] = {(float)tx, (float)ty, (float)tz};
+    const int dm[3] = {v.in[0] - 1, v.in[1] - 1, v.in[2] - 1};
+    sumD += v.deconBlend ? (double)blend_weight(l, dm, v.dBorder, v.dRange) : 1.0;
+    const float wf = v.fusionBlend ? blend_weight(l, dm, v.fBorder, v.fRange) : 1.f;
+    if (wf == 0.f) continue;
+    const float iv = sample_view(v.img, v.in[0], v.in[1], v.in[2], tx, ty, tz, 1, 1.0f, 0.0f, 1, v.adjust != 0,
+                                 v.adjA, v.adjB);
+    sumI += (double)iv * (double)wf;
+    sumW += (double)wf;
+  }
+  const size_t o = ((size_t)z * outDim.y + y) * outDim.x + x;
+  img[o] = sumW > 0.0 ? (float)(sumI / sumW) : 0.f;
+  weight[o] = (float)sumD;
 }
 
 // PSFExtraction.transform (psf/PSFExtraction.java:411-451): out[p] = linear interpolation of the zero-extended
@@ -175,6 +221,14 @@ int launch_transform_blending(const int inXYZ[3], const float border[3], const f
   hipLaunchKernelGGL(k_transform_blending, grid, dim3(256), 0, s, make_int3(inXYZ[0] - 1, inXYZ[1] - 1, inXYZ[2] - 1),
                      make_float3(border[0], border[1], border[2]), make_float3(blending[0], blending[1], blending[2]),
                      inv, make_long3(off[0], off[1], off[2]), make_int3(outXYZ[0], outXYZ[1], outXYZ[2]), out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_fuse_group(const FuseView* views, int V, const long long off[3], const int outXYZ[3], float* img,
+                      float* weight, hipStream_t s) {
+  const dim3 grid((outXYZ[0] + 255) / 256, outXYZ[1], outXYZ[2]);
+  hipLaunchKernelGGL(k_fuse_group, grid, dim3(256), 0, s, views, V, make_long3(off[0], off[1], off[2]),
+                     make_int3(outXYZ[0], outXYZ[1], outXYZ[2]), img, weight);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

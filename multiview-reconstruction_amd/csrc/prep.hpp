@@ -21,6 +21,21 @@ int launch_transform_psf(const float* psf, const int inXYZ[3], const Affine3& in
                          const int outXYZ[3], float* out, hipStream_t s);
 int launch_transform_blending(const int inXYZ[3], const float border[3], const float blending[3], const Affine3& inv,
                               const long long off[3], const int outXYZ[3], float* out, hipStream_t s);
+// one view of a fused group: its image, inverse registration, optional intensity adjustment and blending
+struct FuseView {
+  const float* img;
+  int in[3];                 // image dims x, y, z
+  Affine3 inv;               // world -> input pixel
+  double adjA, adjB;         // intensity v -> (float)(v * adjA + adjB) when adjust != 0
+  int adjust;
+  int fusionBlend, deconBlend;  // 0: constant weight 1
+  float fBorder[3], fRange[3], dBorder[3], dRange[3];
+};
+
+// ProcessInputImages.fuseGroups for one group: per output voxel, AVG fusion of the resampled views weighted
+// by their fusion blending, and the sum of their deconvolution blending weights (views: device array)
+int launch_fuse_group(const FuseView* views, int V, const long long off[3], const int outXYZ[3], float* img,
+                      float* weight, hipStream_t s);
 int launch_normalize_weights(float* const* w, int V, size_t n, double osem, int smooth, float maxDiffRange,
                              float scalingRange, hipStream_t s);
 

@@ -89,6 +89,8 @@ def lib():
                                          ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, vp]
         L.mvr_transform_blending.argtypes = [ctypes.c_int, _i32p, _f32p, _f32p, _f64p,
                                              ctypes.POINTER(ctypes.c_longlong), _i32p, vp]
+        L.mvr_fuse_group.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp), _i32p, _f64p, _f64p, _f32p,
+                                     _f32p, ctypes.POINTER(ctypes.c_longlong), _i32p, vp, vp]
         L.mvr_normalize_weights.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int, _i32p, ctypes.c_double,
                                             ctypes.c_int, ctypes.c_float, ctypes.c_float]
         L.mvr_transform_psf_size.argtypes = [_i32p, _f64p, _i32p]
@@ -467,6 +469,34 @@ def transform_blending(img_shape_zyx, border_xyz, blending_xyz, affine, bb_min_x
     bl = (ctypes.c_float * 3)(*blending_xyz)
     _check(lib().mvr_transform_blending(dev, _dims(img_shape_zyx), bo, bl, ap, bb, _dims(out.shape), _ptr(out)))
     return out
+
+
+def fuse_group(images, affines, bb_min_xyz, bb_shape_zyx, intensity_adjust=None, fusion_blending=None,
+               decon_blending=None, dev=0):
+    """ProcessInputImages.fuseGroups for one group: (fused image, summed decon weights), z, y, x arrays.
+
+    images / affines: the group's views and their registrations; intensity_adjust: per view (m00, m01) or None;
+    fusion_blending / decon_blending: per view (border_xyz, range_xyz) or None (constant weight 1)."""
+    imgs = [_f32(a) for a in images]
+    n = len(imgs)
+    dims = np.ascontiguousarray([[a.shape[2], a.shape[1], a.shape[0]] for a in imgs], np.int32)
+    aff = np.ascontiguousarray([np.asarray(a, np.float64).reshape(12) for a in affines])
+
+    def arr(x, dtype, k):
+        if x is None:
+            return None, None
+        a = np.ascontiguousarray(np.asarray(x, dtype).reshape(n, k))
+        return a.ctypes.data_as(_f64p if dtype == np.float64 else _f32p), a
+
+    adj_p, adj = arr(intensity_adjust, np.float64, 2)
+    fb_p, fb = arr(fusion_blending, np.float32, 6)
+    db_p, db = arr(decon_blending, np.float32, 6)
+    out = np.empty(tuple(bb_shape_zyx), np.float32)
+    wout = np.empty_like(out)
+    bb = (ctypes.c_longlong * 3)(*[int(v) for v in bb_min_xyz])
+    _check(lib().mvr_fuse_group(dev, n, _ptr_array(imgs), dims.ctypes.data_as(_i32p), aff.ctypes.data_as(_f64p),
+                                adj_p, fb_p, db_p, bb, _dims(out.shape), _ptr(out), _ptr(wout)))
+    return out, wout
 
 
 def normalize_weights(weights, osem_speedup=1.0, smooth=False, max_diff_range=0.1, scaling_range=0.05, dev=0):

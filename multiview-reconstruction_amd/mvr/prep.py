@@ -1,10 +1,12 @@
 """Input and PSF preparation for the deconvolution (ProcessInputImages / PSFPreparation), on the GPU.
 
-Mirrors src/main/java/net/preibisch/mvrecon/process/deconvolution/util/ProcessInputImages.java:279-399 for
-groups of one view each: every view is resampled into the deconvolution bounding box
+Mirrors src/main/java/net/preibisch/mvrecon/process/deconvolution/util/ProcessInputImages.java:279-399.
+Groups of one view (``prepare_views``): every view is resampled into the deconvolution bounding box
 (``hip.transform_view``, minValueImg 1 inside, 0 outside), its blending weights are transformed
 (``hip.transform_blending``) and all weights are normalised across views (``hip.normalize_weights``,
-NormalizingRandomAccess).  PSFs follow deconvolution/util/PSFPreparation.java:41-89: each is normalised and
+NormalizingRandomAccess).  Groups of several views (``prepare_groups``): each group is fused in one GPU pass
+(``hip.fuse_group``: intensity adjustment, resampling, AVG fusion with the fusion blending, summed decon
+blending weights), then the group weights are normalised across groups.  PSFs follow deconvolution/util/PSFPreparation.java:41-89: each is normalised and
 resampled by its view's affine (``hip.transform_psf``), a group's PSFs are averaged over their smallest
 common size (PSFCombination.computeAverageImage, psf/PSFCombination.java:74-135) and all are brought to one
 size (PSFCombination.makeSameSize, :182-221).  The resampling runs in the HIP library; the array
@@ -44,6 +46,30 @@ def prepare_views(images, affines, bb_min_xyz, bb_shape_zyx, blending_range_xyz=
     return imgs, ws
 
 
+def prepare_groups(groups, bb_min_xyz, bb_shape_zyx, blending_range_fusion=None, blending_border_fusion=None,
+                   blending_range_decon=None, blending_border_decon=None, osem_speedup=1.0, smooth_blending=False,
+                   max_diff_range=0.1, scaling_range=0.05, dev=0):
+    """ProcessInputImages.fuseGroups + weight normalisation: (imgs [G, z, y, x], weights [G, z, y, x]).
+
+    groups: list of groups, each a list of views {"img": array z, y, x, "affine": 3 x 4 input -> world,
+    optional "intensity": (m00, m01)}.  Blending ranges / borders (x, y, z, after FusionTools.adjustBlending)
+    apply to every view; None keeps the constant weight 1 of the reference's no-blending branch."""
+    G = len(groups)
+    imgs = np.empty((G,) + tuple(bb_shape_zyx), np.float32)
+    ws = np.empty_like(imgs)
+    for g, views in enumerate(groups):
+        n = len(views)
+        adj = None
+        if any("intensity" in v for v in views):
+            adj = [v.get("intensity", (1.0, 0.0)) for v in views]
+        fb = None if blending_range_fusion is None else [list(blending_border_fusion) + list(blending_range_fusion)] * n
+        db = None if blending_range_decon is None else [list(blending_border_decon) + list(blending_range_decon)] * n
+        imgs[g], ws[g] = hip.fuse_group([v["img"] for v in views], [v["affine"] for v in views], bb_min_xyz,
+                                        bb_shape_zyx, adj, fb, db, dev=dev)
+    ws = hip.normalize_weights(ws, osem_speedup, smooth_blending, max_diff_range, scaling_range, dev=dev)
+    return imgs, ws
+
+
 def average_psf(psfs):
     """PSFCombination.computeAverageImage with the minimal common size (max = false): PSFs centred on
     dim / 2 summed into the smallest size, then divided by their count."""
@@ -73,8 +99,14 @@ def make_same_size(psf, size_zyx):
 
 def prepare_psfs(raw_psfs, affines, same_size_for_all=True, dev=0):
     """One PSF per view (groups of one view): normalised, transformed; optionally all padded to one size."""
-    out = [hip.transform_psf(p, a, dev=dev) for p, a in zip(raw_psfs, affines)]
-    out = [average_psf([p]) for p in out]
+    return prepare_group_psfs([[p] for p in raw_psfs], [[a] for a in affines], same_size_for_all, dev)
+
+
+def prepare_group_psfs(group_psfs, group_affines, same_size_for_all=True, dev=0):
+    """PSFPreparation.loadGroupTransformPSFs: per group, every view's PSF normalised and transformed by its
+    registration, averaged over the group's smallest common size; optionally all groups padded to one size."""
+    out = [average_psf([hip.transform_psf(p, a, dev=dev) for p, a in zip(ps, afs)])
+           for ps, afs in zip(group_psfs, group_affines)]
     if same_size_for_all:
         size = tuple(max(p.shape[d] for p in out) for d in range(3))
         out = [make_same_size(p, size) for p in out]

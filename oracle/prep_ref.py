@@ -9,7 +9,11 @@ Reference (src/main/java/net/preibisch/mvrecon/process/):
   transform_blending  fusion/transformed/TransformWeight.java:82-134, weights/BlendingRealRandomAccess.java:46-127
                       (float arithmetic, the 1001-entry cosine table);
   normalize_weights   deconvolution/normalization/NormalizingRandomAccess.java:76-108, 183-214;
-  transform_psf       psf/PSFExtraction.java:182-193, 367-470.
+  transform_psf       psf/PSFExtraction.java:182-193, 367-470;
+  fuse_group          deconvolution/util/ProcessInputImages.java:279-399 (one group): views intensity-adjusted
+                      (fusion/intensityadjust/IntensityAdjuster.java:43-54) and resampled, AVG fusion
+                      (fusion/transformed/FusedRandomAccess.java:67-88) and summed decon weights
+                      (weightcombination/CombineWeightsSumRandomAccess.java:40-50).
 Pins (tests/test_prep.py): identity and integer-translation transforms reproduce the input; trilinear
 interpolation reproduces affine functions exactly; the blending weight with the identity transform is the
 product of the per-axis weights; the normalisation's closed forms and special cases; the PSF transform
@@ -84,8 +88,9 @@ def blending_weight(loc_xyz, dims_xyz, border_xyz, blending_xyz):
         dist.append(t)
         zero |= t <= 0
     for d in range(3):
-        rel = (dist[d] / np.float32(blending_xyz[d])).astype(np.float32)
-        idx = np.clip((rel.astype(np.float64) * 1000.0 + 0.5).astype(np.int64), 0, 1000)
+        with np.errstate(divide="ignore", invalid="ignore"):   # blending range 0: rel = inf, weight kept
+            rel = (dist[d] / np.float32(blending_xyz[d])).astype(np.float32)
+        idx = np.clip((np.where((rel < 1) & np.isfinite(rel), rel, 0).astype(np.float64) * 1000.0 + 0.5).astype(np.int64), 0, 1000)
         w = np.where(rel < 1, (w.astype(np.float64) * lut[idx]).astype(np.float32), w)
     return np.where(zero, np.float32(0), w).astype(np.float32)
 
@@ -95,6 +100,35 @@ def transform_blending(img_shape_zyx, border_xyz, blending_xyz, affine, bb_min_x
     dims = (img_shape_zyx[2], img_shape_zyx[1], img_shape_zyx[0])
     return blending_weight((tx.astype(np.float32), ty.astype(np.float32), tz.astype(np.float32)), dims, border_xyz,
                            blending_xyz)
+
+
+def fuse_group(images, affines, bb_min_xyz, bb_shape_zyx, intensity_adjust=None, fusion_blending=None,
+               decon_blending=None):
+    """(fused image, summed decon weights) of one group; blending args per view (border_xyz, range_xyz)."""
+    sumI = np.zeros(tuple(bb_shape_zyx))
+    sumW = np.zeros(tuple(bb_shape_zyx))
+    sumD = np.zeros(tuple(bb_shape_zyx))
+    for j, (img, a) in enumerate(zip(images, affines)):
+        img = np.asarray(img, np.float32)
+        if intensity_adjust is not None:
+            m00, m01 = intensity_adjust[j]
+            img = (img.astype(np.float64) * m00 + m01).astype(np.float32)
+        v = transform_view(img, a, bb_min_xyz, bb_shape_zyx, 1.0, True, 0.0, 1).astype(np.float64)
+        if decon_blending is None:
+            sumD += 1.0
+        else:
+            b, r = decon_blending[j]
+            sumD += transform_blending(img.shape, b, r, a, bb_min_xyz, bb_shape_zyx).astype(np.float64)
+        if fusion_blending is None:
+            wf = np.ones(tuple(bb_shape_zyx))
+        else:
+            b, r = fusion_blending[j]
+            wf = transform_blending(img.shape, b, r, a, bb_min_xyz, bb_shape_zyx).astype(np.float64)
+        use = wf != 0
+        sumI[use] += v[use] * wf[use]
+        sumW[use] += wf[use]
+    fused = np.where(sumW > 0, sumI / np.where(sumW > 0, sumW, 1.0), 0.0)
+    return fused.astype(np.float32), sumD.astype(np.float32)
 
 
 def normalize_weights(weights, osem_speedup=1.0, smooth=False, max_diff_range=0.1, scaling_range=0.05):

@@ -96,6 +96,52 @@ def test_normalisation_closed_forms():
 
 
 # ---------------------------------------------------------------- HIP kernels (GPU)
+def test_fuse_group_single_view_is_the_transformed_view():
+    img = RNG.uniform(2, 60, (12, 14, 16)).astype(np.float32)
+    a = affine(rot(2, 25), [1.5, -2.0, 0.5])
+    bb_min, bb_shape = (-3, -3, -2), (16, 20, 22)
+    fused, w = P.fuse_group([img], [a], bb_min, bb_shape)
+    np.testing.assert_array_equal(fused, P.transform_view(img, a, bb_min, bb_shape, 1.0, True, 0.0, 1))
+    np.testing.assert_array_equal(w, np.ones(bb_shape, np.float32))
+    fused, w = P.fuse_group([img], [a], bb_min, bb_shape, decon_blending=[((1, 1, 0), (4, 4, 3))])
+    np.testing.assert_array_equal(w, P.transform_blending(img.shape, (1, 1, 0), (4, 4, 3), a, bb_min, bb_shape))
+
+
+def test_fuse_group_duplicate_view_and_weight_sum():
+    img = RNG.uniform(2, 60, (12, 14, 16)).astype(np.float32)
+    a = affine(rot(0, 15), [0.5, 1.0, -1.0])
+    bb_min, bb_shape = (-2, -2, -2), (16, 18, 20)
+    blend = [((1, 1, 1), (5, 5, 3))]
+    one, w1 = P.fuse_group([img], [a], bb_min, bb_shape, fusion_blending=blend, decon_blending=blend)
+    two, w2 = P.fuse_group([img, img], [a, a], bb_min, bb_shape, fusion_blending=blend * 2, decon_blending=blend * 2)
+    np.testing.assert_array_equal(two, one)          # (v w + v w) / 2w = v
+    np.testing.assert_array_equal(w2, 2 * w1)        # the summed decon weights
+
+
+def test_fuse_group_average_closed_forms():
+    # constant views c1 (box 0..9) and c2 shifted by +4 in x; no fusion blending = weight 1 everywhere, so
+    # outside a view its 0 joins the average (the reference's ConstantRandomAccessible branch)
+    c1, c2 = np.float32(10.0), np.float32(30.0)
+    img1 = np.full((10, 10, 10), c1, np.float32)
+    img2 = np.full((10, 10, 10), c2, np.float32)
+    a1, a2 = IDENT, affine(np.eye(3), [4.0, 0.0, 0.0])
+    fused, w = P.fuse_group([img1, img2], [a1, a2], (0, 0, 0), (10, 10, 14))
+    assert fused[5, 5, 6] == np.float32((10.0 + 30.0) / 2)   # strictly inside both
+    assert fused[5, 5, 11] == np.float32(30.0 / 2)           # inside view 2 only
+    assert fused[5, 5, 0] == 0.0                             # on view 1's face (not strictly inside) and outside 2
+    np.testing.assert_array_equal(w, np.full((10, 10, 14), 2.0, np.float32))
+    # with fusion blending a view outside its image has weight 0 and is skipped
+    fb = [((0, 0, 0), (0, 0, 0))] * 2
+    fused, _ = P.fuse_group([img1, img2], [a1, a2], (0, 0, 0), (10, 10, 14), fusion_blending=fb)
+    assert fused[5, 5, 11] == np.float32(30.0)
+    assert fused[5, 5, 6] == np.float32(20.0)
+    # intensity adjustment (float)(v * m00 + m01) before resampling, then the min clamp
+    fused, _ = P.fuse_group([img1], [a1], (0, 0, 0), (10, 10, 10), intensity_adjust=[(2.0, 5.0)])
+    assert fused[5, 5, 5] == np.float32(25.0)
+    fused, _ = P.fuse_group([img1], [a1], (0, 0, 0), (10, 10, 10), intensity_adjust=[(0.0, -3.0)])
+    assert fused[5, 5, 5] == np.float32(1.0)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("interp", [1, 0])
 def test_transform_view_matches_oracle(interp):
@@ -201,3 +247,53 @@ def test_prepare_views_pipeline():
     assert float(np.linalg.norm(imgs - oi) / np.linalg.norm(oi)) <= 1e-6
     assert np.abs(ws - ow).max() <= 1e-6
     assert ws.sum(0).max() <= 1.0 + 1e-6
+
+
+@pytest.mark.gpu
+def test_fuse_group_matches_oracle():
+    from mvr import hip
+    rng = np.random.default_rng(9)
+    imgs = [rng.uniform(0, 50, (30, 34, 38)).astype(np.float32) for _ in range(3)]
+    affs = [affine(rot(1, ang) @ np.diag([1.0, 1.0, 1.6]), [2.0 + ang / 20, -3.0, 1.0]) for ang in (0, 40, 80)]
+    adj = [(1.0, 0.0), (1.3, -2.0), (0.8, 4.0)]
+    fb = [(1, 1, 0, 8, 8, 5)] * 3
+    db = [(0, 0, 0, 12, 12, 6)] * 3
+    bb_min, bb_shape = (-10, -6, -4), (50, 44, 56)
+    g, gw = hip.fuse_group(imgs, affs, bb_min, bb_shape, adj, fb, db)
+    o, ow = P.fuse_group(imgs, affs, bb_min, bb_shape, adj, [((1, 1, 0), (8, 8, 5))] * 3,
+                         [((0, 0, 0), (12, 12, 6))] * 3)
+    edge = np.zeros(bb_shape, bool)   # input positions within 1e-9 of a face may go either way in FP64
+    for a in affs:
+        tx, ty, tz = P._input_coords(a, bb_min, bb_shape)
+        for t, n in ((tx, 38), (ty, 34), (tz, 30)):
+            edge |= (np.abs(t) < 1e-9) | (np.abs(t - (n - 1)) < 1e-9)
+    ok = ~edge
+    assert (o[ok] > 0).sum() > 20000
+    e = float(np.linalg.norm(g[ok] - o[ok]) / np.linalg.norm(o[ok]))
+    print(f"fuse_group 3 views: rel-L2 {e:.2e}, weights max |diff| {np.abs(gw - ow).max():.1e}")
+    assert e <= 1e-6
+    assert np.abs(gw - ow).max() <= 1e-6
+
+
+@pytest.mark.gpu
+def test_prepare_groups_pipeline():
+    from mvr import prep
+    rng = np.random.default_rng(10)
+    ph = rng.uniform(5, 50, (30, 32, 36)).astype(np.float32)
+    groups = [[{"img": ph, "affine": affine(rot(1, ang), [3.0, -2.0, 1.0])},
+               {"img": ph, "affine": affine(rot(1, ang + 180), [3.0, -2.0, 1.0]), "intensity": (1.1, -1.0)}]
+              for ang in (0, 60)]
+    bb_min, bb_shape = (-20, -8, -20), (40, 48, 44)
+    imgs, ws = prep.prepare_groups(groups, bb_min, bb_shape, blending_range_fusion=(6, 6, 4),
+                                   blending_border_fusion=(0, 0, 0), blending_range_decon=(10, 10, 5),
+                                   blending_border_decon=(0, 0, 0))
+    want_i, want_w = [], []
+    for views in groups:
+        adj = [v.get("intensity", (1.0, 0.0)) for v in views]
+        fi, fw = P.fuse_group([v["img"] for v in views], [v["affine"] for v in views], bb_min, bb_shape, adj,
+                              [((0, 0, 0), (6, 6, 4))] * 2, [((0, 0, 0), (10, 10, 5))] * 2)
+        want_i.append(fi)
+        want_w.append(fw)
+    want_w = P.normalize_weights(np.stack(want_w))
+    assert float(np.linalg.norm(imgs - np.stack(want_i)) / np.linalg.norm(np.stack(want_i))) <= 1e-6
+    assert np.abs(ws - want_w).max() <= 1e-6
