@@ -8,6 +8,9 @@ Mirrors the exporters the reference's deconvolution plug-in hands its result to
     dataType and compression; one file per block at <dataset>/<i>/<j>/<k> holding a big-endian header
     {mode, number of dimensions, block size} and the big-endian block data, x fastest; raw or gzip).
 Volumes are numpy arrays indexed [z, y, x].  Readers for both formats are included for round trips.
+PSFs are stored like fiji/spimdata/pointspreadfunctions/PointSpreadFunction.java:46-50, 123-177: one N5
+dataset psf_t<timepoint>_v<setup> per view in <base>/psf.n5 (128^3 blocks, gzip level 1); legacy .tif
+files in <base>/psf are read as well (save_psf / load_psf).
 """
 from __future__ import annotations
 
@@ -114,8 +117,8 @@ def read_tiff_stack(path):
 
 
 # ---------------------------------------------------------------- N5
-def save_n5(vol, root, dataset, block_size_xyz=(64, 64, 64), compression="raw", attributes=None):
-    """Writes vol [z, y, x] as float32 N5 dataset `dataset` under `root`."""
+def save_n5(vol, root, dataset, block_size_xyz=(64, 64, 64), compression="raw", attributes=None, level=-1):
+    """Writes vol [z, y, x] as float32 N5 dataset `dataset` under `root` (gzip at `level`, -1 = default)."""
     vol = np.ascontiguousarray(vol, np.float32)
     nz, ny, nx = vol.shape
     os.makedirs(os.path.join(root, dataset), exist_ok=True)
@@ -123,7 +126,7 @@ def save_n5(vol, root, dataset, block_size_xyz=(64, 64, 64), compression="raw", 
     if not os.path.exists(rattr):
         with open(rattr, "w") as f:
             json.dump({"n5": "4.0.0"}, f)
-    comp = {"type": "raw"} if compression == "raw" else {"type": "gzip", "level": -1}
+    comp = {"type": "raw"} if compression == "raw" else {"type": "gzip", "useZlib": False, "level": int(level)}
     attrs = {"dimensions": [nx, ny, nz], "blockSize": list(block_size_xyz), "dataType": "float32",
              "compression": comp}
     if attributes:
@@ -139,7 +142,7 @@ def save_n5(vol, root, dataset, block_size_xyz=(64, 64, 64), compression="raw", 
                 header = struct.pack(">HHIII", 0, 3, dx, dy, dz)
                 data = np.ascontiguousarray(blk).astype(">f4").tobytes()
                 if compression != "raw":
-                    data = gzip.compress(data)
+                    data = gzip.compress(data, compresslevel=9 if level < 0 else int(level))
                 d = os.path.join(root, dataset, str(i // bx), str(j // by))
                 os.makedirs(d, exist_ok=True)
                 with open(os.path.join(d, str(k // bz)), "wb") as f:
@@ -171,3 +174,32 @@ def read_n5(root, dataset):
                 blk = np.frombuffer(body, ">f4").reshape(dims[2], dims[1], dims[0])
                 out[k:k + dims[2], j:j + dims[1], i:i + dims[0]] = blk
     return out, attrs
+
+
+# ---------------------------------------------------------------- PSFs of a dataset
+PSF_N5 = "psf.n5"      # PointSpreadFunction.subPath
+PSF_TIF_DIR = "psf"    # PointSpreadFunction.subDirFile (legacy .tif files)
+
+
+def psf_file_name(timepoint, setup):
+    """PointSpreadFunction.createPSFFilePath: psf_t<timepoint>_v<setup>."""
+    return f"psf_t{int(timepoint)}_v{int(setup)}"
+
+
+def save_psf(base_dir, timepoint, setup, psf):
+    """PointSpreadFunction.save: the PSF as dataset psf_t<tp>_v<setup> of <base_dir>/psf.n5, replacing an
+    existing one, 128^3 blocks, gzip level 1.  Returns the dataset name (the file the XML records)."""
+    import shutil
+    name = psf_file_name(timepoint, setup)
+    root = os.path.join(base_dir, PSF_N5)
+    if os.path.isdir(os.path.join(root, name)):
+        shutil.rmtree(os.path.join(root, name))
+    save_n5(psf, root, name, (128, 128, 128), "gzip", level=1)
+    return name
+
+
+def load_psf(base_dir, file):
+    """PointSpreadFunction.load: a legacy .tif from <base_dir>/psf, otherwise dataset `file` of <base_dir>/psf.n5."""
+    if file.endswith(".tif"):
+        return read_tiff_stack(os.path.join(base_dir, PSF_TIF_DIR, file))[0]
+    return read_n5(os.path.join(base_dir, PSF_N5), file)[0]

@@ -62,3 +62,22 @@ def test_tiff_stack_readable_by_pillow(tmp_path):
         im.seek(z)
         assert im.size == (9, 6)
         np.testing.assert_array_equal(np.asarray(im, np.float32), vol[z])
+
+
+def test_psf_storage_n5_and_legacy_tif(tmp_path):
+    rng = np.random.default_rng(3)
+    psf = rng.uniform(0, 1, (31, 21, 25)).astype(np.float32)
+    base = str(tmp_path)
+    name = export.save_psf(base, 0, 3, psf)
+    assert name == "psf_t0_v3"
+    attrs = json.load(open(os.path.join(base, "psf.n5", name, "attributes.json")))
+    assert attrs["blockSize"] == [128, 128, 128] and attrs["dimensions"] == [25, 21, 31]
+    assert attrs["compression"] == {"type": "gzip", "useZlib": False, "level": 1}
+    np.testing.assert_array_equal(export.load_psf(base, name), psf)
+    # saving again replaces the dataset
+    export.save_psf(base, 0, 3, psf[:5])
+    np.testing.assert_array_equal(export.load_psf(base, name), psf[:5])
+    # legacy PSF files: <base>/psf/<name>.tif
+    os.makedirs(os.path.join(base, "psf"))
+    export.save_tiff_stack(psf, os.path.join(base, "psf", "psf_t0_v1.tif"))
+    np.testing.assert_array_equal(export.load_psf(base, "psf_t0_v1.tif"), psf)
