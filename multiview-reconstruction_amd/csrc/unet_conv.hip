@@ -35,8 +35,14 @@ __device__ __forceinline__ uint16_t f2bf_rne(float f) {
   return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 
+// planes per work-group: a work-group walks zp consecutive z planes of its (y, x) tile (single-chunk layers
+// stage their weights once); launches with fewer than kConvZPTiles plane tiles use one plane per work-group
+// so that the grid still fills the chip
+constexpr int kConvZP = 4;
+constexpr long long kConvZPTiles = 16384;
+
 template <int TX, int NG>
-__global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a) {
+__global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a, int zp) {
   constexpr int TY = 256 / TX;          // tile rows
   constexpr int HX = TX + 2, HY = TY + 2;
   constexpr int HV = 3 * HY * HX;       // halo voxels
@@ -49,16 +55,17 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a) {
   const int i = lane & 15, g = lane >> 4;
   const int hi = g >> 1;                // tap of the pair carried by this lane
   const int ch8 = (g & 1) * 8;          // first of the lane's eight channels
-  // work-group -> (n, z, y block, x block); x block fastest.  Consecutive work-groups go to the 8 XCDs in
-  // turn, so the tile order is remapped to give each XCD a contiguous range of tiles (whole z planes
+  // work-group -> (n, z group, y block, x block); x block fastest.  Consecutive work-groups go to the 8 XCDs
+  // in turn, so the tile order is remapped to give each XCD a contiguous range of tiles (whole z planes
   // and their neighbours stay in that XCD's L2).
-  const int nbx = (a.W + TX - 1) / TX, nby = (a.H + TY - 1) / TY;
+  const int nbx = (a.W + TX - 1) / TX, nby = (a.H + TY - 1) / TY, nzg = (a.D + zp - 1) / zp;
   const int nbt = (int)gridDim.x;
   int b = nbt % 8 == 0 ? (int)(blockIdx.x % 8) * (nbt / 8) + (int)(blockIdx.x / 8) : (int)blockIdx.x;
   const int bx = b % nbx; b /= nbx;
   const int by = b % nby; b /= nby;
-  const int z = b % a.D, n = b / a.D;
+  const int zg = b % nzg, n = b / nzg;
   const int x0 = bx * TX, y0 = by * TY;
+  const int zs = zg * zp, ze = min(a.D, zs + zp);
   const int co0 = blockIdx.y * WC;
 
   f32x4 acc[4][NG];
@@ -75,14 +82,14 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a) {
     segBase[m] = (s / SEGX) * HX + (s % SEGX) * 16 + i;  // halo row offset (before the tap shift)
   }
 
-  // Staging of one 16-channel chunk: every thread issues all of its 16-byte loads (branch-free, invalid
-  // lanes read a valid dummy address) before the first LDS write, so the loads are in flight together.
+  // Staging of one (plane, 16-channel chunk) item: every thread issues all of its 16-byte loads (branch-free,
+  // invalid lanes read a valid dummy address) before the first LDS write, so the loads are in flight together.
   constexpr int NL = (HV * 2 + 255) / 256;         // halo loads per thread
   constexpr int NW = (28 * WC * 2 + 255) / 256;    // weight loads per thread
   uint4 hbuf[NL], wbuf[NW];
   // per-thread halo coordinates of unit k: v = (threadIdx.x >> 1) + 128 k, decomposed incrementally
   const int half = threadIdx.x & 1;
-  auto issue = [&](int cb) {
+  auto issue = [&](int z, int cb, bool weights) {
     // the chunk comes from the upsampled source (half coordinates) or from in
     const bool up = cb * 16 < a.up_c;
     const int sh = up ? 1 : 0;
@@ -115,6 +122,7 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a) {
       hbuf[k] = *reinterpret_cast<const uint4*>(src + (ok ? gv * cs : 0));
       if (!ok) hbuf[k] = make_uint4(0u, 0u, 0u, 0u);
     }
+    if (!weights) return;
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
       const int e = threadIdx.x + 256 * k;
@@ -125,12 +133,13 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a) {
       if (!ok) wbuf[k] = make_uint4(0u, 0u, 0u, 0u);
     }
   };
-  auto park = [&]() {
+  auto park = [&](bool weights) {
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
       const int e = threadIdx.x + 256 * k;
       if (e < HV * 2) *reinterpret_cast<uint4*>(tile + ((e & 1) * HV + (e >> 1)) * 8) = hbuf[k];
     }
+    if (!weights) return;
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
       const int e = threadIdx.x + 256 * k;
@@ -139,11 +148,15 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a) {
   };
 
   const int nchunk = a.cin / 16;
+  const int nitems = (ze - zs) * nchunk;  // (plane, chunk) items, chunks fastest
   const int hsel = ch8 ? 1 : 0;  // the lane's channel half
-  for (int cb = 0; cb < nchunk; ++cb) {
-    if (cb) __syncthreads();  // previous chunk's reads are done
-    issue(cb);
-    park();
+  // a single-chunk layer stages its weights once per work-group
+  for (int it = 0; it < nitems; ++it) {
+    const int z = zs + it / nchunk, cb = it % nchunk;
+    const bool wts = nchunk > 1 || it == 0;
+    if (it) __syncthreads();  // the previous item's LDS reads are done
+    issue(z, cb, wts);
+    park(wts);
     __syncthreads();
     // full unroll folds the tap arithmetic into constants; single-group layers keep registers for occupancy
 #pragma unroll(NG == 1 ? 2 : 14)
@@ -163,29 +176,33 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a) {
         for (int t = 0; t < NG; ++t) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[t], acc[m][t], 0, 0, 0);
       }
     }
-  }
+    if (cb != nchunk - 1) continue;
 
-  // epilogue: bias, ReLU, bf16 stores (voxel 4g + r of the segment, channel co0 + t * 16 + i)
-  float bias[NG];
+    // epilogue of plane z: bias, ReLU, bf16 stores (voxel 4g + r of the segment, channel co0 + t * 16 + i)
+    float bias[NG];
 #pragma unroll
-  for (int t = 0; t < NG; ++t) bias[t] = a.bias ? a.bias[co0 + t * 16 + i] : 0.f;
-  uint16_t* outPlane = a.out + (((size_t)n * a.D + z) * a.H) * a.W * a.out_cs + a.out_co + co0 + i;
+    for (int t = 0; t < NG; ++t) bias[t] = a.bias ? a.bias[co0 + t * 16 + i] : 0.f;
+    uint16_t* outPlane = a.out + (((size_t)n * a.D + z) * a.H) * a.W * a.out_cs + a.out_co + co0 + i;
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const int s = 4 * wave + m;
-    const int y = y0 + s / SEGX;
-    const int xs = x0 + (s % SEGX) * 16 + 4 * g;
-    if (y >= a.H) continue;
-    uint16_t* orow = outPlane + ((size_t)y * a.W + xs) * a.out_cs;
+    for (int m = 0; m < 4; ++m) {
+      const int s = 4 * wave + m;
+      const int y = y0 + s / SEGX;
+      const int xs = x0 + (s % SEGX) * 16 + 4 * g;
+      if (y < a.H) {
+        uint16_t* orow = outPlane + ((size_t)y * a.W + xs) * a.out_cs;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (xs + r >= a.W) continue;
+        for (int r = 0; r < 4; ++r) {
+          if (xs + r >= a.W) continue;
 #pragma unroll
-      for (int t = 0; t < NG; ++t) {
-        float v = acc[m][t][r] + bias[t];
-        if (a.relu) v = fmaxf(v, 0.f);
-        orow[r * a.out_cs + t * 16] = f2bf_rne(v);
+          for (int t = 0; t < NG; ++t) {
+            float v = acc[m][t][r] + bias[t];
+            if (a.relu) v = fmaxf(v, 0.f);
+            orow[r * a.out_cs + t * 16] = f2bf_rne(v);
+          }
+        }
       }
+#pragma unroll
+      for (int t = 0; t < NG; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
   }
 }
@@ -193,8 +210,11 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a) {
 template <int TX, int NG>
 void launch_tile(const ConvArgs& a, hipStream_t s) {
   constexpr int TY = 256 / TX;
-  const unsigned nb = (unsigned)((a.W + TX - 1) / TX) * (unsigned)((a.H + TY - 1) / TY) * (unsigned)a.D * a.N;
-  hipLaunchKernelGGL((k_conv_tile<TX, NG>), dim3(nb, a.cout / (16 * NG)), dim3(256), 0, s, a);
+  const long long tiles = (long long)((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * a.D * a.N;
+  const int zp = tiles >= kConvZPTiles ? kConvZP : 1;
+  const unsigned nb = (unsigned)((a.W + TX - 1) / TX) * (unsigned)((a.H + TY - 1) / TY) *
+                      (unsigned)((a.D + zp - 1) / zp) * a.N;
+  hipLaunchKernelGGL((k_conv_tile<TX, NG>), dim3(nb, a.cout / (16 * NG)), dim3(256), 0, s, a, zp);
 }
 
 template <int TX>

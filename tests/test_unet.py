@@ -304,20 +304,21 @@ def test_receptive_field_within_inference_halo():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("slab", [8, 36])
-def test_unet_slab_inference_equals_single_window(slab):
+@pytest.mark.parametrize("shape,slab", [((150, 20, 28), 8), ((150, 20, 28), 36), ((512, 128, 40), 36)])
+def test_unet_slab_inference_equals_single_window(shape, slab):
     # z slabs with their halo reproduce the single-window estimate exactly, also for a depth that is not a
-    # multiple of 4 (the last slab carries the padded planes)
+    # multiple of 4 (the last slab carries the padded planes).  At 512 x 128 x 40 the single window's
+    # full-resolution convolutions have 16384 plane tiles and run 4 planes per work-group, the slabs' one
     from mvr import hip
-    shape, V = (150, 20, 28), 3
-    rng = np.random.default_rng(slab)
+    V = 3
+    rng = np.random.default_rng(slab + shape[0])
     imgs = [rng.uniform(0.0, 3.0, shape).astype(np.float32) for _ in range(V)]
     ws = [rng.uniform(0.0, 1.0, shape).astype(np.float32) for _ in range(V)]
     net = _gpu_net(V, 0)
     net.set_params(U.random_params(V, seed=13))
     whole = net.infer(imgs, ws)
     halo = hip.UNet.inference_halo()
-    net.set_inference_window((slab + 2 * halo) * 20 * 28)
+    net.set_inference_window((slab + 2 * halo) * shape[1] * shape[2])
     parts = net.infer(imgs, ws)
     net.set_inference_window(0)
     again = net.infer(imgs, ws)
