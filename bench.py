@@ -60,8 +60,8 @@ def main():
                     help="first run a plain device copy of this size (counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this path "
-              "is described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding this path over several "
+              "GPUs is not built", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

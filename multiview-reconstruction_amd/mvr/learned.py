@@ -3,7 +3,7 @@
 The reference's initialisers (deconvolution/init/PsiInit.java:32-47) produce a constant or a
 blurred fusion; this one follows the PsiInitFromRAI pattern (deconvolution/init/PsiInitFromRAI.java:65-82):
 psi comes from outside (here the network, run by the HIP kernels), max[] and avg from the AvgPrecise
-pass.  Plain RL iterations stay both the fallback and the judge: ``Decon.run_until`` measures the
+pass.  Plain RL iterations stay both the fallback and the measure of its error: ``Decon.run_until`` measures the
 distance to a converged reference in FP64 on the device.
 
 Training data come from this method itself: phantoms of the blob generator of the 256^3 / 6-view
