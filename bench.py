@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the time-to-convergence comparison")
     ap.add_argument("--tol", type=float, default=1e-2)
     ap.add_argument("--reference-iterations", type=int, default=200)
+    ap.add_argument("--block-path", type=int, default=0,
+                    help="also time this many iterations through the per-block drop-in ABI (host buffers)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this size (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -155,6 +157,9 @@ def main():
     learned = None
     if not args.no_learned and args.workload in ("c2", "c5"):
         learned = time_to_converge(d, w, args.tol, args.reference_iterations, args.workload)
+    block_path = None
+    if args.block_path > 0 and imgs is not None:
+        block_path = time_block_path(imgs, ws, ks, w, args.block_path)
     if crop0 is not None:
         crop0["k1"], crop0["k2"] = d.kernels(0, ks.shape[1:])
         crop0["psi"], crop0["max"] = d.max_avg()[1], d.max_avg()[0][0]
@@ -199,7 +204,33 @@ def main():
         out["gpu_over_cpu"] = value / cpu["value"]
     if learned:
         out["learned_init"] = learned
+    if block_path:
+        out["block_path"] = block_path
     print(json.dumps(out))
+
+
+def time_block_path(imgs, ws, ks, w, iterations):
+    """The same workload through the per-block drop-in (mvr_block_run_iteration_seq per block and view, host
+    buffers), driven like MultiViewDeconvolutionSeq with the reference's default block (volume + kernel size
+    - 1 per axis): PCIe copies and the host's block copies are inside the time."""
+    from mvr import decon as MD
+    V = w.n_views
+    views = [MD.DeconView(imgs[v], ws[v], ks[v]) for v in range(V)]
+    dv = MD.DeconViews(views, w.psf_type)
+    m = MD.MultiViewDeconvolution(dv, iterations + 1, MD.ComputeBlockThreadHIPFactory(views[0].block_size, w.lam),
+                                  "SEQ")
+    m.run_next_iteration()  # warm-up: device buffers, kernel spectra
+    t0 = time.perf_counter()
+    for _ in range(iterations):
+        m.run_next_iteration()
+    el = time.perf_counter() - t0
+    m.close()
+    nvox = imgs[0].size
+    log(f"per-block drop-in path: {el * 1e3 / iterations:.1f} ms per iteration")
+    return {"value": nvox * iterations / el, "unit": "voxel-iters/s", "ms_per_iteration": el * 1e3 / iterations,
+            "iterations": iterations, "block_xyz": list(views[0].block_size), "blocks_per_view": views[0].num_blocks,
+            "note": "per-block drop-in ABI with host buffers (one mvr_block_run_iteration_seq per block and view); "
+                    "host-device copies and the host driver's block copies are inside the time"}
 
 
 def cpu_block_inputs(inp, w, K):
