@@ -114,7 +114,7 @@ def main():
     log(f"psf init + spectra + psi init in {time.time() - t0:.2f}s, fft grid {d.geometry()}")
     if args.autotune:
         P, ms = d.autotune(args.autotune)
-        log(f"autotuned FFT lengths {P}: {ms:.3f} ms per view update")
+        log(f"autotuned FFT lengths {P}, tile widths y/z {d.geometry()['tile_widths_yz']}: {ms:.3f} ms per view update")
     d.use_graph(args.graph)
 
     # warmup; the first warmup step times every pass to find the dominant kernel
@@ -193,7 +193,8 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": w.name, "volume_zyx": list(w.shape_zyx), "views": V, "psf": list(K),
                    "psf_type": w.psf_type, "lambda": w.lam, "iterations": w.iterations, "scheme": "SEQ",
-                   "fft_grid_xyz": list(geo["P"]), "parallelism": "single GPU", "psi_finite": finite},
+                   "fft_grid_xyz": list(geo["P"]), "tile_widths_yz": list(geo["tile_widths_yz"]),
+                   "parallelism": "single GPU", "psi_finite": finite},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_s * 1e6, "launches": n_dom,

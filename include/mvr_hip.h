@@ -178,12 +178,18 @@ int mvr_transform_psf(int dev, const float* psf, const int psfXYZ[3], const doub
 /* ---- batching and scheduling of the resident run (no counterpart in the reference, whose iteration
  * loop is MultiViewDeconvolution.runIterations, deconvolution/MultiViewDeconvolution.java:144-200) ----
  * Autotuning: per axis the candidatesPerAxis smallest compiled FFT lengths >= extent + K - 1 are timed on
- * view updates (coordinate descent over x, y, z); the fastest set replaces the plan and is kept for later
- * mvr_decon_init calls.  Must follow mvr_decon_init; psi is restored to the initial estimate. */
+ * view updates (coordinate descent over x, y, z), then the tile width (8 or 16 columns) of the y and z passes;
+ * the fastest schedule replaces the plan and is kept for later mvr_decon_init calls.  Must follow
+ * mvr_decon_init; psi is restored to the initial estimate. */
 int mvr_decon_autotune(mvr_decon* d, int candidatesPerAxis, int chosenXYZ[3], double* msPerUpdate);
 /* enable != 0: each iteration (all views) is captured once as a HIP graph and replayed; the graph is
  * rebuilt after mvr_decon_init / mvr_decon_autotune.  Pass timing (mvr_decon_profile) runs eagerly. */
 int mvr_decon_use_graph(mvr_decon* d, int enable);
+/* tile widths (columns) of the y and z passes in use: 0 = the default for the length, else 8 or 16 as chosen
+ * by mvr_decon_autotune or set here (a width not compiled for a length runs the default); the result does
+ * not depend on the tiling */
+int mvr_decon_get_tile_widths(mvr_decon* d, int out[2]);
+int mvr_decon_set_tile_widths(mvr_decon* d, int wy, int wz);
 
 /* ---- convergence checks against a reference volume (FP64 device reductions).  The reference keeps
  * no iteration-wise distance; its IterationStatistics (deconvolution/DeconvolutionMethods.java:308)

@@ -297,6 +297,27 @@ int mvr_decon_autotune(mvr_decon* d, int candidatesPerAxis, int chosenXYZ[3], do
   return d->d.autotune(candidatesPerAxis, chosenXYZ, msPerUpdate);
 }
 
+int mvr_decon_set_tile_widths(mvr_decon* d, int wy, int wz) {
+  NULLCHECK(d);
+  for (const int w : {wy, wz})
+    if (w != 0 && w != 8 && w != 16) { set_error("tile width must be 0, 8 or 16"); return -1; }
+  Decon& x = d->d;
+  x.tunedW[0] = wy;
+  x.tunedW[1] = wz;
+  x.plan.wy = wy;  // the spectra's layout does not depend on the tiling
+  x.plan.wz = wz;
+  x.drop_graph();
+  return 0;
+}
+
+int mvr_decon_get_tile_widths(mvr_decon* d, int out[2]) {
+  NULLCHECK(d);
+  NULLCHECK(out);
+  out[0] = d->d.plan.wy;
+  out[1] = d->d.plan.wz;
+  return 0;
+}
+
 int mvr_decon_use_graph(mvr_decon* d, int enable) {
   NULLCHECK(d);
   return d->d.set_graph(enable != 0);

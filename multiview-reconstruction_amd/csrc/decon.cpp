@@ -187,6 +187,8 @@ int Decon::init(int psiInit, const float* psiIn) {
 int Decon::build_plan_and_spectra(const int* Pmin, int nviews) {
   plan.destroy();
   if (plan.create(N, K, Pmin)) return -1;
+  plan.wy = tunedW[0];
+  plan.wz = tunedW[1];
   for (auto p : spec1) if (p) (void)hipFree(p);
   for (auto p : spec2) if (p) (void)hipFree(p);
   spec1.assign(V, nullptr);
@@ -266,6 +268,8 @@ int Decon::autotune(int candidates, int chosen[3], double* msOut) {
   const int Vsave = V;
   if (scheme == SCHEME_MUL) V = 1;  // time the shared forward transform + one view
   int rc = 0;
+  const int wSave[2] = {tunedW[0], tunedW[1]};
+  tunedW[0] = tunedW[1] = 0;
   if (trial(best, &bestMs)) rc = -1;
   for (int d = 0; d < 3 && !rc; ++d) {
     const int need = N[d] + K[d] - 1;
@@ -288,6 +292,22 @@ int Decon::autotune(int candidates, int chosen[3], double* msOut) {
       }
     }
   }
+  // tile schedule of the strided passes: 8- or 16-column tiles per axis (lengths whose 16-column tile does
+  // not fit the LDS budget run 8 either way)
+  for (int a = 0; a < 2 && !rc; ++a) {
+    int bestW = 0;
+    for (const int wdt : {8, 16}) {
+      tunedW[a] = wdt;
+      double ms = 0;
+      if (trial(best, &ms)) { rc = -1; break; }
+      if (ms < bestMs) {
+        bestMs = ms;
+        bestW = wdt;
+      }
+    }
+    tunedW[a] = bestW;
+  }
+  if (rc) { tunedW[0] = wSave[0]; tunedW[1] = wSave[1]; }
   V = Vsave;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

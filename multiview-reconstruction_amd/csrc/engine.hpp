@@ -75,6 +75,7 @@ struct StridedGeom {
   size_t lineStride;   // complex elements between consecutive pencil elements
   int ncols;           // valid columns (hx) per line
   int nplanes;
+  int width = 0;       // tile width in columns (8 or 16; 0 = the default for the length)
 };
 // mode: 0 forward, 1 inverse, 2 forward * spec * inverse (spec layout = data layout),
 // 3 forward then scale

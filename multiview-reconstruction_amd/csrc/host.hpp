@@ -36,6 +36,7 @@ struct Plan {
   XUpdateFn xu = nullptr;
   XStoreFn xs = nullptr;
   StridedFn sy = nullptr, sz = nullptr;
+  int wy = 0, wz = 0;  // tile widths of the y and z passes (0 = default)
 
   // chooses the FFT lengths; P[d] >= D[d] + K[d] - 1 (or the given minimum per axis)
   int create(const int D[3], const int K[3], const int* Pmin = nullptr);
@@ -126,6 +127,7 @@ struct Decon {
   double2* graphStats = nullptr;  // per-view (Seq) or per-iteration (Mul) stats of the replayed iteration
   // FFT lengths chosen by autotune (0 = the smallest compiled length)
   int tunedP[3] = {0, 0, 0};
+  int tunedW[2] = {0, 0};  // tile widths of the y and z passes chosen by autotune (0 = default)
 
   int create(int dev, const int N[3], int V, int psfType, int scheme, float lambda, float minValue, int nT);
   int set_view(int v, const float* img, const float* w, const float* kernel, const int K[3]);

@@ -379,19 +379,19 @@ __host__ __device__ constexpr int s2_width() {
   return L <= 640 ? 16 : 8;
 }
 constexpr bool kStridedXcdOrder = true;
-template <int L>
-__host__ __device__ constexpr int s2_threads() {
-  return s2_width<L>() * split2(L).A;
+// tile widths compiled per length: 8 and 16 columns where the tile fits 96 KB of LDS
+template <int L, int W>
+__host__ __device__ constexpr bool s2_fits() {
+  return split2(L).A > 0 && L * W * 8 <= 96 * 1024;
 }
 
-template <int L, int MODE>
-__global__ __launch_bounds__(s2_threads<L>()) void k_strided2(StridedGeom gm, const float2* __restrict__ in,
-                                                               float2* __restrict__ out,
-                                                               const float2* __restrict__ spec, float scale,
-                                                               const float2* __restrict__ twg) {
+template <int L, int W, int MODE>
+__global__ __launch_bounds__(W * split2(L).A) void k_strided2(StridedGeom gm, const float2* __restrict__ in,
+                                                              float2* __restrict__ out,
+                                                              const float2* __restrict__ spec, float scale,
+                                                              const float2* __restrict__ twg) {
   constexpr Split2 S = split2(L);
   constexpr int A = S.A, B = S.B;
-  constexpr int W = s2_width<L>();
   constexpr int NT = W * A;
   __shared__ float2 tile[L * W];
   __shared__ float2 tw[L];
@@ -516,20 +516,29 @@ void launch_x1_store(const Dom& d, const StoreParams& st, const float2* S, const
   const int rows = d.D[1] * d.D[2];
   hipLaunchKernelGGL(k_x_store<M>, dim3((rows + RW - 1) / RW), dim3(kThreads), 0, s, d, st, S, twM, wr);
 }
+template <int L, int W>
+void launch_strided2(const StridedGeom& gm, int mode, const float2* in, float2* out, const float2* spec, float scale,
+                     const float2* tw, hipStream_t s) {
+  constexpr int NT = W * split2(L).A;
+  const dim3 g2((gm.ncols + W - 1) / W, gm.nplanes);
+  switch (mode) {
+    case 0: hipLaunchKernelGGL((k_strided2<L, W, 0>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
+    case 1: hipLaunchKernelGGL((k_strided2<L, W, 1>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
+    case 2: hipLaunchKernelGGL((k_strided2<L, W, 2>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
+    default: hipLaunchKernelGGL((k_strided2<L, W, 3>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
+  }
+}
+
+// gm.width picks the tile width (0 or a width not compiled for L: s2_width<L>())
 template <int L>
 void launch_strided(const StridedGeom& gm, int mode, const float2* in, float2* out, const float2* spec, float scale,
                     const float2* tw, hipStream_t s) {
-  if constexpr (split2(L).A > 0 && L * s2_width<L>() * 8 <= 96 * 1024) {
-    constexpr int W2 = s2_width<L>();
-    constexpr int NT = s2_threads<L>();
-    const dim3 g2((gm.ncols + W2 - 1) / W2, gm.nplanes);
-    switch (mode) {
-      case 0: hipLaunchKernelGGL((k_strided2<L, 0>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
-      case 1: hipLaunchKernelGGL((k_strided2<L, 1>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
-      case 2: hipLaunchKernelGGL((k_strided2<L, 2>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
-      default: hipLaunchKernelGGL((k_strided2<L, 3>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
-    }
-    return;
+  if constexpr (s2_fits<L, s2_width<L>()>()) {
+    if constexpr (s2_fits<L, 16>())
+      if (gm.width == 16) return launch_strided2<L, 16>(gm, mode, in, out, spec, scale, tw, s);
+    if constexpr (s2_fits<L, 8>())
+      if (gm.width == 8) return launch_strided2<L, 8>(gm, mode, in, out, spec, scale, tw, s);
+    return launch_strided2<L, s2_width<L>()>(gm, mode, in, out, spec, scale, tw, s);
   }
   constexpr int W = strided_width(L);
   const dim3 grid((gm.ncols + W - 1) / W, gm.nplanes);

@@ -73,6 +73,7 @@ StridedGeom Plan::geom_y() const {
   g.lineStride = dom.HxP;
   g.ncols = Hx;
   g.nplanes = dom.P[2];
+  g.width = wy;
   return g;
 }
 
@@ -82,6 +83,7 @@ StridedGeom Plan::geom_z() const {
   g.lineStride = (size_t)dom.P[1] * dom.HxP;
   g.ncols = Hx;
   g.nplanes = dom.P[1];
+  g.width = wz;
   return g;
 }
 

@@ -85,6 +85,8 @@ def lib():
         L.mvr_decon_init_learned.argtypes = [vp, vp]
         L.mvr_decon_autotune.argtypes = [vp, ctypes.c_int, _i32p, _f64p]
         L.mvr_decon_use_graph.argtypes = [vp, ctypes.c_int]
+        L.mvr_decon_get_tile_widths.argtypes = [vp, _i32p]
+        L.mvr_decon_set_tile_widths.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.mvr_transform_view.argtypes = [ctypes.c_int, vp, _i32p, _f64p, ctypes.POINTER(ctypes.c_longlong), _i32p,
                                          ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, vp]
         L.mvr_transform_blending.argtypes = [ctypes.c_int, _i32p, _f32p, _f32p, _f64p,
@@ -291,7 +293,9 @@ class Decon:
     def geometry(self):
         g = (ctypes.c_int * 4)()
         _check(lib().mvr_decon_get_geometry(self.h, g))
-        return dict(P=(g[0], g[1], g[2]), row_pitch=g[3])
+        tw = (ctypes.c_int * 2)()
+        _check(lib().mvr_decon_get_tile_widths(self.h, tw))
+        return dict(P=(g[0], g[1], g[2]), row_pitch=g[3], tile_widths_yz=(tw[0], tw[1]))
 
     def profile(self, passes=None):
         """Time the given passes (names of PASSES; None = all, [] = off); clears the totals."""
@@ -315,6 +319,10 @@ class Decon:
         ms = ctypes.c_double()
         _check(lib().mvr_decon_autotune(self.h, candidates, ch, ctypes.byref(ms)))
         return (ch[0], ch[1], ch[2]), ms.value
+
+    def set_tile_widths(self, wy, wz):
+        """Tile widths (columns) of the y and z passes: 0 (default for the length), 8 or 16."""
+        _check(lib().mvr_decon_set_tile_widths(self.h, int(wy), int(wz)))
 
     def use_graph(self, enable=True):
         _check(lib().mvr_decon_use_graph(self.h, 1 if enable else 0))

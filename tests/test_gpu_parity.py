@@ -246,7 +246,9 @@ def test_autotuned_lengths_keep_parity(oracle):
     d.init()
     P, ms = d.autotune(3)
     need = [n + k - 1 for n, k in zip(imgs.shape[1:][::-1], ks.shape[1:][::-1])]
-    print(f"autotuned FFT lengths {P} (need >= {need}), {ms:.3f} ms per view update")
+    tw = d.geometry()["tile_widths_yz"]
+    print(f"autotuned FFT lengths {P} (need >= {need}), tile widths y/z {tw}, {ms:.3f} ms per view update")
+    assert all(w in (0, 8, 16) for w in tw)
     assert all(p >= q for p, q in zip(P, need))
     assert d.geometry()["P"] == P
     d.run(5)
@@ -254,6 +256,26 @@ def test_autotuned_lengths_keep_parity(oracle):
     d.init()  # a later init keeps the tuned lengths
     assert d.geometry()["P"] == P
     d.close()
+
+
+@pytest.mark.parametrize("scheme", ["SEQ", "MUL"])
+def test_tile_widths_give_identical_results(scheme):
+    # 8- and 16-column tiles of the strided passes run the same per-column arithmetic
+    inp = _small_problem(weights="zramp")
+    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+    out = []
+    for wy, wz in ((0, 0), (8, 8), (16, 16), (8, 16)):
+        d = hip.Decon(imgs.shape[1:], 3, "OPTIMIZATION_I", scheme, 6e-4)
+        for v in range(3):
+            d.set_view(v, imgs[v], ws[v], ks[v])
+        d.init()
+        d.set_tile_widths(wy, wz)
+        assert d.geometry()["tile_widths_yz"] == (wy, wz)
+        d.run(3)
+        out.append(d.psi())
+        d.close()
+    for o in out[1:]:
+        np.testing.assert_array_equal(o, out[0])
 
 
 def test_psi_init_avg_approx(oracle):
