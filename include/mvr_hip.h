@@ -208,6 +208,13 @@ int mvr_decon_run_until(mvr_decon* d, int maxIterations, double tol, int* iterat
 typedef struct mvr_unet mvr_unet;
 /* views: 1..8 */
 int mvr_unet_create(mvr_unet** out, int dev, int views, unsigned long long seed);
+/* A trained network from its weights file (the .npz that mvr/learned.py writes: float32 vector `params`,
+ * integer scalar `views`, stored uncompressed), so a caller only passes a path: the role of the weights path
+ * of a PsiInitFromRAI-style factory (deconvolution/init/PsiInitFromRAIFactory.java:28-48). */
+int mvr_unet_load(mvr_unet** out, int dev, const char* path);
+/* host only (no GPU): the weights file's parameter count (returned), parameters (if params != NULL and
+ * cap >= count) and views; -1 on a malformed file */
+long long mvr_unet_read_weights(const char* path, float* params, long long cap, int* views);
 /* number of FP32 parameters; get/set use one flat array in layer order (per conv layer the weights
  * [27][cout][cin] then the bias [cout]; last the 1x1 head: 16 weights and its bias) */
 long long mvr_unet_param_count(mvr_unet* u);

@@ -1,8 +1,11 @@
 // capi.cpp -- extern "C" entry points declared in include/mvr_hip.h.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "../../include/mvr_hip.h"
@@ -598,6 +601,137 @@ int mvr_normalize_weights(int dev, float* const* weights, int numViews, const in
 }
 
 // ---------------------------------------------------------------- learned initial estimate
+}  // extern "C"
+
+namespace {
+
+// One array of an uncompressed .npz (the zip of .npy files that np.savez writes): walks the zip's local file
+// headers (stored entries only; sizes from the header or its zip64 extra field) to member `name`.npy and
+// parses its .npy header.  Returns false with a message on anything else.
+bool npz_array(const std::vector<unsigned char>& z, const std::string& name, std::string& descr,
+               std::vector<long long>& shape, const unsigned char*& data, size_t& bytes, std::string& err) {
+  auto u16 = [&](size_t o) { return (unsigned)z[o] | ((unsigned)z[o + 1] << 8); };
+  auto u32 = [&](size_t o) { return (uint32_t)u16(o) | ((uint32_t)u16(o + 2) << 16); };
+  auto u64 = [&](size_t o) { return (uint64_t)u32(o) | ((uint64_t)u32(o + 4) << 32); };
+  size_t o = 0;
+  while (o + 30 <= z.size() && u32(o) == 0x04034b50u) {
+    const unsigned flags = u16(o + 6), method = u16(o + 8), nl = u16(o + 26), el = u16(o + 28);
+    uint64_t size = u32(o + 18);
+    if (o + 30 + nl + el > z.size()) break;
+    const std::string entry(reinterpret_cast<const char*>(&z[o + 30]), nl);
+    for (size_t e = o + 30 + nl; e + 4 <= o + 30 + nl + el;) {  // zip64 extra: original size, compressed size
+      const unsigned id = u16(e), len = u16(e + 2);
+      if (id == 1 && size == 0xFFFFFFFFu && len >= 16) size = u64(e + 12);
+      e += 4 + len;
+    }
+    const size_t d = o + 30 + nl + el;
+    if (flags & 8) { err = "npz entries with data descriptors are not supported"; return false; }
+    if (d + size > z.size()) break;
+    if (entry == name + ".npy") {
+      if (method != 0) { err = "compressed npz entries are not supported (write with np.savez)"; return false; }
+      const unsigned char* p = &z[d];
+      if (size < 10 || std::memcmp(p, "\x93NUMPY", 6) != 0) { err = "not an .npy array: " + entry; return false; }
+      const size_t hl = p[6] == 1 ? u16(d + 8) : (size_t)u32(d + 8), h0 = p[6] == 1 ? 10 : 12;
+      if (h0 + hl > size) { err = "truncated .npy header"; return false; }
+      const std::string h(reinterpret_cast<const char*>(p + h0), hl);
+      auto field = [&](const char* key) {
+        const size_t k = h.find(key);
+        return k == std::string::npos ? std::string() : h.substr(k + std::strlen(key));
+      };
+      const std::string ds = field("'descr': '");
+      descr = ds.substr(0, ds.find('\''));
+      if (field("'fortran_order': ").rfind("False", 0) != 0) { err = "fortran-ordered arrays are not supported"; return false; }
+      std::string sh = field("'shape': (");
+      sh = sh.substr(0, sh.find(')'));
+      shape.clear();
+      for (size_t q = 0; q < sh.size();) {
+        while (q < sh.size() && (sh[q] == ' ' || sh[q] == ',')) ++q;
+        if (q >= sh.size()) break;
+        shape.push_back(std::stoll(sh.substr(q)));
+        while (q < sh.size() && sh[q] != ',') ++q;
+      }
+      data = p + h0 + hl;
+      bytes = size - h0 - hl;
+      return true;
+    }
+    o = d + size;
+  }
+  err = "no array '" + name + "' in the npz file";
+  return false;
+}
+
+int read_weights(const char* path, std::vector<unsigned char>& buf, const float*& params, long long& count,
+                 int& views) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) { set_error(std::string("cannot open ") + path); return -1; }
+  std::fseek(f, 0, SEEK_END);
+  const long n = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  buf.resize(n > 0 ? (size_t)n : 0);
+  const size_t got = n > 0 ? std::fread(buf.data(), 1, buf.size(), f) : 0;
+  std::fclose(f);
+  if (got != buf.size()) { set_error("short read"); return -1; }
+  std::string descr, err;
+  std::vector<long long> shape;
+  const unsigned char* data = nullptr;
+  size_t bytes = 0;
+  if (!npz_array(buf, "params", descr, shape, data, bytes, err)) { set_error(err); return -1; }
+  if (descr != "<f4" || shape.size() != 1 || (size_t)shape[0] * 4 != bytes) {
+    set_error("params must be a float32 vector");
+    return -1;
+  }
+  params = reinterpret_cast<const float*>(data);  // callers copy out with memcpy (no alignment assumed)
+  count = shape[0];
+  if (!npz_array(buf, "views", descr, shape, data, bytes, err)) { set_error(err); return -1; }
+  if (!shape.empty() || !(descr == "<i4" || descr == "<i8")) { set_error("views must be an integer scalar"); return -1; }
+  int64_t v = 0;
+  std::memcpy(&v, data, descr == "<i4" ? 4 : 8);
+  if (descr == "<i4") v = (int32_t)v;
+  views = (int)v;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+long long mvr_unet_read_weights(const char* path, float* params, long long cap, int* views) {
+  NULLCHECK(path);
+  std::vector<unsigned char> buf;
+  const float* p = nullptr;
+  long long count = 0;
+  int v = 0;
+  if (read_weights(path, buf, p, count, v)) return -1;
+  if (params) {
+    if (cap < count) { set_error("params buffer too small"); return -1; }
+    std::memcpy(params, p, (size_t)count * 4);
+  }
+  if (views) *views = v;
+  return count;
+}
+
+int mvr_unet_load(mvr_unet** out, int dev, const char* path) {
+  NULLCHECK(out);
+  NULLCHECK(path);
+  std::vector<unsigned char> buf;
+  const float* p = nullptr;
+  long long count = 0;
+  int v = 0;
+  if (read_weights(path, buf, p, count, v)) return -1;
+  std::vector<float> params((size_t)count);
+  std::memcpy(params.data(), p, (size_t)count * 4);
+  mvr_unet* h = nullptr;
+  if (mvr_unet_create(&h, dev, v, 0)) return -1;
+  if (count != mvr_unet_param_count(h)) {
+    mvr_unet_destroy(h);
+    set_error("the weights do not match the network's parameter count");
+    return -1;
+  }
+  if (mvr_unet_set_params(h, params.data())) { mvr_unet_destroy(h); return -1; }
+  *out = h;
+  return 0;
+}
+
 int mvr_unet_create(mvr_unet** out, int dev, int views, unsigned long long seed) {
   NULLCHECK(out);
   auto* h = new (std::nothrow) mvr_unet();

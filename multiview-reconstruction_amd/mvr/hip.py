@@ -110,6 +110,9 @@ def lib():
         L.mvr_unet_infer.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), _i32p, ctypes.c_float, vp]
         L.mvr_unet_set_inference_window.argtypes = [vp, ctypes.c_longlong]
         L.mvr_unet_inference_halo.argtypes = []
+        L.mvr_unet_load.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_char_p]
+        L.mvr_unet_read_weights.argtypes = [ctypes.c_char_p, vp, ctypes.c_longlong, ctypes.POINTER(ctypes.c_int)]
+        L.mvr_unet_read_weights.restype = ctypes.c_longlong
         L.mvr_unet_destroy.argtypes = [vp]
         L.mvr_unet_destroy.restype = None
         _lib = L
@@ -142,6 +145,17 @@ def device_name(dev=0) -> str:
     buf = ctypes.create_string_buffer(256)
     _check(lib().mvr_get_device_name(dev, buf, 256))
     return buf.value.decode()
+
+
+def read_weights(path):
+    """(params float32 vector, views) of a weights file, parsed by the library on the host (no GPU)."""
+    v = ctypes.c_int()
+    n = lib().mvr_unet_read_weights(os.fsencode(path), None, 0, ctypes.byref(v))
+    if n < 0:
+        raise MvrError(lib().mvr_last_error().decode())
+    out = np.empty(n, np.float32)
+    _check(lib().mvr_unet_read_weights(os.fsencode(path), _ptr(out), n, ctypes.byref(v)))
+    return out, v.value
 
 
 def fft_lengths():
@@ -378,6 +392,16 @@ class UNet:
     def inference_halo():
         """Planes of context on either side of an inference slab."""
         return int(lib().mvr_unet_inference_halo())
+
+    @classmethod
+    def load(cls, path, dev=0):
+        """A trained network from its weights file (read by the library: mvr_unet_load)."""
+        self = cls.__new__(cls)
+        h = ctypes.c_void_p()
+        _check(lib().mvr_unet_load(ctypes.byref(h), dev, os.fsencode(path)))
+        self.h = h
+        self.V = read_weights(path)[1]
+        return self
 
     def __init__(self, n_views, seed=0, dev=0):
         h = ctypes.c_void_p()

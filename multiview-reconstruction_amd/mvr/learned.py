@@ -133,13 +133,9 @@ def save(net: hip.UNet, path, meta: dict):
 
 
 def load(path=DEFAULT_WEIGHTS, dev=0) -> hip.UNet:
-    """U-Net with trained parameters (numpy arrays only; nothing in the file is executed)."""
-    with np.load(path, allow_pickle=False) as f:
-        params = f["params"]
-        V = int(f["views"])
-    net = hip.UNet(V, dev=dev)
-    net.set_params(params)
-    return net
+    """U-Net with trained parameters, read by the library (mvr_unet_load: the arrays `params` and `views`
+    of the uncompressed .npz; nothing in the file is executed)."""
+    return hip.UNet.load(path, dev)
 
 
 def meta(path=DEFAULT_WEIGHTS) -> dict:

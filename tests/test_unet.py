@@ -327,3 +327,44 @@ def test_unet_slab_inference_equals_single_window(shape, slab):
     print(f"slab {slab} + 2 x {halo} planes: max |diff| {np.abs(parts - whole).max():.3e}")
     np.testing.assert_array_equal(parts, whole)
     np.testing.assert_array_equal(again, whole)
+
+
+def test_weights_file_read_by_the_library(tmp_path):
+    # mvr_unet_read_weights (host code, no GPU) against numpy's reader on the committed files, and its errors
+    from mvr import hip
+    from mvr import learned as L
+    for _, path in L.TRAINING.values():
+        with np.load(path, allow_pickle=False) as f:
+            want, views = f["params"], int(f["views"])
+        got, v = hip.read_weights(path)
+        np.testing.assert_array_equal(got, want)
+        assert v == views and got.size == U.param_count()
+    p = U.random_params(3, seed=1)
+    np.savez(tmp_path / "w.npz", params=p, views=np.int64(3))
+    got, v = hip.read_weights(tmp_path / "w.npz")
+    np.testing.assert_array_equal(got, p)
+    assert v == 3
+    np.savez_compressed(tmp_path / "c.npz", params=p, views=np.int32(3))
+    with pytest.raises(hip.MvrError, match="compressed"):
+        hip.read_weights(tmp_path / "c.npz")
+    np.savez(tmp_path / "d.npz", params=p.astype(np.float64), views=np.int32(3))
+    with pytest.raises(hip.MvrError, match="float32"):
+        hip.read_weights(tmp_path / "d.npz")
+    raw = open(tmp_path / "w.npz", "rb").read()
+    open(tmp_path / "t.npz", "wb").write(raw[:1000])
+    with pytest.raises(hip.MvrError):
+        hip.read_weights(tmp_path / "t.npz")
+    with pytest.raises(hip.MvrError, match="cannot open"):
+        hip.read_weights(tmp_path / "missing.npz")
+
+
+@pytest.mark.gpu
+def test_unet_load_from_weights_file():
+    from mvr import hip
+    from mvr import learned as L
+    path = L.TRAINING["c5"][1]
+    net = hip.UNet.load(path)
+    with np.load(path, allow_pickle=False) as f:
+        np.testing.assert_array_equal(net.params(), f["params"])
+        assert net.V == int(f["views"]) == 8
+    net.close()
