@@ -17,7 +17,10 @@ Reference (src/main/java/net/preibisch/mvrecon/process/):
 Pins (tests/test_prep.py): identity and integer-translation transforms reproduce the input; trilinear
 interpolation reproduces affine functions exactly; the blending weight with the identity transform is the
 product of the per-axis weights; the normalisation's closed forms and special cases; the PSF transform
-with the identity keeps the normalised PSF, a 90 degree rotation permutes its axes exactly.
+with the identity keeps the normalised PSF, a 90 degree rotation permutes its axes exactly; group fusion of
+one view is the transformed view, a duplicated view keeps the average and doubles the decon weight, views with
+fusion weight 0 are skipped while constant-1 weights average in the outside zeros, and the intensity adjustment
+with the min clamp follows its closed form.
 """
 from __future__ import annotations
 
