@@ -159,7 +159,13 @@ int mvr_block_thread_create(mvr_block_thread** out, int dev, const int b[3], flo
   NULLCHECK(b);
   auto* h = new (std::nothrow) mvr_block_thread();
   if (!h) { set_error("out of host memory"); return -1; }
-  if (h->t.create(dev, b, minValue, lambda, id)) { h->t.destroy(); delete h; return -1; }
+  if (h->t.create(dev, b, minValue, lambda, id)) {
+    const std::string why = last_error();  // the clean-up's own device calls must not replace the reason
+    h->t.destroy();
+    delete h;
+    set_error(why);
+    return -1;
+  }
   *out = h;
   return 0;
 }
@@ -205,7 +211,13 @@ int mvr_decon_create(mvr_decon** out, int dev, const int n[3], int V, int psfTyp
   NULLCHECK(n);
   auto* h = new (std::nothrow) mvr_decon();
   if (!h) { set_error("out of host memory"); return -1; }
-  if (h->d.create(dev, n, V, psfType, scheme, lambda, minValue, nT)) { h->d.destroy(); delete h; return -1; }
+  if (h->d.create(dev, n, V, psfType, scheme, lambda, minValue, nT)) {
+    const std::string why = last_error();  // the clean-up's own device calls must not replace the reason
+    h->d.destroy();
+    delete h;
+    set_error(why);
+    return -1;
+  }
   *out = h;
   return 0;
 }
@@ -736,7 +748,13 @@ int mvr_unet_create(mvr_unet** out, int dev, int views, unsigned long long seed)
   NULLCHECK(out);
   auto* h = new (std::nothrow) mvr_unet();
   if (!h) { set_error("out of host memory"); return -1; }
-  if (h->n.create(dev, views, seed)) { h->n.destroy(); delete h; return -1; }
+  if (h->n.create(dev, views, seed)) {
+    const std::string why = last_error();  // the clean-up's own device calls must not replace the reason
+    h->n.destroy();
+    delete h;
+    set_error(why);
+    return -1;
+  }
   *out = h;
   return 0;
 }

@@ -60,3 +60,26 @@ def test_mirror_copy_law():
     psi = np.arange(5, dtype=np.float32).reshape(1, 1, 5)
     b = decon.Block((9, 1, 1), (-2, 0, 0), (5, 1, 1), (0, 0, 0), (2, 0, 0))
     np.testing.assert_array_equal(decon.copy_block_mirror(psi, b)[0, 0], [2, 1, 0, 1, 2, 3, 4, 3, 2])
+
+
+def test_argument_errors_are_reported_before_any_device_work():
+    # argument checks run on the host before the library touches a device
+    import numpy as np
+    for args in [((8, 8, 8), 0), ((8, 8, 8), 33), ((0, 8, 8), 1)]:
+        with pytest.raises(hip.MvrError):
+            hip.Decon(*args)
+    h = ctypes.c_void_p()
+    assert hip.lib().mvr_decon_create(ctypes.byref(h), 0, hip._dims((8, 8, 8)), 1, 7, 0, 0.0, 1e-4, 8) < 0
+    assert "PSF type" in hip.lib().mvr_last_error().decode()
+    with pytest.raises(hip.MvrError, match="views"):
+        hip.UNet(9)
+    img = np.ones((4, 4, 4), np.float32)
+    eye = np.eye(3, 4)
+    with pytest.raises(hip.MvrError, match="singular"):
+        hip.transform_view(img, np.zeros((3, 4)), (0, 0, 0), (4, 4, 4))
+    with pytest.raises(hip.MvrError, match="interpolation"):
+        hip.transform_view(img, eye, (0, 0, 0), (4, 4, 4), interpolation=2)
+    with pytest.raises(hip.MvrError, match="at least one view"):
+        hip.fuse_group([], [], (0, 0, 0), (4, 4, 4))
+    with pytest.raises(hip.MvrError, match="empty bounding box"):
+        hip.fuse_group([img], [eye], (0, 0, 0), (0, 4, 4))

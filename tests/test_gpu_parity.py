@@ -213,6 +213,27 @@ def test_resident_decon_edge_shapes(oracle, shape, kshape, psf_type, scheme):
     d.close()
 
 
+@pytest.mark.parametrize("shape,kshape,scheme", [((1, 1, 1), (1, 1, 1), "SEQ"), ((1, 1, 5), (1, 1, 3), "MUL"),
+                                                 ((2, 2, 2), (3, 3, 3), "SEQ"), ((3, 1, 7), (3, 1, 5), "SEQ")])
+def test_resident_decon_degenerate_sizes(oracle, shape, kshape, scheme):
+    # a single voxel, a single row, volumes smaller than their PSF (mirror extension wraps several periods)
+    rng = np.random.default_rng(sum(shape) * 7 + sum(kshape))
+    V = 2
+    imgs = rng.uniform(1.0, 40.0, (V,) + shape).astype(np.float32)
+    ws = rng.uniform(0.2, 1.0, (V,) + shape).astype(np.float32)
+    ks = rng.uniform(0.01, 1.0, (V,) + kshape).astype(np.float32)
+    ref = oracle.deconvolve(imgs, ws, ks, psf_type="EFFICIENT_BAYESIAN", scheme=scheme, lam=6e-4, iterations=4)
+    d = hip.Decon(shape, V, "EFFICIENT_BAYESIAN", scheme, 6e-4)
+    for v in range(V):
+        d.set_view(v, imgs[v], ws[v], ks[v])
+    d.init()
+    d.run(4)
+    e = rel(d.psi(), ref["psi"])
+    print(f"{shape} K {kshape} {scheme}: rel-L2 vs oracle {e:.2e}")
+    assert e < 1e-4
+    d.close()
+
+
 @pytest.mark.parametrize("scheme", ["SEQ", "MUL"])
 def test_graph_replay_matches_eager(scheme):
     # one captured iteration replayed: the same kernels in the same order, so psi and stats are identical;
