@@ -1,38 +1,11 @@
-// fft_core.hpp -- LDS-resident Stockham FFT building blocks for gfx950.
+// fft_core.hpp -- complex arithmetic and the base-radix DFTs (2, 3, 4, 5, 8) held in registers, the building
+// blocks of the register FFTs in fft_reg.hpp.
 //
-// A work-group transforms a tile of pencils that lives in LDS.  Element n of
-// pencil p sits at buf[p*SW + n*SN] (SN, SW compile-time strides in float2),
-// so the same code serves x rows (contiguous pencils, SN = 1) and y/z tiles
-// (W consecutive columns per LDS row, SW = 1).  Each stage is a radix-R
-// Stockham step (self-sorting, natural-order output); a thread keeps all of
-// its butterflies of one stage in registers, so stages run in place with one
-// barrier between the LDS reads and the LDS writes.
-//
-// Sign convention: DIR = -1 forward (exp(-2 pi i nk/L)), DIR = +1 inverse
-// (unnormalised).  Twiddles come from an LDS table tw[k] = exp(-2 pi i k/L)
-// computed on the host in double precision.
+// Sign convention: DIR = -1 forward (exp(-2 pi i nk/L)), DIR = +1 inverse (unnormalised).
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace mvr {
-
-struct RadixList {
-  int n;
-  int r[24];
-};
-
-// Greedy factorisation into radix 8, 4, 2, 5, 3 (n = -1 if L has another prime factor).
-__host__ __device__ constexpr RadixList factorize(int L) {
-  RadixList f{0, {0}};
-  int m = L;
-  while (m % 8 == 0) { f.r[f.n++] = 8; m /= 8; }
-  while (m % 4 == 0) { f.r[f.n++] = 4; m /= 4; }
-  while (m % 2 == 0) { f.r[f.n++] = 2; m /= 2; }
-  while (m % 5 == 0) { f.r[f.n++] = 5; m /= 5; }
-  while (m % 3 == 0) { f.r[f.n++] = 3; m /= 3; }
-  if (m != 1) f.n = -1;
-  return f;
-}
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -129,64 +102,5 @@ template <>
 __device__ __forceinline__ void dft<5, -1>(float2* v) { dft5<-1>(v); }
 template <>
 __device__ __forceinline__ void dft<5, 1>(float2* v) { dft5<1>(v); }
-
-// One radix-R Stockham stage over the LDS tile.  NS = product of the radices of
-// the previous stages.  TP threads cooperate on one pencil; `p` is this
-// thread's pencil, `tp` its index inside the pencil.
-template <int L, int R, int NS, int TP, int SN, int SW, int DIR>
-__device__ __forceinline__ void fft_stage(float2* buf, const float2* tw, int p, int tp) {
-  constexpr int M = L / R;
-  constexpr int IT = (M + TP - 1) / TP;
-  float2 v[IT][R];
-  float2* pb = buf + p * SW;
-#pragma unroll
-  for (int it = 0; it < IT; ++it) {
-    const int j = tp + it * TP;
-    if ((M % TP) == 0 || j < M) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[it][r] = pb[(j + r * M) * SN];
-      if constexpr (NS > 1) {
-        const int k = j % NS;
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          float2 w = tw[k * r * (L / (NS * R))];
-          if constexpr (DIR > 0) w.y = -w.y;
-          v[it][r] = cmul(v[it][r], w);
-        }
-      }
-      dft<R, DIR>(v[it]);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int it = 0; it < IT; ++it) {
-    const int j = tp + it * TP;
-    if ((M % TP) == 0 || j < M) {
-      const int base = (j / NS) * NS * R + (j % NS);
-#pragma unroll
-      for (int r = 0; r < R; ++r) pb[(base + r * NS) * SN] = v[it][r];
-    }
-  }
-  __syncthreads();
-}
-
-template <int L, int IDX, int NS, int TP, int SN, int SW, int DIR>
-__device__ __forceinline__ void fft_stages(float2* buf, const float2* tw, int p, int tp) {
-  constexpr RadixList F = factorize(L);
-  static_assert(F.n > 0, "FFT length must factor into 2, 3, 5");
-  if constexpr (IDX < F.n) {
-    constexpr int R = F.r[IDX];
-    fft_stage<L, R, NS, TP, SN, SW, DIR>(buf, tw, p, tp);
-    fft_stages<L, IDX + 1, NS * R, TP, SN, SW, DIR>(buf, tw, p, tp);
-  }
-}
-
-// Full in-place length-L FFT of every pencil of the tile.  Must be reached by
-// all threads of the work-group (it contains barriers).  The caller provides a
-// barrier between filling `buf` and this call.
-template <int L, int TP, int SN, int SW, int DIR>
-__device__ __forceinline__ void fft_lds(float2* buf, const float2* tw, int p, int tp) {
-  fft_stages<L, 0, 1, TP, SN, SW, DIR>(buf, tw, p, tp);
-}
 
 }  // namespace mvr

@@ -469,54 +469,42 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_store(Dom d, StoreParams st,
   }
 }
 
-// ------------------------------------------------------------------ host launchers (two-step when M splits)
+// ------------------------------------------------------------------ host launchers
 template <int M>
 void launch_x_fwd(const Dom& d, const GatherParams& g, float2* S, const float2* twM, const float2* wr,
                   hipStream_t s) {
-  if constexpr (x2_ok<M>()) {
-    using C = XCfg<M>;
-    const int rows = d.P[1] * d.P[2];
-    hipLaunchKernelGGL(k_x2_fwd<M>, dim3((rows + C::RW - 1) / C::RW), dim3(C::NT), 0, s, d, g, S, twM, wr);
-  } else {
-    launch_x1_fwd<M>(d, g, S, twM, wr, s);
-  }
+  static_assert(x2_ok<M>(), "every compiled x length has a two-step split");
+  using C = XCfg<M>;
+  const int rows = d.P[1] * d.P[2];
+  hipLaunchKernelGGL(k_x2_fwd<M>, dim3((rows + C::RW - 1) / C::RW), dim3(C::NT), 0, s, d, g, S, twM, wr);
 }
 template <int M>
 void launch_x_ratio(const Dom& d, const RatioParams& q, const float2* Sin, float2* Sout, const float2* twM,
                     const float2* wr, hipStream_t s) {
-  if constexpr (x2_ok<M>()) {
-    using C = XCfg<M>;
-    const int rows = d.P[1] * d.P[2];
-    hipLaunchKernelGGL(k_x2_ratio<M>, dim3((rows + C::RW - 1) / C::RW), dim3(C::NT), 0, s, d, q, Sin, Sout, twM,
-                       wr);
-  } else {
-    launch_x1_ratio<M>(d, q, Sin, Sout, twM, wr, s);
-  }
+  static_assert(x2_ok<M>(), "every compiled x length has a two-step split");
+  using C = XCfg<M>;
+  const int rows = d.P[1] * d.P[2];
+  hipLaunchKernelGGL(k_x2_ratio<M>, dim3((rows + C::RW - 1) / C::RW), dim3(C::NT), 0, s, d, q, Sin, Sout, twM,
+                     wr);
 }
 template <int M>
 int launch_x_update(const Dom& d, const UpdateParams& u, const float2* S, const float2* twM, const float2* wr,
                     hipStream_t s) {
-  if constexpr (x2_ok<M>()) {
-    using C = XCfg<M>;
-    const int rows = d.D[1] * d.D[2];
-    const int nb = (rows + C::RW - 1) / C::RW;
-    hipLaunchKernelGGL(k_x2_update<M>, dim3(nb), dim3(C::NT), 0, s, d, u, FastDiv((unsigned)d.D[0]), S, twM, wr);
-    return nb;
-  } else {
-    return launch_x1_update<M>(d, u, S, twM, wr, s);
-  }
+  static_assert(x2_ok<M>(), "every compiled x length has a two-step split");
+  using C = XCfg<M>;
+  const int rows = d.D[1] * d.D[2];
+  const int nb = (rows + C::RW - 1) / C::RW;
+  hipLaunchKernelGGL(k_x2_update<M>, dim3(nb), dim3(C::NT), 0, s, d, u, FastDiv((unsigned)d.D[0]), S, twM, wr);
+  return nb;
 }
 template <int M>
 void launch_x_store(const Dom& d, const StoreParams& st, const float2* S, const float2* twM, const float2* wr,
                     hipStream_t s) {
-  if constexpr (x2_ok<M>()) {
-    using C = XCfg<M>;
-    const int rows = d.D[1] * d.D[2];
-    hipLaunchKernelGGL(k_x2_store<M>, dim3((rows + C::RW - 1) / C::RW), dim3(C::NT), 0, s, d, st,
-                       FastDiv((unsigned)d.D[0]), S, twM, wr);
-  } else {
-    launch_x1_store<M>(d, st, S, twM, wr, s);
-  }
+  static_assert(x2_ok<M>(), "every compiled x length has a two-step split");
+  using C = XCfg<M>;
+  const int rows = d.D[1] * d.D[2];
+  hipLaunchKernelGGL(k_x2_store<M>, dim3((rows + C::RW - 1) / C::RW), dim3(C::NT), 0, s, d, st,
+                     FastDiv((unsigned)d.D[0]), S, twM, wr);
 }
 
 }  // namespace mvr
