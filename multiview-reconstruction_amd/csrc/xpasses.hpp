@@ -22,11 +22,16 @@ struct FastDiv {
   }
 };
 
-// {sum, max} of a work-group of NT threads (NT <= 256, any multiple of 1)
 template <int NT>
-__device__ __forceinline__ void block_reduce_stats_nt(double s, double mx, double2* out) {
-  constexpr int P = NT <= 256 ? 256 : NT <= 512 ? 512 : 1024;  // power of two >= NT
-  __shared__ double rs[P], rm[P];
+constexpr int reduce_width() {
+  return NT <= 256 ? 256 : NT <= 512 ? 512 : 1024;  // power of two >= NT
+}
+
+// {sum, max} of a work-group of NT threads (NT <= 1024) in the caller's LDS: rs and rm hold
+// reduce_width<NT>() doubles each.  Every thread must be done with that memory before the call.
+template <int NT>
+__device__ __forceinline__ void block_reduce_stats_nt(double s, double mx, double2* out, double* rs, double* rm) {
+  constexpr int P = reduce_width<NT>();
   rs[threadIdx.x] = s;
   rm[threadIdx.x] = mx;
   for (int i = NT + (int)threadIdx.x; i < P; i += NT) {
@@ -56,9 +61,11 @@ struct XCfg {
   static constexpr int NT = RW * T;                // threads per work-group
   static constexpr int RS = M + 5;                 // LDS row stride (complex): M + 1 used, +4 against bank conflicts
   static constexpr int PX = 2 * M;
-  // x_update: twiddle tables in LDS unless leaving them in global memory (read through L1) fits one more
-  // work-group per CU into the 160 KB of LDS
-  static constexpr int LDS_ROWS = RW * RS * 8 + RW * 16;   // the update kernel's rows and row tables
+  // x_update: twiddle tables (2M + 1 complex) in LDS unless leaving them in global memory (read through
+  // L1) fits one more work-group per CU into the 160 KB of LDS; its other LDS is the rows and 16 B of row
+  // tables per row (the stats reduction reuses the rows).  The forward passes keep their twiddles in LDS:
+  // at M = 150 and 576 they were slower with global twiddles despite the extra work-group.
+  static constexpr int LDS_ROWS = RW * RS * 8 + RW * 16;
   static constexpr int LDS_TW = (2 * M + 1) * 8;
   static constexpr bool TWL = (160 * 1024) / (LDS_ROWS + LDS_TW) >= (160 * 1024) / LDS_ROWS;
 };
@@ -359,8 +366,6 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
                                                             const float2* __restrict__ wr) {
   using C = XCfg<M>;
   __shared__ float2 buf[C::RW * C::RS];
-  // the update pass only runs inverse transforms: its twiddles stay in global memory when that fits one
-  // more work-group per CU (the forward passes read theirs more often and keep them in LDS)
   __shared__ float2 twS[C::TWL ? M : 1];
   __shared__ float2 twrS[C::TWL ? M + 1 : 1];
   const float2* tw = C::TWL ? twS : twM;
@@ -424,7 +429,12 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
       mx = fmax(mx, (double)ch);
     }
   }
-  block_reduce_stats_nt<C::NT>(sum, mx, u.partial + blockIdx.x);
+  // the stats reduction reuses the row buffer once every thread has read its rows
+  static_assert(C::RW * C::RS * sizeof(float2) >= 2 * reduce_width<C::NT>() * sizeof(double),
+                "the row buffer holds the reduction");
+  __syncthreads();
+  double* red = reinterpret_cast<double*>(buf);
+  block_reduce_stats_nt<C::NT>(sum, mx, u.partial + blockIdx.x, red, red + reduce_width<C::NT>());
 }
 
 // ------------------------------------------------------------------ inverse + store of the output region
