@@ -179,8 +179,9 @@ int mvr_transform_psf(int dev, const float* psf, const int psfXYZ[3], const doub
  * loop is MultiViewDeconvolution.runIterations, deconvolution/MultiViewDeconvolution.java:144-200) ----
  * Autotuning: per axis the candidatesPerAxis smallest compiled FFT lengths >= extent + K - 1 are timed on
  * view updates (coordinate descent over x, y, z), then the tile width (8 or 16 columns) of the y and z passes;
- * the fastest schedule replaces the plan and is kept for later mvr_decon_init calls.  Must follow
- * mvr_decon_init; psi is restored to the initial estimate. */
+ * a trial is the fastest of 3 timed view updates, and a candidate replaces the incumbent only when it is more
+ * than 1 % faster.  The chosen schedule replaces the plan and is kept for later mvr_decon_init calls.  Must
+ * follow mvr_decon_init; psi is restored to the initial estimate. */
 int mvr_decon_autotune(mvr_decon* d, int candidatesPerAxis, int chosenXYZ[3], double* msPerUpdate);
 /* enable != 0: each iteration (all views) is captured once as a HIP graph and replayed; the graph is
  * rebuilt after mvr_decon_init / mvr_decon_autotune.  Pass timing (mvr_decon_profile) runs eagerly. */

@@ -247,22 +247,36 @@ int Decon::autotune(int candidates, int chosen[3], double* msOut) {
   if (plan.create(N, K, nullptr)) return -1;
   for (int d = 0; d < 3; ++d) best[d] = plan.dom.P[d];
   double bestMs = 1e30;
+  // a trial is the fastest of 3 separately timed view updates after a warm-up one (a transient slowdown of
+  // the device does not pick the schedule); a candidate replaces the incumbent only when it is faster by
+  // more than kMargin, so that timing noise does not trade a smaller grid for a larger one
+  constexpr double kMargin = 0.01;
   auto trial = [&](const int P[3], double* ms) -> int {
     if (build_plan_and_spectra(P, 1)) return -1;
     const bool timing = timer.on;
     timer.on = false;
-    int rc = 0;
-    for (int rep = 0; rep < 3 && !rc; ++rep) {  // first repetition warms up
-      if (rep == 1) MVR_HIP(hipEventRecord(e0, stream));
-      rc = scheme == SCHEME_SEQ ? view_seq(0, st) : iteration_mul(st);
+    int rc = scheme == SCHEME_SEQ ? view_seq(0, st) : iteration_mul(st);
+    double best = 1e30;
+    for (int rep = 0; rep < 3 && !rc; ++rep) {
+      float t = 0;
+      if (hipEventRecord(e0, stream) != hipSuccess) {
+        set_error("autotune: event record failed");
+        rc = -1;
+        break;
+      }
+      rc = scheme == SCHEME_SEQ ? view_seq(0, st) : iteration_mul(st);  // sets its own error
+      if (rc) break;
+      if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+        set_error("autotune: event timing failed");
+        rc = -1;
+        break;
+      }
+      best = std::min(best, (double)t);
     }
     timer.on = timing;
     if (rc) return -1;
-    MVR_HIP(hipEventRecord(e1, stream));
-    MVR_HIP(hipEventSynchronize(e1));
-    float t = 0;
-    MVR_HIP(hipEventElapsedTime(&t, e0, e1));
-    *ms = t / 2.0;
+    *ms = best;
     return 0;
   };
   const int Vsave = V;
@@ -286,7 +300,7 @@ int Decon::autotune(int candidates, int chosen[3], double* msOut) {
       double ms = 0;
       if (trial(P, &ms)) { rc = -1; break; }
       ++tried;
-      if (ms < bestMs) {
+      if (ms < bestMs * (1.0 - kMargin)) {
         bestMs = ms;
         best[d] = *p;
       }
@@ -300,7 +314,7 @@ int Decon::autotune(int candidates, int chosen[3], double* msOut) {
       tunedW[a] = wdt;
       double ms = 0;
       if (trial(best, &ms)) { rc = -1; break; }
-      if (ms < bestMs) {
+      if (ms < bestMs * (1.0 - kMargin)) {
         bestMs = ms;
         bestW = wdt;
       }
