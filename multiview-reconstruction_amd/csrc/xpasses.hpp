@@ -125,6 +125,23 @@ __device__ __forceinline__ void x2_inverse_half(float2* rb, const float2* tw, co
   }
 }
 
+// forward transform whose first-step inputs v[a] = z[q + B a] (thread q < B) are already in registers;
+// LDS row `rb` is only written -> out[k2] = Z[q + A k2] (thread q < A)
+template <int M>
+__device__ __forceinline__ void x2_forward_regs(float2* rb, const float2* tw, int q, float2* v, float2* out) {
+  using C = XCfg<M>;
+  constexpr int A = C::A, B = C::B;
+  if (q < B) {
+    rdft<A, -1>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) rb[q + B * k1] = k1 == 0 ? v[k1] : cmul(v[k1], tw[q * k1]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < B; ++b) out[b] = rb[b + B * q];
+  rdft<B, -1>(out);
+}
+
 // registers -> natural order in LDS.  Forward result: Z[q + A k2] from out[k2].
 template <int M>
 __device__ __forceinline__ void x2_park_forward(float2* rb, int q, const float2* z) {
@@ -256,27 +273,28 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
       dstRow[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : kNoRow;
     }
     __syncthreads();
-    constexpr int TOT = C::RW * C::PX;
-    constexpr int IT = (TOT + C::NT - 1) / C::NT;
-    float v[IT];
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int idx = threadIdx.x + it * C::NT;
-      const int r = min(idx / C::PX, C::RW - 1), x = idx - r * C::PX;
+    // the first transform step reads its values z[q + B a] = (x[2n], x[2n + 1]) straight from psi
+    const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
+    float2 v[C::A];
+    if (q < C::B) {
       const long long off = srcRow[r];
-      const int gx = xmap[min(x, C::PX - 1)];
-      const bool ok = idx < TOT && off >= 0 && gx >= 0;
-      v[it] = g.src[ok ? off + gx : 0];
-      if (!ok) v[it] = g.value;
-    }
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int idx = threadIdx.x + it * C::NT;
-      if (idx < TOT) {
-        const int r = idx / C::PX, x = idx - r * C::PX;
-        fb[r * 2 * C::RS + x] = v[it];
+      for (int a = 0; a < C::A; ++a) {
+        const int x = 2 * (q + C::B * a);
+        const int g0 = xmap[x], g1 = xmap[x + 1];
+        const bool ok0 = off >= 0 && g0 >= 0, ok1 = off >= 0 && g1 >= 0;
+        v[a].x = g.src[ok0 ? off + g0 : 0];
+        v[a].y = g.src[ok1 ? off + g1 : 0];
+        if (!ok0) v[a].x = g.value;
+        if (!ok1) v[a].y = g.value;
       }
     }
+    float2* rb = buf + r * C::RS;
+    float2 z[C::B];
+    x2_forward_regs<M>(rb, tw, q, v, z);
+    x2_park_forward<M>(rb, q, z);
+    x2_post_store<M>(buf, twr, dstRow, S);
+    return;
   }
   __syncthreads();
   const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
