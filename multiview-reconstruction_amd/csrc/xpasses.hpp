@@ -343,12 +343,45 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
   float2* rb = buf + r * C::RS;
   float2 z[C::A];
   x2_inverse_half<M>(rb, tw, twr, q, z);
-  x2_park_inverse<M>(rb, q, z);
-  float* fb = reinterpret_cast<float*>(buf);
-  {
+  constexpr int TOT = C::RW * C::PX;
+  constexpr int IT = (TOT + C::NT - 1) / C::NT;
+  float2 zf[C::B];
+  if constexpr (IT > 32) {
+    // Long rows: thread q < B holds blurred values z[a] = (x[2n], x[2n + 1]), n = q + B a, exactly the
+    // inputs of its first forward step, so the quotient is formed in registers with the image read straight
+    // from HBM (the flat pass below would keep IT image values per thread in flight; at M = 576 this is 11 %
+    // faster, at M = 150 the flat pass is 2 % faster)
+    if (q < C::B) {
+      const long long ir = imgRow[r];
+      const int x0 = d.m[0], x1 = d.m[0] + d.D[0];
+      constexpr int H = (C::A + 1) / 2;  // two halves of loads in flight
+#pragma unroll
+      for (int a0 = 0; a0 < C::A; a0 += H) {
+        float im[2 * H];
+#pragma unroll
+        for (int j = 0; j < 2 * H; ++j) {
+          const int a = a0 + (j >> 1);
+          const int x = 2 * (q + C::B * a) + (j & 1);
+          const bool ok = a < C::A && ir != kOuterRow && x >= x0 && x < x1;
+          im[j] = q_.img[ok ? ir + x : 0];
+          if (!ok) im[j] = 0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < 2 * H; j += 2) {
+          const int a = a0 + (j >> 1);
+          if (a < C::A) {
+            z[a].x = im[j] > 0.f ? im[j] / z[a].x : 1.f;
+            z[a].y = im[j + 1] > 0.f ? im[j + 1] / z[a].y : 1.f;
+          }
+        }
+      }
+    }
+    __syncthreads();  // the inverse's reads of the row are done before the forward writes it
+    x2_forward_regs<M>(rb, tw, q, z, zf);
+  } else {
+    x2_park_inverse<M>(rb, q, z);
+    float* fb = reinterpret_cast<float*>(buf);
     const int x0 = d.m[0], x1 = d.m[0] + d.D[0];
-    constexpr int TOT = C::RW * C::PX;
-    constexpr int IT = (TOT + C::NT - 1) / C::NT;
     float im[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -368,10 +401,9 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
         *p = im[it] > 0.f ? im[it] / *p : 1.f;
       }
     }
+    __syncthreads();
+    x2_forward<M>(rb, tw, q, zf);
   }
-  __syncthreads();
-  float2 zf[C::B];
-  x2_forward<M>(rb, tw, q, zf);
   x2_park_forward<M>(rb, q, zf);
   x2_post_store<M>(buf, twr, outOff, Sout);
 }

@@ -90,6 +90,31 @@ def test_resident_decon_long_axes(oracle, shape):
     assert e < 1e-4
 
 
+@pytest.mark.parametrize("px", [720, 800, 864, 900, 960, 1080])
+def test_resident_decon_long_x_rows(oracle, px):
+    # x lengths whose quotient pass forms img / blurred in the registers between its two transforms (rows of
+    # px / 2 complex values split with more than 16 per thread); zero image pixels take the quotient's 1 branch
+    rng = np.random.default_rng(px)
+    shape = (5, 6, px - 2)
+    V = 2
+    imgs = rng.uniform(1.0, 40.0, (V,) + shape).astype(np.float32)
+    imgs[:, :, :, ::7] = 0.0
+    ws = rng.uniform(0.0, 1.0, (V,) + shape).astype(np.float32)
+    ks = rng.uniform(0.01, 1.0, (V, 3, 3, 3)).astype(np.float32)
+    ref = oracle.deconvolve(imgs, ws, ks, psf_type="EFFICIENT_BAYESIAN", lam=6e-4, iterations=3)
+    d = hip.Decon(shape, V, "EFFICIENT_BAYESIAN", "SEQ", 6e-4)
+    for v in range(V):
+        d.set_view(v, imgs[v], ws[v], ks[v])
+    d.init()
+    d.run(3)
+    P = d.geometry()["P"]
+    e = rel(d.psi(), ref["psi"])
+    d.close()
+    print(f"{shape}: FFT grid {P}, rel-L2 vs oracle {e:.2e}")
+    assert P[0] == px
+    assert e < 1e-4
+
+
 def test_inplace_convolution_is_mirror_convolution(oracle):
     img = RNG.uniform(0.5, 2.0, (24, 20, 16)).astype(np.float32)
     k = RNG.uniform(0.0, 1.0, (5, 7, 3)).astype(np.float32)
