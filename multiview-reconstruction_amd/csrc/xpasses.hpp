@@ -70,6 +70,13 @@ struct XCfg {
   static constexpr bool TWL = (160 * 1024) / (LDS_ROWS + LDS_TW) >= (160 * 1024) / LDS_ROWS;
 };
 
+// Rows whose second split factor B exceeds 16 (M >= 360 except 384 and 512): every lane of a row holds
+// values after the transforms, and x_ratio / x_update work on them in registers instead of a flat LDS pass
+template <int M>
+__host__ __device__ constexpr bool x2_long_rows() {
+  return split2(M).B > 16;
+}
+
 template <int M>
 __host__ __device__ constexpr bool x2_ok() {
   return split2(M).A > 0 && split2(M).B > 1;
@@ -346,11 +353,11 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
   constexpr int TOT = C::RW * C::PX;
   constexpr int IT = (TOT + C::NT - 1) / C::NT;
   float2 zf[C::B];
-  if constexpr (IT > 32) {
+  if constexpr (x2_long_rows<M>()) {
     // Long rows: thread q < B holds blurred values z[a] = (x[2n], x[2n + 1]), n = q + B a, exactly the
     // inputs of its first forward step, so the quotient is formed in registers with the image read straight
-    // from HBM (the flat pass below would keep IT image values per thread in flight; at M = 576 this is 11 %
-    // faster, at M = 150 the flat pass is 2 % faster)
+    // from HBM (at M = 576 11 % faster than the flat pass below; at M = 150, where only B of A lanes hold
+    // values, the flat pass is 2 % faster)
     if (q < C::B) {
       const long long ir = imgRow[r];
       const int x0 = d.m[0], x1 = d.m[0] + d.D[0];
@@ -444,39 +451,74 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
   float2* rb = buf + r * C::RS;
   float2 z[C::A];
   x2_inverse_half<M>(rb, tw, twr, q, z);
-  x2_park_inverse<M>(rb, q, z);
-  const float* fb = reinterpret_cast<const float*>(buf);
   double sum = 0.0, mx = -1.0;
   const int DX = d.D[0];
-  const unsigned total = (unsigned)(C::RW * DX);
-  constexpr int U = 4;
-  for (unsigned base = threadIdx.x; base < total; base += U * C::NT) {
-    float last[U], wt[U];
-    size_t gi[U];
-    unsigned rr[U];
-    bool ok[U];
+  if constexpr (x2_long_rows<M>()) {
+    // Long rows: thread q < B holds the values z[a] = (x[2n], x[2n + 1]), n = q + B a, of its row, and updates
+    // psi there directly (no park to LDS and flat pass)
+    const long long pr = psiRow[r];
+    if (q < C::B && pr >= 0) {
+      const int m0 = d.m[0];
+      float* psiRowPtr = u.psi + pr;
+      const float* wRowPtr = u.w + pr;
+      constexpr int H = (C::A + 3) / 4;  // loads of a quarter of the row's values in flight
 #pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const unsigned idx = base + j * C::NT;
-      rr[j] = min(divD0.div(idx), (unsigned)C::RW - 1);
-      const int lx = (int)(idx - rr[j] * (unsigned)DX);
-      const long long pr = psiRow[rr[j]];
-      ok[j] = idx < total && pr >= 0;
-      gi[j] = ok[j] ? (size_t)(pr + lx) : 0;
-      last[j] = u.psi[gi[j]];
-      wt[j] = u.w[gi[j]];
+      for (int a0 = 0; a0 < C::A; a0 += H) {
+        float last[2 * H], wt[2 * H];
+        int lx[2 * H];
+#pragma unroll
+        for (int j = 0; j < 2 * H; ++j) {
+          const int a = a0 + (j >> 1);
+          lx[j] = 2 * (q + C::B * a) + (j & 1) - m0;
+          if (a >= C::A || lx[j] < 0 || lx[j] >= DX) lx[j] = -1;
+          last[j] = psiRowPtr[max(lx[j], 0)];
+          wt[j] = wRowPtr[max(lx[j], 0)];
+        }
+#pragma unroll
+        for (int j = 0; j < 2 * H; ++j) {
+          if (lx[j] < 0) continue;
+          const int a = min(a0 + (j >> 1), C::A - 1);
+          const float next = rl_next_value(last[j], (j & 1) ? z[a].y : z[a].x, wt[j], u.lambda, u.minValue, u.maxI);
+          psiRowPtr[lx[j]] = next;
+          const float ch = __fsub_rn(next, last[j]);
+          sum += (double)ch;
+          mx = fmax(mx, (double)ch);
+        }
+      }
     }
+  } else {
+    x2_park_inverse<M>(rb, q, z);
+    const float* fb = reinterpret_cast<const float*>(buf);
+    const unsigned total = (unsigned)(C::RW * DX);
+    constexpr int U = 4;
+    for (unsigned base = threadIdx.x; base < total; base += U * C::NT) {
+      float last[U], wt[U];
+      size_t gi[U];
+      unsigned rr[U];
+      bool ok[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) {
-      if (!ok[j]) continue;
-      const unsigned idx = base + j * C::NT;
-      const int lx = (int)(idx - rr[j] * (unsigned)DX);
-      const float next = rl_next_value(last[j], fb[rr[j] * 2 * C::RS + d.m[0] + lx], wt[j], u.lambda, u.minValue,
-                                       u.maxI);
-      u.psi[gi[j]] = next;
-      const float ch = __fsub_rn(next, last[j]);
-      sum += (double)ch;
-      mx = fmax(mx, (double)ch);
+      for (int j = 0; j < U; ++j) {
+        const unsigned idx = base + j * C::NT;
+        rr[j] = min(divD0.div(idx), (unsigned)C::RW - 1);
+        const int lx = (int)(idx - rr[j] * (unsigned)DX);
+        const long long pr = psiRow[rr[j]];
+        ok[j] = idx < total && pr >= 0;
+        gi[j] = ok[j] ? (size_t)(pr + lx) : 0;
+        last[j] = u.psi[gi[j]];
+        wt[j] = u.w[gi[j]];
+      }
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        if (!ok[j]) continue;
+        const unsigned idx = base + j * C::NT;
+        const int lx = (int)(idx - rr[j] * (unsigned)DX);
+        const float next = rl_next_value(last[j], fb[rr[j] * 2 * C::RS + d.m[0] + lx], wt[j], u.lambda, u.minValue,
+                                         u.maxI);
+        u.psi[gi[j]] = next;
+        const float ch = __fsub_rn(next, last[j]);
+        sum += (double)ch;
+        mx = fmax(mx, (double)ch);
+      }
     }
   }
   // the stats reduction reuses the row buffer once every thread has read its rows
