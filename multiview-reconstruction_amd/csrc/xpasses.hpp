@@ -1,11 +1,15 @@
 // xpasses.hpp -- x (real axis) passes with the two-step register FFT.
 //
-// A work-group owns RW rows of one x-pass.  Global traffic is staged through
-// LDS with flat, coalesced mappings (row data is contiguous in HBM); the
-// length-M complex transform of a row (M = P0/2, real FFT by the even/odd
-// split) runs as M = A x B with one LDS exchange, and the C2R pre-processing
-// is folded into the first inverse step.  Index arithmetic that depends on
-// runtime sizes is done once per row (row tables in LDS) or with a
+// A work-group owns RW rows of one x-pass.  The length-M complex transform of
+// a row (M = P0/2, real FFT by the even/odd split) runs as M = A x B with one
+// LDS exchange, and the C2R pre-processing is folded into the first inverse
+// step.  Spectra are staged through LDS with flat, coalesced mappings (row
+// data is contiguous in HBM).  Real-valued data meets the transforms in the
+// registers where that is possible: the forward pass reads psi straight into
+// its first transform step, and on long rows (B > 16) the quotient and the
+// psi update work on the values the inverse leaves in registers; otherwise
+// they run as flat passes over the row in LDS.  Index arithmetic that depends
+// on runtime sizes is done once per row (row tables in LDS) or with a
 // multiply-shift divider, never per element with '/' or '%'.
 #pragma once
 #include "passes.hpp"
