@@ -144,6 +144,14 @@ __global__ __launch_bounds__(W * split2(L).A) void k_strided2(StridedGeom gm, co
         tile[(q + B * k1) * W + w] = t;
       }
     }
+    // convolution: the kernel spectrum values of this thread's outputs are requested before the exchange,
+    // so that their latency overlaps it
+    float2 kv[MODE == 2 ? B : 1];
+    if constexpr (MODE == 2) {
+      const float2* sp = spec + (size_t)plane * gm.planeStride + col0 + w;
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) kv[k2] = colOk ? sp[(size_t)(q + A * k2) * ls] : make_float2(0.f, 0.f);
+    }
     __syncthreads();
     // ---- forward step 2: thread q = k1 < A
     float2 u[B];
@@ -162,12 +170,8 @@ __global__ __launch_bounds__(W * split2(L).A) void k_strided2(StridedGeom gm, co
       return;
     } else {
       // spectrum product, then inverse step 1 over k2 in registers
-      const float2* sp = spec + (size_t)plane * gm.planeStride + col0 + w;
 #pragma unroll
-      for (int k2 = 0; k2 < B; ++k2) {
-        const float2 kv = colOk ? sp[(size_t)(q + A * k2) * ls] : make_float2(0.f, 0.f);
-        u[k2] = cmul(u[k2], kv);
-      }
+      for (int k2 = 0; k2 < B; ++k2) u[k2] = cmul(u[k2], kv[k2]);
       rdft<B, 1>(u);  // u[n2] = sum_k2 X[q + A k2] W_B^{-n2 k2}
       __syncthreads();  // every thread has read its forward values from the tile
 #pragma unroll
