@@ -104,17 +104,30 @@ __host__ __device__ constexpr bool s2_fits() {
   return split2(L).A > 0 && L * W * 8 <= 96 * 1024;
 }
 
+// The twiddles stay in global memory (read through L1) when that lets a second work-group share the CU's
+// 160 KB of LDS; such kernels are then compiled for the waves per SIMD that two work-groups need.
+template <int L, int W>
+__host__ __device__ constexpr bool s2_global_tw() {
+  return 2 * (L * W * 8 + L * 8) > 160 * 1024 && 2 * L * W * 8 <= 160 * 1024;
+}
+template <int L, int W>
+__host__ __device__ constexpr int s2_min_waves() {
+  return s2_global_tw<L, W>() ? (2 * ((W * split2(L).A + 63) / 64) + 3) / 4 : 1;
+}
+
 template <int L, int W, int MODE>
-__global__ __launch_bounds__(W * split2(L).A) void k_strided2(StridedGeom gm, const float2* __restrict__ in,
-                                                              float2* __restrict__ out,
-                                                              const float2* __restrict__ spec, float scale,
-                                                              const float2* __restrict__ twg) {
+__global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W>())) void k_strided2(
+    StridedGeom gm, const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ spec,
+    float scale, const float2* __restrict__ twg) {
   constexpr Split2 S = split2(L);
   constexpr int A = S.A, B = S.B;
   constexpr int NT = W * A;
+  constexpr bool TWG = s2_global_tw<L, W>();
   __shared__ float2 tile[L * W];
-  __shared__ float2 tw[L];
-  for (int i = threadIdx.x; i < L; i += NT) tw[i] = twg[i];
+  __shared__ float2 twS[TWG ? 1 : L];
+  const float2* tw = TWG ? twg : twS;
+  if (!TWG)
+    for (int i = threadIdx.x; i < L; i += NT) twS[i] = twg[i];
   const int w = threadIdx.x % W, q = threadIdx.x / W;
   // Work-groups are dealt to the 8 XCDs in turn; the tile order is remapped so that each XCD sweeps a
   // contiguous range of (column tile, plane) tiles: neighbouring column tiles, which share cache lines
