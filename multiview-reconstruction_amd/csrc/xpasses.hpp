@@ -67,11 +67,19 @@ struct XCfg {
   static constexpr int PX = 2 * M;
   // x_update: twiddle tables (2M + 1 complex) in LDS unless leaving them in global memory (read through
   // L1) fits one more work-group per CU into the 160 KB of LDS; its other LDS is the rows and 16 B of row
-  // tables per row (the stats reduction reuses the rows).  The forward passes keep their twiddles in LDS:
-  // at M = 150 and 576 they were slower with global twiddles despite the extra work-group.
+  // tables per row (the stats reduction reuses the rows).
   static constexpr int LDS_ROWS = RW * RS * 8 + RW * 16;
   static constexpr int LDS_TW = (2 * M + 1) * 8;
   static constexpr bool TWL = (160 * 1024) / (LDS_ROWS + LDS_TW) >= (160 * 1024) / LDS_ROWS;
+  // The forward passes follow the same rule on long rows (B > 16; their other LDS: the x gather map, 8 more
+  // bytes of row tables in the quotient pass): at M = 576 both gain a third work-group per CU and run 2-5 %
+  // faster.  On shorter rows they keep LDS twiddles: at M = 150 global twiddles were slower (x_fwd 5.9 ->
+  // 6.2, x_ratio 8.8 -> 9.9 ms per 256^3 run) despite the extra work-group.
+  static constexpr bool long_rows = S.B > 16;
+  static constexpr bool TWL_FWD =
+      !long_rows || (160 * 1024) / (LDS_ROWS + PX * 4 + LDS_TW) >= (160 * 1024) / (LDS_ROWS + PX * 4);
+  static constexpr bool TWL_RATIO =
+      !long_rows || (160 * 1024) / (LDS_ROWS + RW * 8 + LDS_TW) >= (160 * 1024) / (LDS_ROWS + RW * 8);
 };
 
 // Rows whose second split factor B exceeds 16 (M >= 360 except 384 and 512): every lane of a row holds
@@ -239,12 +247,14 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
                                                          const float2* __restrict__ wr) {
   using C = XCfg<M>;
   __shared__ float2 buf[C::RW * C::RS];
-  __shared__ float2 tw[M];
-  __shared__ float2 twr[M + 1];
+  __shared__ float2 twS[C::TWL_FWD ? M : 1];
+  __shared__ float2 twrS[C::TWL_FWD ? M + 1 : 1];
+  const float2* tw = C::TWL_FWD ? twS : twM;
+  const float2* twr = C::TWL_FWD ? twrS : wr;
   __shared__ int xmap[C::PX];
   __shared__ long long srcRow[C::RW];
   __shared__ size_t dstRow[C::RW];
-  x2_load_twiddles<M>(tw, twr, twM, wr);
+  if (C::TWL_FWD) x2_load_twiddles<M>(twS, twrS, twM, wr);
   const int nrows = d.P[1] * d.P[2];
   const int row0 = blockIdx.x * C::RW;
   float* fb = reinterpret_cast<float*>(buf);
@@ -326,13 +336,15 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
                                                            const float2* __restrict__ wr) {
   using C = XCfg<M>;
   __shared__ float2 buf[C::RW * C::RS];
-  __shared__ float2 tw[M];
-  __shared__ float2 twr[M + 1];
+  __shared__ float2 twS[C::TWL_RATIO ? M : 1];
+  __shared__ float2 twrS[C::TWL_RATIO ? M + 1 : 1];
+  const float2* tw = C::TWL_RATIO ? twS : twM;
+  const float2* twr = C::TWL_RATIO ? twrS : wr;
   // img index of FFT position x is imgRow + x (may be negative: x starts at m0); kOuterRow: ratio row of ones
   __shared__ long long imgRow[C::RW];
   __shared__ size_t inOff[C::RW];    // spectrum row to read (kNoRow: outer row, not read)
   __shared__ size_t outOff[C::RW];   // spectrum row to write
-  x2_load_twiddles<M>(tw, twr, twM, wr);
+  if (C::TWL_RATIO) x2_load_twiddles<M>(twS, twrS, twM, wr);
   const int nrows = d.P[1] * d.P[2];
   const int row0 = blockIdx.x * C::RW;
   if (threadIdx.x < C::RW) {
