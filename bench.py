@@ -10,8 +10,9 @@ weights, PSF spectra and psi resident in HBM.
 
 Printed: one JSON line with value = voxel-iterations/s (psi voxels x
 iterations / second) and, for the dominant kernel, its achieved HBM bandwidth
-from algorithmic bytes (HIP events around each of its launches in the timed
-region), plus a CPU baseline: the oracle restatement of the reference's CPU
+from algorithmic bytes (HIP start / stop events of each of its launches in the
+timed region, recorded by the kernel launch itself, so that they hold the
+kernel's own execution), plus a CPU baseline: the oracle restatement of the reference's CPU
 path timed on a bounded sample of the same workload on this host.
 """
 from __future__ import annotations

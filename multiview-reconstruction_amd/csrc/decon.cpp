@@ -345,6 +345,12 @@ int Decon::reset_psi() {
   return 0;
 }
 
+// geometry of a strided launch whose kernel execution is timed under category c (untimed when c is off)
+static StridedGeom timed(Timer& t, StridedGeom g, int c) {
+  t.kernel(c, &g.evStart, &g.evStop);
+  return g;
+}
+
 // One view of one Seq iteration over the whole psi volume.
 int Decon::view_seq(int v, double2* statOut) {
   const Dom& d = plan.dom;
@@ -356,30 +362,18 @@ int Decon::view_seq(int v, double2* statOut) {
   timer.begin(stream, PASS_X_FWD);
   plan.xf(d, g, S, plan.twM, plan.wr, stream);
   timer.end(stream);
-  timer.begin(stream, PASS_Z_FWD);
-  plan.sz(gz, 0, S, S, nullptr, 1.f, plan.twZ, stream);
-  timer.end(stream);
-  timer.begin(stream, PASS_Y_CONV);
-  plan.sy(gy, 2, S, S, spec1[v], 1.f, plan.twY, stream);
-  timer.end(stream);
-  timer.begin(stream, PASS_Z_INV);
-  plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
-  timer.end(stream);
+  plan.sz(timed(timer, gz, PASS_Z_FWD), 0, S, S, nullptr, 1.f, plan.twZ, stream);
+  plan.sy(timed(timer, gy, PASS_Y_CONV), 2, S, S, spec1[v], 1.f, plan.twY, stream);
+  plan.sz(timed(timer, gz, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
   RatioParams q{};
   q.img = img[v];
   for (int k = 0; k < 3; ++k) { q.N[k] = N[k]; q.o[k] = 0; }
   timer.begin(stream, PASS_X_RATIO);
   plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
   timer.end(stream);
-  timer.begin(stream, PASS_Z_FWD);
-  plan.sz(gz, 0, S, S, nullptr, 1.f, plan.twZ, stream);
-  timer.end(stream);
-  timer.begin(stream, PASS_Y_CONV);
-  plan.sy(gy, 2, S, S, spec2[v], 1.f, plan.twY, stream);
-  timer.end(stream);
-  timer.begin(stream, PASS_Z_INV);
-  plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
-  timer.end(stream);
+  plan.sz(timed(timer, gz, PASS_Z_FWD), 0, S, S, nullptr, 1.f, plan.twZ, stream);
+  plan.sy(timed(timer, gy, PASS_Y_CONV), 2, S, S, spec2[v], 1.f, plan.twY, stream);
+  plan.sz(timed(timer, gz, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
   UpdateParams u{};
   u.psi = psi;
   u.w = w[v];
@@ -410,32 +404,20 @@ int Decon::iteration_mul(double2* statOut) {
   timer.begin(stream, PASS_X_FWD);
   plan.xf(d, g, Spsi, plan.twM, plan.wr, stream);
   timer.end(stream);
-  timer.begin(stream, PASS_Z_FWD);
-  plan.sz(gz, 0, Spsi, Spsi, nullptr, 1.f, plan.twZ, stream);
-  timer.end(stream);
+  plan.sz(timed(timer, gz, PASS_Z_FWD), 0, Spsi, Spsi, nullptr, 1.f, plan.twZ, stream);
   std::vector<const float*> t2p(V), wp(V);
   for (int v = 0; v < V; ++v) {
-    timer.begin(stream, PASS_Y_CONV);
-    plan.sy(gy, 2, Spsi, S, spec1[v], 1.f, plan.twY, stream);
-    timer.end(stream);
-    timer.begin(stream, PASS_Z_INV);
-    plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
-    timer.end(stream);
+    plan.sy(timed(timer, gy, PASS_Y_CONV), 2, Spsi, S, spec1[v], 1.f, plan.twY, stream);
+    plan.sz(timed(timer, gz, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
     RatioParams q{};
     q.img = img[v];
     for (int k = 0; k < 3; ++k) { q.N[k] = N[k]; q.o[k] = 0; }
     timer.begin(stream, PASS_X_RATIO);
     plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
     timer.end(stream);
-    timer.begin(stream, PASS_Z_FWD);
-    plan.sz(gz, 0, S, S, nullptr, 1.f, plan.twZ, stream);
-    timer.end(stream);
-    timer.begin(stream, PASS_Y_CONV);
-    plan.sy(gy, 2, S, S, spec2[v], 1.f, plan.twY, stream);
-    timer.end(stream);
-    timer.begin(stream, PASS_Z_INV);
-    plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
-    timer.end(stream);
+    plan.sz(timed(timer, gz, PASS_Z_FWD), 0, S, S, nullptr, 1.f, plan.twZ, stream);
+    plan.sy(timed(timer, gy, PASS_Y_CONV), 2, S, S, spec2[v], 1.f, plan.twY, stream);
+    plan.sz(timed(timer, gz, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
     StoreParams st{T2 + nvox() * v};
     timer.begin(stream, PASS_X_STORE);
     plan.xs(d, st, S, plan.twM, plan.wr, stream);

@@ -76,6 +76,8 @@ struct StridedGeom {
   int ncols;           // valid columns (hx) per line
   int nplanes;
   int width = 0;       // tile width in columns (8 or 16; 0 = the default for the length)
+  // timing: when set, the launch records the kernel's own start and stop in these events
+  hipEvent_t evStart = nullptr, evStop = nullptr;
 };
 // mode: 0 forward, 1 inverse, 2 forward * spec * inverse (spec layout = data layout),
 // 3 forward then scale

@@ -78,6 +78,8 @@ struct Timer {
   size_t used = 0;
   void begin(hipStream_t s, int c);
   void end(hipStream_t s);
+  // a start / stop pair for a launch that records its own kernel's execution (both null when c is off)
+  void kernel(int c, hipEvent_t* start, hipEvent_t* stop);
   int collect(double* ms, long long* n);  // accumulates into arrays of PASS_COUNT
   void release();
   void reset(unsigned mask);  // enable the categories in mask (bit c = category c), clear totals

@@ -11,6 +11,8 @@
 // even/odd split); y and z passes work on tiles of W consecutive spectrum
 // columns so that every global access is a run of W*8 contiguous bytes.
 #pragma once
+#include <hip/hip_ext.h>
+
 #include "engine.hpp"
 #include "fft_reg.hpp"
 
@@ -222,16 +224,26 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W>())) void k_str
 }
 
 // ------------------------------------------------------------------ host launchers
+template <int L, int W, int MODE>
+void launch_strided2_mode(const StridedGeom& gm, const float2* in, float2* out, const float2* spec, float scale,
+                          const float2* tw, hipStream_t s) {
+  constexpr int NT = W * split2(L).A;
+  const dim3 g2((gm.ncols + W - 1) / W, gm.nplanes);
+  if (gm.evStart)  // timed: the events hold the kernel's own start and stop
+    hipExtLaunchKernelGGL((k_strided2<L, W, MODE>), g2, dim3(NT), 0, s, gm.evStart, gm.evStop, 0u, gm, in, out, spec,
+                          scale, tw);
+  else
+    hipLaunchKernelGGL((k_strided2<L, W, MODE>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw);
+}
+
 template <int L, int W>
 void launch_strided2(const StridedGeom& gm, int mode, const float2* in, float2* out, const float2* spec, float scale,
                      const float2* tw, hipStream_t s) {
-  constexpr int NT = W * split2(L).A;
-  const dim3 g2((gm.ncols + W - 1) / W, gm.nplanes);
   switch (mode) {
-    case 0: hipLaunchKernelGGL((k_strided2<L, W, 0>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
-    case 1: hipLaunchKernelGGL((k_strided2<L, W, 1>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
-    case 2: hipLaunchKernelGGL((k_strided2<L, W, 2>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
-    default: hipLaunchKernelGGL((k_strided2<L, W, 3>), g2, dim3(NT), 0, s, gm, in, out, spec, scale, tw); break;
+    case 0: return launch_strided2_mode<L, W, 0>(gm, in, out, spec, scale, tw, s);
+    case 1: return launch_strided2_mode<L, W, 1>(gm, in, out, spec, scale, tw, s);
+    case 2: return launch_strided2_mode<L, W, 2>(gm, in, out, spec, scale, tw, s);
+    default: return launch_strided2_mode<L, W, 3>(gm, in, out, spec, scale, tw, s);
   }
 }
 

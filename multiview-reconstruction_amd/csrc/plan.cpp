@@ -123,6 +123,19 @@ void Timer::begin(hipStream_t s, int c) {
   cat.push_back(c);
   used++;
 }
+void Timer::kernel(int c, hipEvent_t* start, hipEvent_t* stop) {
+  *start = *stop = nullptr;
+  if (!on || !((mask >> c) & 1u)) return;
+  if (used + 2 > ev.size()) {
+    size_t old = ev.size();
+    ev.resize(old + 256);
+    for (size_t i = old; i < ev.size(); ++i) (void)hipEventCreate(&ev[i]);
+  }
+  *start = ev[used];
+  *stop = ev[used + 1];
+  cat.push_back(c);
+  used += 2;
+}
 void Timer::end(hipStream_t s) {
   if (!pending) return;
   (void)hipEventRecord(ev[used], s);
