@@ -130,9 +130,10 @@ int mvr_decon_get_stats(mvr_decon* d, double* out, int cap);
 int mvr_decon_reset_psi(mvr_decon* d);
 /* FFT geometry: out[0..2] = FFT lengths, out[3] = half-spectrum row pitch */
 int mvr_decon_get_geometry(mvr_decon* d, int out[4]);
-/* per-pass GPU timing with HIP events recorded around each launch on the library's stream.  mask bit c
- * enables category c (0 x_fwd, 1 z_fwd, 2 y_conv, 3 z_inv, 4 x_ratio, 5 x_update, 6 x_store,
- * 7 update_mul, 8 reduce); 0 disables.  Totals are cleared by every call. */
+/* per-pass GPU timing on the library's stream: the strided passes (z_fwd, y_conv, z_inv) with start and stop
+ * events recorded by the launch itself (the kernel's own execution), the others with events recorded around
+ * the launch.  mask bit c enables category c (0 x_fwd, 1 z_fwd, 2 y_conv, 3 z_inv, 4 x_ratio, 5 x_update,
+ * 6 x_store, 7 update_mul, 8 reduce); 0 disables.  Totals are cleared by every call. */
 int mvr_decon_profile(mvr_decon* d, unsigned mask);
 int mvr_decon_pass_times(mvr_decon* d, double msTotal[9], long long launches[9]);
 void mvr_decon_destroy(mvr_decon* d);
