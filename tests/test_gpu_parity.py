@@ -282,6 +282,40 @@ def test_graph_replay_matches_eager(scheme):
     np.testing.assert_array_equal(out[0][2], out[1][2])
 
 
+@pytest.mark.parametrize("scheme", ["SEQ", "MUL"])
+def test_pass_timing_counts_and_leaves_results_unchanged(scheme):
+    # the timed strided launches carry start/stop events of their own kernel; the timed run must give the same
+    # psi and stats as an untimed one, and every pass must be counted once per launch with a positive time
+    inp = _small_problem(weights="zramp")
+    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+    V, its = 3, 2
+    out = []
+    for timed in (False, True):
+        d = hip.Decon(imgs.shape[1:], V, "EFFICIENT_BAYESIAN", scheme, 6e-4)
+        for v in range(V):
+            d.set_view(v, imgs[v], ws[v], ks[v])
+        d.init()
+        d.profile(None if timed else [])
+        d.run(its)
+        out.append((d.psi(), d.stats(), d.pass_times()))
+        d.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    t = out[1][2]
+    if scheme == "SEQ":  # per view update: x_fwd, z_fwd, y_conv, z_inv, x_ratio, z_fwd, y_conv, z_inv, x_update
+        want = {"x_fwd": 1, "z_fwd": 2, "y_conv": 2, "z_inv": 2, "x_ratio": 1, "x_update": 1, "reduce": 1}
+        want = {k: n * V * its for k, n in want.items()}
+    else:  # per iteration: shared x_fwd, z_fwd; per view y_conv, z_inv, x_ratio, z_fwd, y_conv, z_inv, x_store
+        want = {"x_fwd": its, "z_fwd": its * (1 + V), "y_conv": 2 * V * its, "z_inv": 2 * V * its,
+                "x_ratio": V * its, "x_store": V * its, "update_mul": its, "reduce": its}
+    for name, n in want.items():
+        ms, count = t[name]
+        assert count == n, (name, count, n)
+        assert 0.0 < ms < 1e4, (name, ms)
+    assert all(t[name][1] == 0 for name in hip.PASSES if name not in want)
+    assert all(c == 0 for _, c in out[0][2].values())
+
+
 def test_autotuned_lengths_keep_parity(oracle):
     inp = _small_problem(weights="zramp")
     imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
