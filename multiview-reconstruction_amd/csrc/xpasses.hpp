@@ -506,7 +506,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
     x2_park_inverse<M>(rb, q, z);
     const float* fb = reinterpret_cast<const float*>(buf);
     const unsigned total = (unsigned)(C::RW * DX);
-    constexpr int U = 4;
+    constexpr int U = 6;
     for (unsigned base = threadIdx.x; base < total; base += U * C::NT) {
       float last[U], wt[U];
       size_t gi[U];
