@@ -73,6 +73,20 @@ int mvr_fft_convolve(int dev, const float* img, const int nXYZ[3], int ext, floa
  * The reference picks its padded sizes inside imglib2's FFTMethods; this list has no counterpart there. */
 int mvr_fft_lengths(int* out, int cap);
 
+/* ---- DeconvolutionMethods on whole arrays (deconvolution/iteration/sequential/DeconvolutionMethods.java): the
+ * elementwise steps that the deconvolution passes fuse into their transforms, run alone over n values in flat order
+ * (same float and double rounding as the Java methods, no FMA contraction).  stats = {sumChange, maxChange} of
+ * IterationStatistics; sumChange is summed in double in the library's order (the reference sums per thread portion). */
+/* computeQuotient (:46-98): psiBlurred[i] = img[i] > 0 ? img[i] / psiBlurred[i] : 1 */
+int mvr_compute_quotient(int dev, const float* img, float* psiBlurred, long long n);
+/* computeFinalValues (:103-182) with computeNextValue (:320-358) and tikhonov (:421): psi updated in place */
+int mvr_compute_final_values(int dev, float* psi, const float* integral, const float* weight, long long n,
+                             double lambda, float minIntensity, float maxIntensity, double stats[2]);
+/* computeFinalValuesMul (:187-305) with computeNextValueMul (:370-419): numViews integrals and weights, n < 2^31 */
+int mvr_compute_final_values_mul(int dev, float* psi, int numViews, const float* const* integral,
+                                 const float* const* weight, long long n, double lambda, float minIntensity,
+                                 float maxIntensity, double stats[2]);
+
 /* ---- DeconViewPSF.init for all views in list order (deconvolution/DeconViewPSF.java:119-254,
  * deconvolution/DeconViews.java:69-70): kernel1 normalised to sum 1 (AdjustInput.normToSum1 with the
  * numeric thread count nT), kernel2 per PSF type; compound-kernel convolutions run on the GPU.
@@ -118,6 +132,9 @@ int mvr_decon_set_view(mvr_decon* d, int view, const float* img, const float* we
 int mvr_decon_init(mvr_decon* d, int psiInit, const float* psi0);
 /* runIterations: `iterations` more iterations */
 int mvr_decon_run(mvr_decon* d, int iterations);
+/* one view pass of MultiViewDeconvolutionSeq.runNextIteration (MultiViewDeconvolutionSeq.java:69-162): the update
+ * of psi by view `view` alone (Seq scheme only); its statistics are appended like a run's */
+int mvr_decon_run_view(mvr_decon* d, int view);
 int mvr_decon_get_psi(mvr_decon* d, float* out);
 /* PsiInit.getMax / getAvg */
 int mvr_decon_get_max(mvr_decon* d, float* maxOut, double* avgOut);

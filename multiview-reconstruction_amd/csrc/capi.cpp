@@ -243,6 +243,11 @@ int mvr_decon_run(mvr_decon* d, int iterations) {
   return d->d.run(iterations);
 }
 
+int mvr_decon_run_view(mvr_decon* d, int view) {
+  NULLCHECK(d);
+  return d->d.run_view(view);
+}
+
 int mvr_decon_get_psi(mvr_decon* d, float* out) {
   NULLCHECK(d);
   NULLCHECK(out);

@@ -1,7 +1,9 @@
 // decon.cpp -- resident multiview driver and per-block compute threads.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 
 #include "host.hpp"
 
@@ -230,28 +232,34 @@ int Decon::set_graph(bool on) {
 
 // Coordinate descent over the compiled FFT lengths of each axis.  A trial rebuilds the plan and view 0's
 // spectra and times view updates (Seq) or iterations (Mul) of view 0 with HIP events; psi is restored
-// from psi0 afterwards.
+// from psi0 afterwards.  On any failure the handle is put back on its previous schedule (plan, every view's
+// spectra, work buffers), so that later runs use a complete plan; if even that fails, run() refuses until the
+// next init.
 int Decon::autotune(int candidates, int chosen[3], double* msOut) {
   MVR_CHECK(ready, "init first");
   MVR_CHECK(candidates >= 1 && candidates <= 8, "candidates per axis must be in [1, 8]");
   MVR_HIP(hipSetDevice(dev));
   drop_graph();
-  if (ensure_partials((int)std::max<size_t>(2048, (size_t)N[1] * N[2] / 4 + 1))) return -1;
+  const int Vsave = V;
+  const int wSave[2] = {tunedW[0], tunedW[1]};
+  const int pSave[3] = {tunedP[0], tunedP[1], tunedP[2]};
   double2* st = nullptr;
-  MVR_HIP(hipMalloc(&st, sizeof(double2) * 4));
-  hipEvent_t e0, e1;
-  MVR_HIP(hipEventCreate(&e0));
-  MVR_HIP(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int best[3] = {0, 0, 0};
-  plan.destroy();
-  if (plan.create(N, K, nullptr)) return -1;
-  for (int d = 0; d < 3; ++d) best[d] = plan.dom.P[d];
   double bestMs = 1e30;
+  // a host-side failure injection for tests of this recovery: MVR_TEST_FAIL_AUTOTUNE_TRIAL=k fails trial k
+  const char* inj = std::getenv("MVR_TEST_FAIL_AUTOTUNE_TRIAL");
+  const int failTrial = inj ? std::atoi(inj) : -1;
+  int trialNo = 0;
   // a trial is the fastest of 3 separately timed view updates after a warm-up one (a transient slowdown of
   // the device does not pick the schedule); a candidate replaces the incumbent only when it is faster by
   // more than kMargin, so that timing noise does not trade a smaller grid for a larger one
   constexpr double kMargin = 0.01;
   auto trial = [&](const int P[3], double* ms) -> int {
+    if (trialNo++ == failTrial) {
+      set_error("autotune: injected trial failure");
+      return -1;
+    }
     if (build_plan_and_spectra(P, 1)) return -1;
     const bool timing = timer.on;
     timer.on = false;
@@ -279,12 +287,23 @@ int Decon::autotune(int candidates, int chosen[3], double* msOut) {
     *ms = best;
     return 0;
   };
-  const int Vsave = V;
-  if (scheme == SCHEME_MUL) V = 1;  // time the shared forward transform + one view
   int rc = 0;
-  const int wSave[2] = {tunedW[0], tunedW[1]};
-  tunedW[0] = tunedW[1] = 0;
-  if (trial(best, &bestMs)) rc = -1;
+  if (ensure_partials((int)std::max<size_t>(2048, (size_t)N[1] * N[2] / 4 + 1)) ||
+      hipMalloc(&st, sizeof(double2) * 4) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    if (st == nullptr || e0 == nullptr || e1 == nullptr) set_error("autotune: device allocation failed");
+    rc = -1;
+  }
+  if (!rc) {
+    plan.destroy();
+    if (plan.create(N, K, nullptr)) rc = -1;
+  }
+  if (!rc) {
+    for (int d = 0; d < 3; ++d) best[d] = plan.dom.P[d];
+    if (scheme == SCHEME_MUL) V = 1;  // time the shared forward transform + one view
+    tunedW[0] = tunedW[1] = 0;
+    if (trial(best, &bestMs)) rc = -1;
+  }
   for (int d = 0; d < 3 && !rc; ++d) {
     const int need = N[d] + K[d] - 1;
     int tried = 1;
@@ -321,14 +340,31 @@ int Decon::autotune(int candidates, int chosen[3], double* msOut) {
     }
     tunedW[a] = bestW;
   }
-  if (rc) { tunedW[0] = wSave[0]; tunedW[1] = wSave[1]; }
   V = Vsave;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(st);
-  if (rc) return -1;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipFree(st);
+  if (rc) {
+    // back to the schedule in use before the call (the reason of the failure is kept)
+    const std::string why = last_error();
+    tunedW[0] = wSave[0];
+    tunedW[1] = wSave[1];
+    for (int d = 0; d < 3; ++d) tunedP[d] = pSave[d];
+    if (build_plan_and_spectra(tunedP[0] ? tunedP : nullptr, V) ||
+        hipMemcpyAsync(psi, psi0, nvox() * 4, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+      ready = false;
+      set_error(why + "; restoring the previous schedule failed too: " + last_error() + " (init again)");
+      return -1;
+    }
+    set_error(why);
+    return -1;
+  }
   for (int d = 0; d < 3; ++d) tunedP[d] = best[d];
-  if (build_plan_and_spectra(tunedP, V)) return -1;
+  if (build_plan_and_spectra(tunedP, V)) {
+    ready = false;
+    return -1;
+  }
   MVR_HIP(hipMemcpyAsync(psi, psi0, nvox() * 4, hipMemcpyDeviceToDevice, stream));
   MVR_HIP(hipStreamSynchronize(stream));
   if (chosen) for (int d = 0; d < 3; ++d) chosen[d] = best[d];
@@ -499,6 +535,30 @@ int Decon::run(int iterations) {
   if (need > 0) MVR_HIP(hipMemcpyAsync(h.data(), statsDev, sizeof(double2) * need, hipMemcpyDeviceToHost, stream));
   MVR_HIP(hipStreamSynchronize(stream));
   for (int i = 0; i < need; ++i) { stats.push_back(h[i].x); stats.push_back(h[i].y); }
+  timer.collect(nullptr, nullptr);
+  return 0;
+}
+
+// One view pass of the Seq scheme (MultiViewDeconvolutionSeq.runNextIteration's view loop body,
+// D/MultiViewDeconvolutionSeq.java:69-162) on its own; the statistics are appended like a run's.
+int Decon::run_view(int v) {
+  MVR_CHECK(ready, "init first");
+  MVR_CHECK(scheme == SCHEME_SEQ, "single view updates belong to the Seq scheme");
+  MVR_CHECK(v >= 0 && v < V, "view index out of range");
+  MVR_HIP(hipSetDevice(dev));
+  if (statsCap < 1) {
+    if (statsDev) (void)hipFree(statsDev);
+    statsDev = nullptr;
+    MVR_HIP(hipMalloc(&statsDev, sizeof(double2)));
+    statsCap = 1;
+  }
+  if (ensure_partials((int)std::max<size_t>(2048, plan.rows_inner() / 4 + 1))) return -1;
+  if (view_seq(v, statsDev)) return -1;
+  double2 h;
+  MVR_HIP(hipMemcpyAsync(&h, statsDev, sizeof(double2), hipMemcpyDeviceToHost, stream));
+  MVR_HIP(hipStreamSynchronize(stream));
+  stats.push_back(h.x);
+  stats.push_back(h.y);
   timer.collect(nullptr, nullptr);
   return 0;
 }

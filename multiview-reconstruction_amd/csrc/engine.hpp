@@ -101,6 +101,9 @@ int launch_psi_init_partials(const float* const* imgs, int V, size_t n, double* 
 void launch_update_mul(float* psi, const float* const* T2, const float* const* w, int V, const int N[3],
                        const int o[3], const int D[3], double lambda, float minValue, float maxI, double2* partial,
                        int* nPartialOut, hipStream_t s);
+void launch_quotient(const float* img, float* blurred, size_t n, hipStream_t s);
+int launch_final_values(float* psi, const float* integral, const float* weight, size_t n, double lambda,
+                        float minValue, float maxI, double2* partial, int maxPartials, hipStream_t s);
 void launch_scale_copy_kernel_spectrum(const float2* in, float2* out, size_t n, float scale, hipStream_t s);
 void launch_rel_l2(const float* a, const float* b, size_t n, double2* partial, double2* out, hipStream_t s);
 int launch_psi_approx_slice(const float* img, const int N[3], double2* partial, hipStream_t s);

@@ -137,6 +137,7 @@ struct Decon {
   // psi0 := the psi currently resident (after an external initialiser wrote it)
   int commit_psi0();
   int run(int iterations);
+  int run_view(int v);
   int reset_psi();
   int set_reference(const float* r);
   int rel_l2(double* out);

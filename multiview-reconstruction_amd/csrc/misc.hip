@@ -17,14 +17,14 @@ void launch_fill(float* dst, size_t n, float v, hipStream_t s) {
 __global__ void k_reduce_stats(const double2* __restrict__ partial, int n, double2* __restrict__ out) {
   __shared__ double ss[kThreads], sm[kThreads];
   double s = 0.0, m = -1.0;
-  for (int i = threadIdx.x; i < n; i += kThreads) { s += partial[i].x; m = fmax(m, partial[i].y); }
+  for (int i = threadIdx.x; i < n; i += kThreads) { s += partial[i].x; m = java_max(m, partial[i].y); }
   ss[threadIdx.x] = s;
   sm[threadIdx.x] = m;
   __syncthreads();
   for (int off = kThreads / 2; off > 0; off >>= 1) {
     if (threadIdx.x < off) {
       ss[threadIdx.x] += ss[threadIdx.x + off];
-      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
+      sm[threadIdx.x] = java_max(sm[threadIdx.x], sm[threadIdx.x + off]);
     }
     __syncthreads();
   }
@@ -193,6 +193,8 @@ struct ViewPtrsW { const float* t2[kMaxViews]; const float* w[kMaxViews]; };
 __global__ __launch_bounds__(kThreads) void k_update_mul(float* __restrict__ psi, ViewPtrsW vp, int V, int3 N,
                                                          int3 o, int3 D, double lambda, float minI, float maxI,
                                                          double2* __restrict__ partial) {
+  // Java rounds every float and double operation: no FMA contraction in the update
+#pragma clang fp contract(off)
   const size_t nvox = (size_t)D.x * D.y * D.z;
   double sumc = 0.0, mxc = -1.0;
   const double invV = 1.0 / (double)V;
@@ -203,32 +205,18 @@ __global__ __launch_bounds__(kThreads) void k_update_mul(float* __restrict__ psi
     const size_t gi = ((size_t)(o.z + lz) * N.y + (o.y + ly)) * N.x + (o.x + lx);
     double sumW = 0.0, prod = 1.0;
     for (int v = 0; v < V; ++v) {
-      prod = __dmul_rn(prod, (double)vp.t2[v][i]);
-      sumW = __dadd_rn(sumW, (double)vp.w[v][gi]);
+      prod = prod * (double)vp.t2[v][i];
+      sumW = sumW + (double)vp.w[v][gi];
     }
     prod = pow(prod, invV);
     sumW = fmin(1.0, sumW);
     const float last = psi[gi];
-    const float value = __fmul_rn(last, (float)prod);
-    float adjusted;
-    if (value > 0.f) {
-      if (lambda > 0.0) {
-        const double vn = (double)__fdiv_rn(value, maxI);
-        const double t =
-            __ddiv_rn(__dadd_rn(__dsqrt_rn(__dadd_rn(1.0, __dmul_rn(__dmul_rn(2.0, lambda), vn))), -1.0), lambda);
-        adjusted = __fmul_rn((float)t, maxI);
-      } else {
-        adjusted = value;
-      }
-    } else {
-      adjusted = minI;
-    }
-    const float next = isnan(adjusted) ? minI : fmaxf(minI, adjusted);
-    const float res = __fadd_rn(last, __fmul_rn(__fsub_rn(next, last), (float)sumW));
+    const float next = rl_tikhonov_adjust(last * (float)prod, lambda, minI, maxI);
+    const float res = last + (next - last) * (float)sumW;
     psi[gi] = res;
-    const float ch = __fsub_rn(res, last);
+    const float ch = res - last;
     sumc += (double)ch;
-    mxc = fmax(mxc, (double)ch);
+    mxc = java_max(mxc, (double)ch);
   }
   block_reduce_stats(sumc, mxc, partial + blockIdx.x);
 }
@@ -243,6 +231,45 @@ void launch_update_mul(float* psi, const float* const* T2, const float* const* w
   hipLaunchKernelGGL(k_update_mul, dim3(nb), dim3(kThreads), 0, s, psi, vp, V, make_int3(N[0], N[1], N[2]),
                      make_int3(o[0], o[1], o[2]), make_int3(D[0], D[1], D[2]), lambda, minValue, maxI, partial);
   *nPartialOut = nb;
+}
+
+// DeconvolutionMethods.computeQuotient (D/iteration/sequential/DeconvolutionMethods.java:46-98) over a flat array:
+// blurred[i] = img[i] > 0 ? img[i] / blurred[i] : 1.
+__global__ void k_quotient(const float* __restrict__ img, float* __restrict__ blurred, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float im = img[i];
+    blurred[i] = im > 0.f ? im / blurred[i] : 1.f;
+  }
+}
+
+void launch_quotient(const float* img, float* blurred, size_t n, hipStream_t s) {
+  const int nb = (int)std::max<size_t>(1, std::min<size_t>((n + kThreads - 1) / kThreads, 8192));
+  hipLaunchKernelGGL(k_quotient, dim3(nb), dim3(kThreads), 0, s, img, blurred, n);
+}
+
+// DeconvolutionMethods.computeFinalValues (D/iteration/sequential/DeconvolutionMethods.java:103-182) over a flat
+// array: psi[i] = computeNextValue(psi[i], integral[i], weight[i]); per work-group {sum, max} of the change.
+__global__ __launch_bounds__(kThreads) void k_final_values(float* __restrict__ psi, const float* __restrict__ integral,
+                                                           const float* __restrict__ weight, size_t n, double lambda,
+                                                           float minI, float maxI, double2* __restrict__ partial) {
+  double sumc = 0.0, mxc = -1.0;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+    const float last = psi[i];
+    const float next = rl_next_value(last, integral[i], weight[i], lambda, minI, maxI);
+    psi[i] = next;
+    const float ch = next - last;
+    sumc += (double)ch;
+    mxc = java_max(mxc, (double)ch);
+  }
+  block_reduce_stats(sumc, mxc, partial + blockIdx.x);
+}
+
+int launch_final_values(float* psi, const float* integral, const float* weight, size_t n, double lambda,
+                        float minValue, float maxI, double2* partial, int maxPartials, hipStream_t s) {
+  const int nb = (int)std::max<size_t>(1, std::min<size_t>((n + kThreads - 1) / kThreads, (size_t)maxPartials));
+  hipLaunchKernelGGL(k_final_values, dim3(nb), dim3(kThreads), 0, s, psi, integral, weight, n, lambda, minValue, maxI,
+                     partial);
+  return nb;
 }
 
 // Distance of psi to a reference volume in FP64: per work-group {sum (a - b)^2, sum b^2},

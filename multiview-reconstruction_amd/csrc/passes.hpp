@@ -49,39 +49,48 @@ __device__ __forceinline__ float gather_value(const GatherParams& g, const Dom& 
 
 // ------------------------------------------------------------------ update of psi
 // DeconvolutionMethods.computeNextValue (D/iteration/sequential/DeconvolutionMethods.java:320-358),
-// float operations in Java order without contraction; Tikhonov in double.
-__device__ __forceinline__ float rl_next_value(float last, float integral, float weight, double lambda, float minI,
-                                               float maxI) {
-  const float value = __fmul_rn(last, integral);
+// float operations in Java order; Tikhonov in double.  Java rounds every float and double operation, so the
+// function body forbids FMA contraction (also where it is inlined).  Plain operators are used on purpose: the
+// __f*_rn helpers are header functions outside this pragma's scope, whose products the compiler may still fuse.
+__device__ __forceinline__ float rl_tikhonov_adjust(float value, double lambda, float minI, float maxI) {
+#pragma clang fp contract(off)
   float adjusted;
   if (value > 0.f) {
     if (lambda > 0.0) {
-      const double vn = (double)__fdiv_rn(value, maxI);
-      const double t = __ddiv_rn(__dadd_rn(__dsqrt_rn(__dadd_rn(1.0, __dmul_rn(__dmul_rn(2.0, lambda), vn))), -1.0),
-                                 lambda);
-      adjusted = __fmul_rn((float)t, maxI);
+      const double vn = (double)(value / maxI);
+      const double t = (__dsqrt_rn(1.0 + (2.0 * lambda) * vn) - 1.0) / lambda;
+      adjusted = (float)t * maxI;
     } else {
       adjusted = value;
     }
   } else {
     adjusted = minI;
   }
-  const float next = isnan(adjusted) ? minI : fmaxf(minI, adjusted);
-  return __fadd_rn(last, __fmul_rn(__fsub_rn(next, last), weight));
+  return isnan(adjusted) ? minI : fmaxf(minI, adjusted);
 }
+
+__device__ __forceinline__ float rl_next_value(float last, float integral, float weight, double lambda, float minI,
+                                               float maxI) {
+#pragma clang fp contract(off)
+  const float next = rl_tikhonov_adjust(last * integral, lambda, minI, maxI);
+  return last + (next - last) * weight;
+}
+
+// Math.max of Java on doubles for the maxChange statistic: a NaN change stays NaN (fmax would drop it)
+__device__ __forceinline__ double java_max(double a, double b) { return a != a ? a : b != b ? b : fmax(a, b); }
 
 __device__ __forceinline__ void block_reduce_stats(double s, double mx, double2* out) {
   __shared__ double rs[kThreads / 64], rm[kThreads / 64];
   for (int off = 32; off > 0; off >>= 1) {
     s += __shfl_down(s, off);
-    mx = fmax(mx, __shfl_down(mx, off));
+    mx = java_max(mx, __shfl_down(mx, off));
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) { rs[wv] = s; rm[wv] = mx; }
   __syncthreads();
   if (threadIdx.x == 0) {
     double a = 0.0, b = -1.0;
-    for (int i = 0; i < kThreads / 64; ++i) { a += rs[i]; b = fmax(b, rm[i]); }
+    for (int i = 0; i < kThreads / 64; ++i) { a += rs[i]; b = java_max(b, rm[i]); }
     *out = make_double2(a, b);
   }
 }

@@ -47,7 +47,7 @@ __device__ __forceinline__ void block_reduce_stats_nt(double s, double mx, doubl
   for (int off = P / 2; off > 0; off >>= 1) {
     if ((int)threadIdx.x < off) {
       rs[threadIdx.x] += rs[threadIdx.x + off];
-      rm[threadIdx.x] = fmax(rm[threadIdx.x], rm[threadIdx.x + off]);
+      rm[threadIdx.x] = java_max(rm[threadIdx.x], rm[threadIdx.x + off]);
     }
     __syncthreads();
   }
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
           psiRowPtr[lx[j]] = next;
           const float ch = __fsub_rn(next, last[j]);
           sum += (double)ch;
-          mx = fmax(mx, (double)ch);
+          mx = java_max(mx, (double)ch);
         }
       }
     }
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
         u.psi[gi[j]] = next;
         const float ch = __fsub_rn(next, last[j]);
         sum += (double)ch;
-        mx = fmax(mx, (double)ch);
+        mx = java_max(mx, (double)ch);
       }
     }
   }

@@ -55,6 +55,13 @@ def lib():
         L.mvr_convolution3d_fft_inplace.argtypes = [vp, _i32p, vp, _i32p, ctypes.c_int]
         L.mvr_fft_convolve.argtypes = [ctypes.c_int, vp, _i32p, ctypes.c_int, ctypes.c_float, vp, _i32p, vp]
         L.mvr_fft_lengths.argtypes = [vp, ctypes.c_int]
+        L.mvr_compute_quotient.argtypes = [ctypes.c_int, vp, vp, ctypes.c_longlong]
+        L.mvr_compute_final_values.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_longlong, ctypes.c_double,
+                                               ctypes.c_float, ctypes.c_float, _f64p]
+        L.mvr_compute_final_values_mul.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.POINTER(vp),
+                                                   ctypes.POINTER(vp), ctypes.c_longlong, ctypes.c_double,
+                                                   ctypes.c_float, ctypes.c_float, _f64p]
+        L.mvr_decon_run_view.argtypes = [vp, ctypes.c_int]
         L.mvr_init_psfs.argtypes = [ctypes.c_int, vp, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, vp, vp]
         L.mvr_block_thread_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _i32p, ctypes.c_float,
                                               ctypes.c_float, ctypes.c_int]
@@ -196,6 +203,34 @@ def init_psfs(kernels, psf_type, nT=8, dev=0):
     return k1, k2
 
 
+def compute_quotient(img, blurred, dev=0):
+    """DeconvolutionMethods.computeQuotient over a flat array (mvr_compute_quotient)."""
+    img = _f32(img).ravel()
+    out = _f32(blurred).ravel().copy()
+    _check(lib().mvr_compute_quotient(dev, _ptr(img), _ptr(out), out.size))
+    return out
+
+
+def compute_final_values(psi, integral, weight, lam, min_i, max_i, dev=0):
+    """DeconvolutionMethods.computeFinalValues: (new psi, (sumChange, maxChange))."""
+    out = _f32(psi).ravel().copy()
+    st = (ctypes.c_double * 2)()
+    _check(lib().mvr_compute_final_values(dev, _ptr(out), _ptr(_f32(integral).ravel()), _ptr(_f32(weight).ravel()),
+                                          out.size, float(lam), float(min_i), float(max_i), st))
+    return out, (st[0], st[1])
+
+
+def compute_final_values_mul(psi, integrals, weights, lam, min_i, max_i, dev=0):
+    """DeconvolutionMethods.computeFinalValuesMul: integrals / weights are [V, n]."""
+    out = _f32(psi).ravel().copy()
+    ii = [_f32(a).ravel() for a in integrals]
+    ww = [_f32(a).ravel() for a in weights]
+    st = (ctypes.c_double * 2)()
+    _check(lib().mvr_compute_final_values_mul(dev, _ptr(out), len(ii), _ptr_array(ii), _ptr_array(ww), out.size,
+                                              float(lam), float(min_i), float(max_i), st))
+    return out, (st[0], st[1])
+
+
 class BlockThread:
     """ComputeBlockSeqThread / ComputeBlockMulThread running on one GPU."""
 
@@ -277,6 +312,10 @@ class Decon:
     def run(self, iterations):
         _check(lib().mvr_decon_run(self.h, iterations))
 
+    def run_view(self, view):
+        """One Seq view update of psi (MultiViewDeconvolutionSeq's per-view pass)."""
+        _check(lib().mvr_decon_run_view(self.h, view))
+
     def psi(self):
         out = np.empty(self.shape, np.float32)
         _check(lib().mvr_decon_get_psi(self.h, _ptr(out)))
@@ -299,6 +338,8 @@ class Decon:
         buf = np.zeros(max(n, 1), np.float64)
         lib().mvr_decon_get_stats(self.h, buf.ctypes.data_as(_f64p), n)
         per = self.V if self.scheme == "SEQ" else 1
+        if n % (2 * per):        # single view updates (run_view) that do not fill whole iterations
+            return buf[:n].reshape(-1, 2)
         return buf[:n].reshape(-1, per, 2) if n else np.zeros((0, per, 2))
 
     def reset_psi(self):

@@ -417,6 +417,9 @@ static std::vector<T> conv_same_zero(const std::vector<T>& input, const int id[3
 // ---------------------------------------------------------------------------
 // DeconvolutionMethods (D/iteration/sequential/DeconvolutionMethods.java)
 // ---------------------------------------------------------------------------
+// Math.max(double, double): a NaN argument is the result (std::max would drop a NaN second argument)
+static inline double jmax(double a, double b) { return std::isnan(a) ? a : std::isnan(b) ? b : std::max(a, b); }
+
 static inline double tikhonov(double value, double lambda) {
   return (std::sqrt(1.0 + 2.0 * lambda * value) - 1.0) / lambda;
 }
@@ -716,14 +719,14 @@ static void block_iteration_seq(T* psiBlk, const int bd[3], const T* imgBlk, con
       psiBlk[i] = next;
       const T change = next - last;
       sumChange += (double)change;
-      maxChange = std::max(maxChange, (double)change);
+      maxChange = jmax(maxChange, (double)change);
     }
     ps[p] = sumChange;
     pm[p] = maxChange;
   }
   stats[0] = 0;
   stats[1] = -1;
-  for (size_t p = 0; p < portions.size(); ++p) { stats[0] += ps[p]; stats[1] = std::max(stats[1], pm[p]); }
+  for (size_t p = 0; p < portions.size(); ++p) { stats[0] += ps[p]; stats[1] = jmax(stats[1], pm[p]); }
 }
 
 // ComputeBlockMulThreadCPU.runIteration (D/iteration/mul/ComputeBlockMulThreadCPU.java:87-188)
@@ -762,14 +765,14 @@ static void block_iteration_mul(T* psiBlk, const int bd[3], const std::vector<st
       psiBlk[i] = next;
       const T change = next - last;
       sumChange += (double)change;
-      maxChange = std::max(maxChange, (double)change);
+      maxChange = jmax(maxChange, (double)change);
     }
     ps[p] = sumChange;
     pm[p] = maxChange;
   }
   stats[0] = 0;
   stats[1] = -1;
-  for (size_t p = 0; p < portions.size(); ++p) { stats[0] += ps[p]; stats[1] = std::max(stats[1], pm[p]); }
+  for (size_t p = 0; p < portions.size(); ++p) { stats[0] += ps[p]; stats[1] = jmax(stats[1], pm[p]); }
 }
 
 // ---------------------------------------------------------------------------
@@ -869,7 +872,7 @@ static int deconvolve(const Params& p, const float* const* imgs, const float* co
             block_iteration_seq(psiBlk.data(), bd, imgBlk.data(), wBlk.data(), k1f[v], k2f[v], (double)p.lambda,
                                 (T)p.minValue, (T)maxOut[v], p.nT, st);
             sumChange += st[0];
-            maxChange = std::max(maxChange, st[1]);
+            maxChange = jmax(maxChange, st[1]);
             if (totalBlocks == 1) paste_block(psi.data(), N, b, psiBlk.data());
             else curQ.push_back({bi, psiBlk});
           }
@@ -898,7 +901,7 @@ static int deconvolve(const Params& p, const float* const* imgs, const float* co
           block_iteration_mul(psiBlk.data(), bd, imgBlk, wBlk, k1f, k2f, (double)p.lambda, (T)p.minValue, maxOut,
                               p.nT, st);
           sumChange += st[0];
-          maxChange = std::max(maxChange, st[1]);
+          maxChange = jmax(maxChange, st[1]);
           if (totalBlocks == 1) paste_block(psi.data(), N, b, psiBlk.data());
           else curQ.push_back({bi, psiBlk});
         }
@@ -986,6 +989,56 @@ double oracle_next_value_f64(double last, double integral, double weight, double
 float oracle_next_value_mul_f32(float last, const float* integral, const float* weight, int nv, double lambda,
                                 float minI, float maxI) {
   return next_value_mul<float>(last, integral, weight, nv, lambda, minI, maxI);
+}
+
+// DeconvolutionMethods.computeQuotient / computeFinalValues / computeFinalValuesMul over flat arrays
+// (D/iteration/sequential/DeconvolutionMethods.java:46-98, 103-182, 187-305): the same per-value functions as
+// the block iterations, statistics summed per portion of divideIntoPortions(n, nT) as the block threads do.
+void oracle_compute_quotient_f32(const float* img, float* blurred, long long n) {
+  for (i64 i = 0; i < n; ++i) blurred[i] = img[i] > 0 ? img[i] / blurred[i] : 1.0f;
+}
+
+void oracle_compute_final_values_f32(float* psi, const float* integral, const float* weight, long long n,
+                                     double lambda, float minI, float maxI, int nT, double* stats) {
+  stats[0] = 0;
+  stats[1] = -1;
+  for (const Portion& p : divide_into_portions(n, nT)) {
+    double sumChange = 0, maxChange = -1;
+    for (i64 j = 0; j < p.len; ++j) {
+      const i64 i = p.start + j;
+      const float last = psi[i];
+      const float next = next_value<float>(last, integral[i], weight[i], lambda, minI, maxI);
+      psi[i] = next;
+      const float change = next - last;
+      sumChange += (double)change;
+      maxChange = jmax(maxChange, (double)change);
+    }
+    stats[0] += sumChange;
+    stats[1] = jmax(stats[1], maxChange);
+  }
+}
+
+// integral / weight: V arrays of n values back to back
+void oracle_compute_final_values_mul_f32(float* psi, int V, const float* integral, const float* weight, long long n,
+                                         double lambda, float minI, float maxI, int nT, double* stats) {
+  stats[0] = 0;
+  stats[1] = -1;
+  std::vector<float> iv(V), wv(V);
+  for (const Portion& p : divide_into_portions(n, nT)) {
+    double sumChange = 0, maxChange = -1;
+    for (i64 j = 0; j < p.len; ++j) {
+      const i64 i = p.start + j;
+      for (int v = 0; v < V; ++v) { iv[v] = integral[(i64)v * n + i]; wv[v] = weight[(i64)v * n + i]; }
+      const float last = psi[i];
+      const float next = next_value_mul<float>(last, iv.data(), wv.data(), V, lambda, minI, maxI);
+      psi[i] = next;
+      const float change = next - last;
+      sumChange += (double)change;
+      maxChange = jmax(maxChange, (double)change);
+    }
+    stats[0] += sumChange;
+    stats[1] = jmax(stats[1], maxChange);
+  }
 }
 
 // blocks: out array of 15 longs per block

@@ -83,6 +83,12 @@ def lib():
         L.oracle_next_value_mul_f32.argtypes = [ctypes.c_float, _f32p, _f32p, ctypes.c_int, ctypes.c_double,
                                                 ctypes.c_float, ctypes.c_float]
         L.oracle_next_value_mul_f32.restype = ctypes.c_float
+        L.oracle_compute_quotient_f32.argtypes = [_f32p, _f32p, ctypes.c_longlong]
+        L.oracle_compute_final_values_f32.argtypes = [_f32p, _f32p, _f32p, ctypes.c_longlong, ctypes.c_double,
+                                                      ctypes.c_float, ctypes.c_float, ctypes.c_int, _f64p]
+        L.oracle_compute_final_values_mul_f32.argtypes = [_f32p, ctypes.c_int, _f32p, _f32p, ctypes.c_longlong,
+                                                          ctypes.c_double, ctypes.c_float, ctypes.c_float,
+                                                          ctypes.c_int, _f64p]
         L.oracle_divide_into_blocks.argtypes = [_i64p, _i64p, _i64p, _i64p, ctypes.c_int]
         L.oracle_sort_blocks.argtypes = [_i64p, _i64p, _i64p, ctypes.c_int, _i32p, _i32p, _i32p]
         L.oracle_init_psfs_f32.argtypes = [_f32p, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, _f32p, _f32p]
@@ -181,6 +187,35 @@ def next_value_mul(last, integrals, weights, lam, min_i, max_i):
     i = np.ascontiguousarray(integrals, np.float32)
     w = np.ascontiguousarray(weights, np.float32)
     return np.float32(lib().oracle_next_value_mul_f32(last, i, w, i.size, lam, min_i, max_i))
+
+
+def compute_quotient(img, blurred):
+    """DeconvolutionMethods.computeQuotient over a flat array: img > 0 ? img / blurred : 1."""
+    img = np.ascontiguousarray(img, np.float32).ravel()
+    out = np.ascontiguousarray(blurred, np.float32).ravel().copy()
+    lib().oracle_compute_quotient_f32(img, out, out.size)
+    return out
+
+
+def compute_final_values(psi, integral, weight, lam, min_i, max_i, nT=8):
+    """DeconvolutionMethods.computeFinalValues over flat arrays: (new psi, (sumChange, maxChange))."""
+    out = np.ascontiguousarray(psi, np.float32).ravel().copy()
+    st = np.zeros(2, np.float64)
+    lib().oracle_compute_final_values_f32(out, np.ascontiguousarray(integral, np.float32).ravel(),
+                                          np.ascontiguousarray(weight, np.float32).ravel(), out.size, lam, min_i,
+                                          max_i, nT, st)
+    return out, (float(st[0]), float(st[1]))
+
+
+def compute_final_values_mul(psi, integrals, weights, lam, min_i, max_i, nT=8):
+    """DeconvolutionMethods.computeFinalValuesMul: integrals / weights are [V, n]."""
+    out = np.ascontiguousarray(psi, np.float32).ravel().copy()
+    i = np.ascontiguousarray(integrals, np.float32).reshape(-1, out.size)
+    w = np.ascontiguousarray(weights, np.float32).reshape(-1, out.size)
+    st = np.zeros(2, np.float64)
+    lib().oracle_compute_final_values_mul_f32(out, i.shape[0], i.ravel(), w.ravel(), out.size, lam, min_i, max_i, nT,
+                                              st)
+    return out, (float(st[0]), float(st[1]))
 
 
 def divide_into_blocks(img_xyz, kernel_size_xyz, block_size_xyz):

@@ -2,9 +2,10 @@
 
 CPU: the oracle still reproduces the committed small fixture.
 GPU: the HIP library reproduces the small fixture and the full-size workloads
-(128^3 x 3 views, 10 iterations; 256^3 x 6 views, 20 iterations) within the
-parity bar (rel-L2 <= 1e-4 against the oracle's FP32 result, also reported and
-bounded against its FP64 result).
+(128^3 x 3 views, 10 iterations; 256^3 x 6 views, 20 iterations; the uneven-view
+frame against the blocked oracle) within rel-L2 1e-5 of the oracle's FP32 result
+(also bounded against its FP64 result).  The parity bar of the path is 1e-4; the
+runs reach 5e-7 to 1.1e-6, so 1e-5 catches a regression long before that bar.
 """
 import os
 
@@ -95,11 +96,11 @@ def test_hip_full_size_workload(name):
         e64 = rel(sub, z["sub_f64"])
         print(f"{name}: subsampled rel-L2 vs oracle FP32 {e32:.2e}, vs FP64 {e64:.2e}; "
               f"oracle FP32 vs FP64 {rel(z['sub_f32'], z['sub_f64']):.2e}")
-        assert e64 < 1e-4
+        assert e64 < 1e-5, f"rel-L2 vs FP64 {e64:.2e} (bar 1e-5, spec 1e-4)"
     else:
         # the oracle ran the reference's blocked driver (block_size); the GPU runs one resident domain
         print(f"{name}: subsampled rel-L2 vs oracle FP32 (blocks {z['block_size']}) {e32:.2e}")
-    assert e32 < 1e-4
+    assert e32 < 1e-5, f"rel-L2 vs FP32 {e32:.2e} (bar 1e-5, spec 1e-4)"
     tag = "f64" if "sum_f64" in z.files else "f32"
     assert psi.sum() == pytest.approx(float(z[f"sum_{tag}"][0]), rel=1e-5)
     assert np.linalg.norm(psi) == pytest.approx(float(z[f"norm_{tag}"][0]), rel=1e-5)

@@ -1,9 +1,11 @@
 """Parity of the HIP library (called through its C ABI) with the CPU oracle.
 
-Tolerances: FP32 results within rel-L2 1e-4 of the oracle's FP32 restatement
-after 10 iterations (the parity bar of the path); single FFT convolutions and
-one block iteration within 1e-5; every deconvolution also checked against the
-FP64 oracle.
+Tolerances: the parity bar of the path is FP32 results within rel-L2 1e-4 of
+the oracle's FP32 restatement after 10 iterations (SPEC).  The deconvolution
+cases reach 1e-7 to 1e-6, so they assert BAR = 1e-5: a regression by an order of
+magnitude fails long before the spec bar is crossed.  Single FFT convolutions
+within 3e-6 and one block iteration within 1e-5; every deconvolution is also
+checked against the FP64 oracle.
 """
 import numpy as np
 import pytest
@@ -14,6 +16,8 @@ from mvr import synthetic as syn
 pytestmark = pytest.mark.gpu
 
 RNG = np.random.default_rng(2024)
+SPEC = 1e-4   # parity bar of the path (rel-L2 after 10 iterations)
+BAR = 1e-5    # what the deconvolution cases below reach, with an order of magnitude to spare
 
 
 def rel(a, b):
@@ -87,7 +91,7 @@ def test_resident_decon_long_axes(oracle, shape):
     d.close()
     print(f"{shape}: FFT grid {P}, rel-L2 vs oracle {e:.2e}")
     assert max(P) >= 1152
-    assert e < 1e-4
+    assert e < BAR, f"rel-L2 {e:.2e} (bar {BAR}, spec {SPEC})"
 
 
 @pytest.mark.parametrize("px", [720, 800, 864, 900, 960, 1080])
@@ -112,7 +116,7 @@ def test_resident_decon_long_x_rows(oracle, px):
     d.close()
     print(f"{shape}: FFT grid {P}, rel-L2 vs oracle {e:.2e}")
     assert P[0] == px
-    assert e < 1e-4
+    assert e < BAR, f"rel-L2 {e:.2e} (bar {BAR}, spec {SPEC})"
 
 
 def test_inplace_convolution_is_mirror_convolution(oracle):
@@ -209,8 +213,8 @@ def test_resident_decon_matches_oracle(oracle, psf_type, scheme, lam):
     e32, e64 = rel(psi, ref["psi"]), rel(psi, ref64["psi"])
     print(f"{psf_type} {scheme} lam={lam}: rel-L2 vs oracle FP32 {e32:.2e}, vs FP64 {e64:.2e}, "
           f"oracle FP32 vs FP64 {rel(ref['psi'], ref64['psi']):.2e}")
-    assert e32 < 1e-4
-    assert e64 < 1e-4
+    assert e32 < BAR, f"rel-L2 {e32:.2e} (bar {BAR}, spec {SPEC})"
+    assert e64 < BAR, f"rel-L2 {e64:.2e} (bar {BAR}, spec {SPEC})"
     assert d.stats().shape == ((it, 3, 2) if scheme == "SEQ" else (it, 1, 2))
     d.close()
 
@@ -234,7 +238,7 @@ def test_resident_decon_edge_shapes(oracle, shape, kshape, psf_type, scheme):
     d.run(5)
     e = rel(d.psi(), ref["psi"])
     print(f"{shape} K {kshape} {psf_type} {scheme}: rel-L2 vs oracle {e:.2e}, fft grid {d.geometry()['P']}")
-    assert e < 1e-4
+    assert e < BAR, f"rel-L2 {e:.2e} (bar {BAR}, spec {SPEC})"
     d.close()
 
 
@@ -255,7 +259,7 @@ def test_resident_decon_degenerate_sizes(oracle, shape, kshape, scheme):
     d.run(4)
     e = rel(d.psi(), ref["psi"])
     print(f"{shape} K {kshape} {scheme}: rel-L2 vs oracle {e:.2e}")
-    assert e < 1e-4
+    assert e < BAR, f"rel-L2 {e:.2e} (bar {BAR}, spec {SPEC})"
     d.close()
 
 
@@ -316,25 +320,54 @@ def test_pass_timing_counts_and_leaves_results_unchanged(scheme):
     assert all(c == 0 for _, c in out[0][2].values())
 
 
-def test_autotuned_lengths_keep_parity(oracle):
+@pytest.mark.parametrize("scheme", ["SEQ", "MUL"])
+def test_autotuned_lengths_keep_parity(oracle, scheme):
+    # the Mul tuner times one view (V swapped to 1 during the trials), then rebuilds T2 and every view's spectra
     inp = _small_problem(weights="zramp")
     imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
-    ref = oracle.deconvolve(imgs, ws, ks, psf_type="OPTIMIZATION_II", lam=6e-4, iterations=5)
-    d = hip.Decon(imgs.shape[1:], 3, "OPTIMIZATION_II", "SEQ", 6e-4)
+    ref = oracle.deconvolve(imgs, ws, ks, psf_type="OPTIMIZATION_II", scheme=scheme, lam=6e-4, iterations=5)
+    d = hip.Decon(imgs.shape[1:], 3, "OPTIMIZATION_II", scheme, 6e-4)
     for v in range(3):
         d.set_view(v, imgs[v], ws[v], ks[v])
     d.init()
     P, ms = d.autotune(3)
     need = [n + k - 1 for n, k in zip(imgs.shape[1:][::-1], ks.shape[1:][::-1])]
     tw = d.geometry()["tile_widths_yz"]
-    print(f"autotuned FFT lengths {P} (need >= {need}), tile widths y/z {tw}, {ms:.3f} ms per view update")
+    print(f"{scheme}: autotuned FFT lengths {P} (need >= {need}), tile widths y/z {tw}, {ms:.3f} ms per update")
     assert all(w in (0, 8, 16) for w in tw)
     assert all(p >= q for p, q in zip(P, need))
     assert d.geometry()["P"] == P
     d.run(5)
-    assert rel(d.psi(), ref["psi"]) < 1e-4
+    e = rel(d.psi(), ref["psi"])
+    assert e < BAR, f"rel-L2 {e:.2e} (bar {BAR}, spec {SPEC})"
     d.init()  # a later init keeps the tuned lengths
     assert d.geometry()["P"] == P
+    d.run(5)
+    e = rel(d.psi(), ref["psi"])
+    assert e < BAR, f"rel-L2 {e:.2e} after a later init"
+    d.close()
+
+
+@pytest.mark.parametrize("scheme,fail_at", [("SEQ", 2), ("MUL", 1), ("MUL", 4)])
+def test_failed_autotune_keeps_previous_schedule(oracle, monkeypatch, scheme, fail_at):
+    # a trial that fails (injected from the host) must leave a complete plan: every view's spectra, T2 for all
+    # views (Mul), the previous FFT lengths, and psi at its initial estimate
+    inp = _small_problem(weights="zramp")
+    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+    ref = oracle.deconvolve(imgs, ws, ks, psf_type="EFFICIENT_BAYESIAN", scheme=scheme, lam=6e-4, iterations=3)
+    d = hip.Decon(imgs.shape[1:], 3, "EFFICIENT_BAYESIAN", scheme, 6e-4)
+    for v in range(3):
+        d.set_view(v, imgs[v], ws[v], ks[v])
+    d.init()
+    geo = d.geometry()
+    monkeypatch.setenv("MVR_TEST_FAIL_AUTOTUNE_TRIAL", str(fail_at))
+    with pytest.raises(hip.MvrError, match="injected"):
+        d.autotune(3)
+    monkeypatch.delenv("MVR_TEST_FAIL_AUTOTUNE_TRIAL")
+    assert d.geometry() == geo
+    d.run(3)
+    e = rel(d.psi(), ref["psi"])
+    assert e < BAR, f"rel-L2 {e:.2e} after a failed autotune"
     d.close()
 
 
@@ -373,7 +406,8 @@ def test_psi_init_avg_approx(oracle):
     ref = oracle.deconvolve(imgs, ws, ks, iterations=3, psi0=d.psi())
     d.run(3)
     # the oracle driver takes max[] from AvgPrecise; with lambda 0 max[] does not enter the update
-    assert rel(d.psi(), ref["psi"]) < 1e-4
+    e = rel(d.psi(), ref["psi"])
+    assert e < BAR, f"rel-L2 {e:.2e} (bar {BAR}, spec {SPEC})"
     d.close()
 
 
@@ -395,7 +429,8 @@ def test_psi_init_blurred_fused(oracle):
     assert gavg == pytest.approx(avg, rel=1e-12)
     out = oracle.deconvolve(imgs, ws, ks, psf_type="OPTIMIZATION_II", lam=6e-4, iterations=10, psi0=psi)
     d.run(10)
-    assert rel(d.psi(), out["psi"]) < 1e-4
+    e = rel(d.psi(), out["psi"])
+    assert e < BAR, f"rel-L2 {e:.2e} (bar {BAR}, spec {SPEC})"
     d.close()
 
 
@@ -409,7 +444,8 @@ def test_resident_psi_init_from_array(oracle):
         d.set_view(v, imgs[v], ws[v], ks[v])
     d.init(psi0)
     d.run(3)
-    assert rel(d.psi(), ref["psi"]) < 1e-4
+    e = rel(d.psi(), ref["psi"])
+    assert e < BAR, f"rel-L2 {e:.2e} (bar {BAR}, spec {SPEC})"
     d.reset_psi()
     np.testing.assert_array_equal(d.psi(), psi0)
     d.close()
@@ -428,7 +464,8 @@ def test_blocked_driver_matches_oracle(oracle, scheme):
     dv = decon.DeconViews(views, "EFFICIENT_BAYESIAN")
     m = decon.MultiViewDeconvolution(dv, 4, decon.ComputeBlockThreadHIPFactory(bs, 6e-4), scheme)
     m.run_iterations()
-    assert rel(m.psi, ref["psi"]) < 1e-4
+    e = rel(m.psi, ref["psi"])
+    assert e < BAR, f"rel-L2 {e:.2e} (bar {BAR}, spec {SPEC})"
     m.close()
 
 

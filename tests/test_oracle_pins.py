@@ -241,6 +241,95 @@ def test_next_value_f64_tracks_f32(oracle):
         assert float(a) == pytest.approx(b, rel=1e-6)
 
 
+def next_value_numpy(last, integral, weight, lam, min_i, max_i):
+    """computeNextValue (DeconvolutionMethods.java:320-358) vectorised in numpy: every float32 and float64
+    operation rounds on its own, as Java's do."""
+    last, integral, weight = (np.asarray(a, np.float32) for a in (last, integral, weight))
+    m, mx = np.float32(min_i), np.float32(max_i)
+    with np.errstate(all="ignore"):
+        value = last * integral
+        if lam > 0:
+            vn = (value / mx).astype(np.float64)
+            adj = ((np.sqrt(1.0 + (2.0 * lam) * vn) - 1.0) / lam).astype(np.float32) * mx
+        else:
+            adj = value
+        adj = np.where(value > 0, adj, m).astype(np.float32)
+        nxt = np.where(np.isnan(adj), m, np.maximum(m, adj)).astype(np.float32)
+        return last + (nxt - last) * weight
+
+
+def update_inputs(n, seed):
+    """psi, integral, weight with the special values of the update: integral <= 0, NaN, inf and tiny; weights 0
+    and 1; psi at minValue."""
+    rng = np.random.default_rng(seed)
+    last = rng.uniform(1e-4, 800.0, n).astype(np.float32)
+    last[::97] = np.float32(1e-4)
+    integral = rng.uniform(0.2, 3.0, n).astype(np.float32)
+    integral[::13] = 0.0
+    integral[::17] = -1.5
+    integral[::19] = np.nan
+    integral[::23] = np.inf
+    integral[::29] = np.float32(1e-30)
+    weight = rng.uniform(0.0, 1.0, n).astype(np.float32)
+    weight[::7] = 0.0
+    weight[::11] = 1.0
+    return last, integral, weight
+
+
+@pytest.mark.parametrize("lam,max_i", [(0.0, 300.0), (6e-4, 300.0), (0.006, 1234.5), (6e-4, 0.0)])
+def test_final_values_array_matches_numpy_restatement(oracle, lam, max_i):
+    last, integral, weight = update_inputs(5003, 3)
+    got, (s, mx) = oracle.compute_final_values(last, integral, weight, lam, 1e-4, max_i, nT=8)
+    want = next_value_numpy(last, integral, weight, lam, 1e-4, max_i)
+    np.testing.assert_array_equal(got, want)
+    # an infinite integral with weight 0 gives NaN (inf * 0); Java's Math.max keeps a NaN maxChange
+    ch = (want - last).astype(np.float64)
+    assert np.isnan(ch).any() == np.isnan(mx) == np.isnan(s)
+    assert np.isnan(ch).any() or max_i == 0.0   # with maxIntensity 0 every positive value becomes minValue
+    fin = np.isfinite(integral)
+    got, (s, mx) = oracle.compute_final_values(last[fin], integral[fin], weight[fin], lam, 1e-4, max_i, nT=8)
+    ch = (got - last[fin]).astype(np.float64)
+    assert mx == ch.max()
+    assert s == pytest.approx(ch.sum(), rel=1e-12, abs=1e-9)
+    # scalar entry point agrees with the array one
+    for i in range(0, fin.sum(), 251):
+        assert oracle.next_value(last[fin][i], integral[fin][i], weight[fin][i], lam, 1e-4, max_i) == got[i]
+
+
+def test_final_values_mul_array_matches_numpy_restatement(oracle):
+    rng = np.random.default_rng(4)
+    n, V = 3001, 3
+    last = rng.uniform(1e-4, 500.0, n).astype(np.float32)
+    integ = rng.uniform(0.3, 3.0, (V, n)).astype(np.float32)
+    integ[1, ::31] = 0.0
+    w = rng.uniform(0.0, 0.6, (V, n)).astype(np.float32)
+    got, _ = oracle.compute_final_values_mul(last, integ, w, 6e-4, 1e-4, 250.0)
+    prod = np.ones(n)
+    sw = np.zeros(n)
+    for v in range(V):                       # Java multiplies and adds view by view, in double
+        prod = prod * integ[v].astype(np.float64)
+        sw = sw + w[v].astype(np.float64)
+    prod = np.power(prod, 1.0 / V)
+    sw = np.minimum(1.0, sw)
+    with np.errstate(all="ignore"):
+        value = last * prod.astype(np.float32)
+        vn = (value / np.float32(250.0)).astype(np.float64)
+        adj = ((np.sqrt(1.0 + (2.0 * 6e-4) * vn) - 1.0) / 6e-4).astype(np.float32) * np.float32(250.0)
+        adj = np.where(value > 0, adj, np.float32(1e-4)).astype(np.float32)
+        nxt = np.maximum(np.float32(1e-4), adj)
+        want = last + (nxt - last) * sw.astype(np.float32)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_quotient_array(oracle):
+    rng = np.random.default_rng(5)
+    img = rng.uniform(-2.0, 50.0, 4000).astype(np.float32)
+    img[::9] = 0.0
+    blurred = rng.uniform(0.1, 40.0, 4000).astype(np.float32)
+    got = oracle.compute_quotient(img, blurred)
+    np.testing.assert_array_equal(got, np.where(img > 0, img / blurred, np.float32(1.0)).astype(np.float32))
+
+
 # ----------------------------------------------------------------- blocks (BlockGeneratorFixedSizePrecise, BlockSorter)
 def test_blocks_reference_main_inputs(oracle):
     """Block.main (C/Block.java:421-436): 1024^2 image, kernel 16x32, block 384^2 (z kept trivial)."""
