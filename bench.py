@@ -308,27 +308,35 @@ def golden_parity(d, psi, w, workload):
 
 
 def time_block_path(imgs, ws, ks, w, iterations):
-    """The same workload through the per-block drop-in (mvr_block_run_iteration_seq per block and view, host
-    buffers), driven like MultiViewDeconvolutionSeq with the reference's default block (volume + kernel size
-    - 1 per axis): PCIe copies and the host's block copies are inside the time."""
+    """The same workload through the per-block drop-in (one mvr_block_run_iteration_seq* call per block and view),
+    driven like MultiViewDeconvolutionSeq with the reference's default block (volume + kernel size - 1 per axis):
+    PCIe copies and the host driver's block copies are inside the time.  Two variants: the view data resident
+    on the device (uploaded once, blocks cut there; only psi crosses the bus per call) and every image and weight
+    block cut on the host and sent per call, as the reference's drivers hand them over."""
     from mvr import decon as MD
     V = w.n_views
-    views = [MD.DeconView(imgs[v], ws[v], ks[v]) for v in range(V)]
-    dv = MD.DeconViews(views, w.psf_type)
-    m = MD.MultiViewDeconvolution(dv, iterations + 1, MD.ComputeBlockThreadHIPFactory(views[0].block_size, w.lam),
-                                  "SEQ")
-    m.run_next_iteration()  # warm-up: device buffers, kernel spectra
-    t0 = time.perf_counter()
-    for _ in range(iterations):
-        m.run_next_iteration()
-    el = time.perf_counter() - t0
-    m.close()
     nvox = imgs[0].size
-    log(f"per-block drop-in path: {el * 1e3 / iterations:.1f} ms per iteration")
-    return {"value": nvox * iterations / el, "unit": "voxel-iters/s", "ms_per_iteration": el * 1e3 / iterations,
-            "iterations": iterations, "block_xyz": list(views[0].block_size), "blocks_per_view": views[0].num_blocks,
-            "note": "per-block drop-in ABI with host buffers (one mvr_block_run_iteration_seq per block and view); "
-                    "host-device copies and the host driver's block copies are inside the time"}
+    out = {"iterations": iterations, "unit": "voxel-iters/s",
+           "note": "per-block drop-in ABI with host buffers; host-device copies and the host driver's block "
+                   "copies are inside the time"}
+    for resident in (True, False):
+        views = [MD.DeconView(imgs[v], ws[v], ks[v]) for v in range(V)]
+        dv = MD.DeconViews(views, w.psf_type)
+        m = MD.MultiViewDeconvolution(dv, iterations + 1, MD.ComputeBlockThreadHIPFactory(views[0].block_size, w.lam),
+                                      "SEQ", resident_views=resident)
+        m.run_next_iteration()  # warm-up: device buffers, kernel spectra
+        t0 = time.perf_counter()
+        for _ in range(iterations):
+            m.run_next_iteration()
+        el = time.perf_counter() - t0
+        m.close()
+        key = "resident_views" if resident else "host_blocks"
+        log(f"per-block drop-in path ({key}): {el * 1e3 / iterations:.1f} ms per iteration")
+        out[key] = {"value": nvox * iterations / el, "ms_per_iteration": el * 1e3 / iterations}
+        out["block_xyz"] = list(views[0].block_size)
+        out["blocks_per_view"] = views[0].num_blocks
+    out["value"] = out["resident_views"]["value"]
+    return out
 
 
 def cpu_block_inputs(inp, w, K):

@@ -113,6 +113,19 @@ int mvr_block_run_iteration_seq(mvr_block_thread* t, int view, float* psiBlock, 
 /* ComputeBlockMulThread.runIteration (iteration/mul/ComputeBlockMulThread.java:56-62) over views 0..numViews-1 */
 int mvr_block_run_iteration_mul(mvr_block_thread* t, int numViews, float* psiBlock, const float* const* imgBlocks,
                                 const float* const* weightBlocks, const float* maxIntensityViews, double stats[2]);
+/* The view's whole image and weights (nXYZ, the same for every view), kept on the device for the thread's
+ * lifetime.  The reference's drivers cut a zero-extended image and weight block from them for every block in every
+ * iteration (MultiViewDeconvolutionSeq.java:115-126, MultiViewDeconvolutionMul.java:173-183); with the data
+ * resident, the _at entry points cut those blocks on the device and only psiBlock crosses the bus. */
+int mvr_block_thread_set_view_data(mvr_block_thread* t, int view, const float* img, const float* weight,
+                                   const int nXYZ[3]);
+/* mvr_block_run_iteration_seq with imgBlock / weightBlock = the zero-extended view data over the block whose first
+ * voxel is at volume coordinate offsetXYZ (the Block interval min, cuda/Block.java:91-133; may be negative) */
+int mvr_block_run_iteration_seq_at(mvr_block_thread* t, int view, float* psiBlock, const long long offsetXYZ[3],
+                                   float maxIntensityView, double stats[2]);
+/* mvr_block_run_iteration_mul with the blocks of views 0..numViews-1 cut from their resident data */
+int mvr_block_run_iteration_mul_at(mvr_block_thread* t, int numViews, float* psiBlock, const long long offsetXYZ[3],
+                                   const float* maxIntensityViews, double stats[2]);
 void mvr_block_thread_destroy(mvr_block_thread* t);
 
 /* ---- resident deconvolution: the whole MultiViewDeconvolution{Seq,Mul} run on one GPU
@@ -128,8 +141,17 @@ int mvr_decon_create(mvr_decon** out, int dev, const int psiDimXYZ[3], int numVi
 /* DeconView(image, weight, kernel) (deconvolution/DeconView.java:118-184) */
 int mvr_decon_set_view(mvr_decon* d, int view, const float* img, const float* weight, const float* kernel,
                        const int kXYZ[3]);
+/* DeconViewPSF.getKernel1 / getKernel2 (deconvolution/DeconViewPSF.java:76-77) of a view whose DeconViewPSF.init
+ * already ran on the caller's side (the DeconViews ctor does it, DeconViews.java:69-70; kernel1 is then
+ * normalised in place): when every view has them, mvr_decon_init uses them as they are instead of running
+ * DeconViewPSF.init on the raw kernels of mvr_decon_set_view. */
+int mvr_decon_set_view_kernels(mvr_decon* d, int view, const float* kernel1, const float* kernel2,
+                               const int kXYZ[3]);
 /* DeconViewPSF.init for all views + PsiInit.runInitialization (MultiViewDeconvolution.java:102-135) */
 int mvr_decon_init(mvr_decon* d, int psiInit, const float* psi0);
+/* PsiInit.getMax of an initialiser that ran on the caller's side (MultiViewDeconvolution.java:124-134), after
+ * mvr_decon_init(MVR_PSI_INIT_FROM_ARRAY, psi): the per-view maxima that normalise the Tikhonov term */
+int mvr_decon_set_max(mvr_decon* d, const float* maxPerView);
 /* runIterations: `iterations` more iterations */
 int mvr_decon_run(mvr_decon* d, int iterations);
 /* one view pass of MultiViewDeconvolutionSeq.runNextIteration (MultiViewDeconvolutionSeq.java:69-162): the update

@@ -198,6 +198,24 @@ int mvr_block_run_iteration_mul(mvr_block_thread* t, int nv, float* psi, const f
   return t->t.run_mul(nv, psi, imgs, ws, maxIs, stats);
 }
 
+int mvr_block_thread_set_view_data(mvr_block_thread* t, int view, const float* img, const float* weight,
+                                   const int nXYZ[3]) {
+  NULLCHECK(t);
+  return t->t.set_view_data(view, img, weight, nXYZ);
+}
+
+int mvr_block_run_iteration_seq_at(mvr_block_thread* t, int view, float* psiBlock, const long long offsetXYZ[3],
+                                   float maxIntensityView, double stats[2]) {
+  NULLCHECK(t);
+  return t->t.run_seq_at(view, psiBlock, offsetXYZ, maxIntensityView, stats);
+}
+
+int mvr_block_run_iteration_mul_at(mvr_block_thread* t, int numViews, float* psiBlock, const long long offsetXYZ[3],
+                                   const float* maxIntensityViews, double stats[2]) {
+  NULLCHECK(t);
+  return t->t.run_mul_at(numViews, psiBlock, offsetXYZ, maxIntensityViews, stats);
+}
+
 void mvr_block_thread_destroy(mvr_block_thread* t) {
   if (!t) return;
   t->t.destroy();
@@ -226,6 +244,17 @@ int mvr_decon_set_view(mvr_decon* d, int v, const float* img, const float* w, co
   NULLCHECK(d);
   NULLCHECK(kd);
   return d->d.set_view(v, img, w, k, kd);
+}
+
+int mvr_decon_set_view_kernels(mvr_decon* d, int view, const float* kernel1, const float* kernel2,
+                               const int kXYZ[3]) {
+  NULLCHECK(d);
+  return d->d.set_view_kernels(view, kernel1, kernel2, kXYZ);
+}
+
+int mvr_decon_set_max(mvr_decon* d, const float* maxPerView) {
+  NULLCHECK(d);
+  return d->d.set_max(maxPerView);
 }
 
 int mvr_decon_init(mvr_decon* d, int psiInit, const float* psi0) {

@@ -40,6 +40,8 @@ int Decon::create(int dev_, const int N_[3], int V_, int psfType_, int scheme_, 
   img.assign(V, nullptr);
   w.assign(V, nullptr);
   kraw.assign(V, {});
+  kfin1.assign(V, {});
+  kfin2.assign(V, {});
   for (int v = 0; v < V; ++v) {
     MVR_HIP(hipMalloc(&img[v], nvox() * 4));
     MVR_HIP(hipMalloc(&w[v], nvox() * 4));
@@ -68,6 +70,32 @@ int Decon::set_view(int v, const float* im, const float* wt, const float* kernel
   return 0;
 }
 
+// DeconViewPSF.getKernel1/getKernel2 of a view whose DeconViewPSF.init has run on the caller's side (kernel1
+// normalised, kernel2 per PSF type): init then uses them as they are
+int Decon::set_view_kernels(int v, const float* k1v, const float* k2v, const int Kd[3]) {
+  MVR_CHECK(v >= 0 && v < V, "view index out of range");
+  MVR_CHECK(k1v && k2v && Kd, "null kernel");
+  MVR_CHECK(K[0] != 0, "set_view first");
+  for (int d = 0; d < 3; ++d) MVR_CHECK(K[d] == Kd[d], "the kernels must have the size given to set_view");
+  const size_t nk = (size_t)K[0] * K[1] * K[2];
+  kfin1[v].resize(nk);
+  kfin2[v].resize(nk);
+  MVR_HIP(hipSetDevice(dev));
+  MVR_HIP(hipMemcpy(kfin1[v].data(), k1v, nk * 4, hipMemcpyDefault));
+  MVR_HIP(hipMemcpy(kfin2[v].data(), k2v, nk * 4, hipMemcpyDefault));
+  ready = false;
+  return 0;
+}
+
+// PsiInit.getMax of an initialiser that ran on the caller's side (MultiViewDeconvolution.java:124-134)
+int Decon::set_max(const float* mx) {
+  MVR_CHECK(ready, "init first");
+  MVR_CHECK(mx != nullptr, "null max");
+  for (int v = 0; v < V; ++v) maxI[v] = mx[v];
+  drop_graph();  // a captured iteration holds the old maxima as kernel arguments
+  return 0;
+}
+
 int Decon::ensure_partials(int n) {
   if (n <= partialCap) return 0;
   if (partial) (void)hipFree(partial);
@@ -79,9 +107,16 @@ int Decon::ensure_partials(int n) {
 int Decon::init(int psiInit, const float* psiIn) {
   MVR_HIP(hipSetDevice(dev));
   for (int v = 0; v < V; ++v) MVR_CHECK(!kraw[v].empty(), "every view needs set_view before init");
-  // PSFs (DeconViews ctor -> DeconViewPSF.init)
-  k1 = kraw;
-  if (init_psfs(dev, k1, K, psfType, nT, k2, stream)) return -1;
+  // PSFs (DeconViews ctor -> DeconViewPSF.init), unless the caller handed over every view's final kernels
+  bool allFinal = true;
+  for (int v = 0; v < V; ++v) allFinal = allFinal && !kfin1[v].empty();
+  if (allFinal) {
+    k1 = kfin1;
+    k2 = kfin2;
+  } else {
+    k1 = kraw;
+    if (init_psfs(dev, k1, K, psfType, nT, k2, stream)) return -1;
+  }
   // plan over the whole psi volume, mirror-single extension at its faces
   drop_graph();
   if (build_plan_and_spectra(tunedP[0] ? tunedP : nullptr, V)) return -1;
@@ -688,14 +723,32 @@ int BlockThread::set_view_kernels(int v, const float* K1, const float* K2, const
   return 0;
 }
 
-int BlockThread::run_seq(int v, float* psiBlock, const float* imgBlock, const float* wBlock, float maxI,
-                         double stats[2]) {
-  MVR_CHECK(plan.xf != nullptr && v >= 0 && v < (int)spec1.size() && spec1[v], "set_view_kernels first");
+// The whole image and weights of a view, resident for the thread's lifetime: the drivers cut every block of
+// them again in every iteration (MultiViewDeconvolutionSeq.java:115-126), so they cross the bus once here.
+int BlockThread::set_view_data(int v, const float* im, const float* wt, const int n[3]) {
+  MVR_CHECK(v >= 0 && v < 1024, "view index out of range");
+  MVR_CHECK(im && wt && n, "null view data");
+  for (int d = 0; d < 3; ++d) {
+    MVR_CHECK(n[d] >= 1, "view dimensions must be positive");
+    if (VN[0] != 0) MVR_CHECK(VN[d] == n[d], "all views must share one size");
+  }
   MVR_HIP(hipSetDevice(dev));
-  const size_t nb = (size_t)B[0] * B[1] * B[2];
-  MVR_HIP(hipMemcpyAsync(psiB, psiBlock, nb * 4, hipMemcpyDefault, stream));
-  MVR_HIP(hipMemcpyAsync(imgB, imgBlock, nb * 4, hipMemcpyDefault, stream));
-  MVR_HIP(hipMemcpyAsync(wB, wBlock, nb * 4, hipMemcpyDefault, stream));
+  for (int d = 0; d < 3; ++d) VN[d] = n[d];
+  const size_t nv = (size_t)VN[0] * VN[1] * VN[2];
+  if ((int)viewImg.size() <= v) {
+    viewImg.resize(v + 1, nullptr);
+    viewW.resize(v + 1, nullptr);
+  }
+  if (!viewImg[v]) MVR_HIP(hipMalloc(&viewImg[v], nv * 4));
+  if (!viewW[v]) MVR_HIP(hipMalloc(&viewW[v], nv * 4));
+  MVR_HIP(hipMemcpyAsync(viewImg[v], im, nv * 4, hipMemcpyDefault, stream));
+  MVR_HIP(hipMemcpyAsync(viewW[v], wt, nv * 4, hipMemcpyDefault, stream));
+  MVR_HIP(hipStreamSynchronize(stream));
+  return 0;
+}
+
+// ComputeBlockSeqThreadCPU.runIteration on the device buffers psiB, imgB, wB (stats left in statDev)
+int BlockThread::iterate_seq(int v, float maxI) {
   const Dom& d = plan.dom;
   const StridedGeom gy = plan.geom_y(), gz = plan.geom_z();
   // convolve1: psi block with mirror-single extension of the block itself (ComputeBlockSeqThreadCPU.java:171-189)
@@ -726,6 +779,11 @@ int BlockThread::run_seq(int v, float* psiBlock, const float* imgBlock, const fl
   const int np = plan.xu(d, u, S, plan.twM, plan.wr, stream);
   launch_reduce_stats(partial, np, statDev, stream);
   MVR_HIP(hipGetLastError());
+  return 0;
+}
+
+int BlockThread::finish(float* psiBlock, double stats[2]) {
+  const size_t nb = (size_t)B[0] * B[1] * B[2];
   double2 h;
   MVR_HIP(hipMemcpyAsync(psiBlock, psiB, nb * 4, hipMemcpyDefault, stream));
   MVR_HIP(hipMemcpyAsync(&h, statDev, sizeof(double2), hipMemcpyDeviceToHost, stream));
@@ -734,26 +792,37 @@ int BlockThread::run_seq(int v, float* psiBlock, const float* imgBlock, const fl
   return 0;
 }
 
-int BlockThread::run_mul(int nv, float* psiBlock, const float* const* imgBlocks, const float* const* wBlocks,
-                         const float* maxIs, double stats[2]) {
-  MVR_CHECK(nv >= 1 && nv <= 32 && nv <= (int)spec1.size(), "set_view_kernels for every view first");
-  for (int v = 0; v < nv; ++v) MVR_CHECK(spec1[v] != nullptr, "set_view_kernels for every view first");
+int BlockThread::run_seq(int v, float* psiBlock, const float* imgBlock, const float* wBlock, float maxI,
+                         double stats[2]) {
+  MVR_CHECK(plan.xf != nullptr && v >= 0 && v < (int)spec1.size() && spec1[v], "set_view_kernels first");
+  MVR_CHECK(psiBlock && imgBlock && wBlock, "null block");
   MVR_HIP(hipSetDevice(dev));
   const size_t nb = (size_t)B[0] * B[1] * B[2];
-  while ((int)imgV.size() < nv) {
-    float *a = nullptr, *b = nullptr;
-    MVR_HIP(hipMalloc(&a, nb * 4));
-    MVR_HIP(hipMalloc(&b, nb * 4));
-    imgV.push_back(a);
-    wV.push_back(b);
-  }
-  if (!T2) MVR_HIP(hipMalloc(&T2, nb * 4 * 32));
-  if (!Spsi) MVR_HIP(hipMalloc(&Spsi, plan.ncpx() * 8));
   MVR_HIP(hipMemcpyAsync(psiB, psiBlock, nb * 4, hipMemcpyDefault, stream));
-  for (int v = 0; v < nv; ++v) {
-    MVR_HIP(hipMemcpyAsync(imgV[v], imgBlocks[v], nb * 4, hipMemcpyDefault, stream));
-    MVR_HIP(hipMemcpyAsync(wV[v], wBlocks[v], nb * 4, hipMemcpyDefault, stream));
-  }
+  MVR_HIP(hipMemcpyAsync(imgB, imgBlock, nb * 4, hipMemcpyDefault, stream));
+  MVR_HIP(hipMemcpyAsync(wB, wBlock, nb * 4, hipMemcpyDefault, stream));
+  if (iterate_seq(v, maxI)) return -1;
+  return finish(psiBlock, stats);
+}
+
+// the same iteration with the image and weight blocks cut on the device from the view's resident data
+int BlockThread::run_seq_at(int v, float* psiBlock, const long long off[3], float maxI, double stats[2]) {
+  MVR_CHECK(plan.xf != nullptr && v >= 0 && v < (int)spec1.size() && spec1[v], "set_view_kernels first");
+  MVR_CHECK(v < (int)viewImg.size() && viewImg[v], "set_view_data for this view first");
+  MVR_CHECK(psiBlock && off, "null block or offset");
+  MVR_HIP(hipSetDevice(dev));
+  const size_t nb = (size_t)B[0] * B[1] * B[2];
+  MVR_HIP(hipMemcpyAsync(psiB, psiBlock, nb * 4, hipMemcpyDefault, stream));
+  launch_copy_block_zero(viewImg[v], VN, off, imgB, B, stream);
+  launch_copy_block_zero(viewW[v], VN, off, wB, B, stream);
+  MVR_HIP(hipGetLastError());
+  if (iterate_seq(v, maxI)) return -1;
+  return finish(psiBlock, stats);
+}
+
+// ComputeBlockMulThreadCPU.runIteration on psiB and the per-view blocks imgV / wV (stats left in statDev)
+int BlockThread::iterate_mul(int nv, const float* maxIs) {
+  const size_t nb = (size_t)B[0] * B[1] * B[2];
   const Dom& d = plan.dom;
   const StridedGeom gy = plan.geom_y(), gz = plan.geom_z();
   GatherParams g{};
@@ -789,12 +858,54 @@ int BlockThread::run_mul(int nv, float* psiBlock, const float* const* imgBlocks,
                     stream);
   launch_reduce_stats(partial, np, statDev, stream);
   MVR_HIP(hipGetLastError());
-  double2 h;
-  MVR_HIP(hipMemcpyAsync(psiBlock, psiB, nb * 4, hipMemcpyDefault, stream));
-  MVR_HIP(hipMemcpyAsync(&h, statDev, sizeof(double2), hipMemcpyDeviceToHost, stream));
-  MVR_HIP(hipStreamSynchronize(stream));
-  if (stats) { stats[0] = h.x; stats[1] = h.y; }
   return 0;
+}
+
+int BlockThread::prepare_mul(int nv) {
+  MVR_CHECK(nv >= 1 && nv <= 32 && nv <= (int)spec1.size(), "set_view_kernels for every view first");
+  for (int v = 0; v < nv; ++v) MVR_CHECK(spec1[v] != nullptr, "set_view_kernels for every view first");
+  MVR_HIP(hipSetDevice(dev));
+  const size_t nb = (size_t)B[0] * B[1] * B[2];
+  while ((int)imgV.size() < nv) {
+    float *a = nullptr, *b = nullptr;
+    MVR_HIP(hipMalloc(&a, nb * 4));
+    imgV.push_back(a);
+    MVR_HIP(hipMalloc(&b, nb * 4));
+    wV.push_back(b);
+  }
+  if (!T2) MVR_HIP(hipMalloc(&T2, nb * 4 * 32));
+  if (!Spsi) MVR_HIP(hipMalloc(&Spsi, plan.ncpx() * 8));
+  return 0;
+}
+
+int BlockThread::run_mul(int nv, float* psiBlock, const float* const* imgBlocks, const float* const* wBlocks,
+                         const float* maxIs, double stats[2]) {
+  MVR_CHECK(psiBlock && imgBlocks && wBlocks && maxIs, "null block");
+  if (prepare_mul(nv)) return -1;
+  const size_t nb = (size_t)B[0] * B[1] * B[2];
+  MVR_HIP(hipMemcpyAsync(psiB, psiBlock, nb * 4, hipMemcpyDefault, stream));
+  for (int v = 0; v < nv; ++v) {
+    MVR_HIP(hipMemcpyAsync(imgV[v], imgBlocks[v], nb * 4, hipMemcpyDefault, stream));
+    MVR_HIP(hipMemcpyAsync(wV[v], wBlocks[v], nb * 4, hipMemcpyDefault, stream));
+  }
+  if (iterate_mul(nv, maxIs)) return -1;
+  return finish(psiBlock, stats);
+}
+
+int BlockThread::run_mul_at(int nv, float* psiBlock, const long long off[3], const float* maxIs, double stats[2]) {
+  MVR_CHECK(psiBlock && off && maxIs, "null block, offset or maxima");
+  for (int v = 0; v < nv; ++v)
+    MVR_CHECK(v < (int)viewImg.size() && viewImg[v], "set_view_data for every view first");
+  if (prepare_mul(nv)) return -1;
+  const size_t nb = (size_t)B[0] * B[1] * B[2];
+  MVR_HIP(hipMemcpyAsync(psiB, psiBlock, nb * 4, hipMemcpyDefault, stream));
+  for (int v = 0; v < nv; ++v) {
+    launch_copy_block_zero(viewImg[v], VN, off, imgV[v], B, stream);
+    launch_copy_block_zero(viewW[v], VN, off, wV[v], B, stream);
+  }
+  MVR_HIP(hipGetLastError());
+  if (iterate_mul(nv, maxIs)) return -1;
+  return finish(psiBlock, stats);
 }
 
 void BlockThread::destroy() {
@@ -804,10 +915,12 @@ void BlockThread::destroy() {
   for (auto p : spec2) if (p) (void)hipFree(p);
   for (auto p : imgV) if (p) (void)hipFree(p);
   for (auto p : wV) if (p) (void)hipFree(p);
+  for (auto p : viewImg) if (p) (void)hipFree(p);
+  for (auto p : viewW) if (p) (void)hipFree(p);
   for (void* p : {(void*)psiB, (void*)imgB, (void*)wB, (void*)T2, (void*)S, (void*)Spsi, (void*)partial,
                   (void*)statDev})
     if (p) (void)hipFree(p);
-  spec1.clear(); spec2.clear(); imgV.clear(); wV.clear();
+  spec1.clear(); spec2.clear(); imgV.clear(); wV.clear(); viewImg.clear(); viewW.clear();
   psiB = imgB = wB = T2 = nullptr; S = Spsi = nullptr; partial = nullptr; statDev = nullptr;
   plan.destroy();
   if (stream) (void)hipStreamDestroy(stream);

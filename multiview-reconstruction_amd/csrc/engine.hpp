@@ -101,6 +101,8 @@ int launch_psi_init_partials(const float* const* imgs, int V, size_t n, double* 
 void launch_update_mul(float* psi, const float* const* T2, const float* const* w, int V, const int N[3],
                        const int o[3], const int D[3], double lambda, float minValue, float maxI, double2* partial,
                        int* nPartialOut, hipStream_t s);
+void launch_copy_block_zero(const float* src, const int N[3], const long long off[3], float* dst, const int B[3],
+                            hipStream_t s);
 void launch_quotient(const float* img, float* blurred, size_t n, hipStream_t s);
 int launch_final_values(float* psi, const float* integral, const float* weight, size_t n, double lambda,
                         float minValue, float maxI, double2* partial, int maxPartials, hipStream_t s);

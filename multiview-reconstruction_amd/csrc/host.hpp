@@ -40,6 +40,9 @@ struct Plan {
 
   // chooses the FFT lengths; P[d] >= D[d] + K[d] - 1 (or the given minimum per axis)
   int create(const int D[3], const int K[3], const int* Pmin = nullptr);
+  // the same geometry and kernel choice without device memory (twiddle pointers stay null): for callers that
+  // place the twiddle tables themselves
+  int create_geometry(const int D[3], const int K[3], const int* Pmin = nullptr);
   void destroy();
   size_t ncpx() const { return (size_t)dom.HxP * dom.P[1] * dom.P[2]; }
   size_t rows_inner() const { return (size_t)dom.D[1] * dom.D[2]; }
@@ -100,6 +103,7 @@ struct Decon {
   int K[3] = {0, 0, 0};
   std::vector<float*> img, w;
   std::vector<std::vector<float>> kraw, k1, k2;
+  std::vector<std::vector<float>> kfin1, kfin2;  // final kernels handed over by the caller (empty: none)
   float* psi = nullptr;
   float* psi0 = nullptr;
   Plan plan;
@@ -133,6 +137,8 @@ struct Decon {
 
   int create(int dev, const int N[3], int V, int psfType, int scheme, float lambda, float minValue, int nT);
   int set_view(int v, const float* img, const float* w, const float* kernel, const int K[3]);
+  int set_view_kernels(int v, const float* k1, const float* k2, const int K[3]);
+  int set_max(const float* mx);
   int init(int psiInit, const float* psiIn);
   // psi0 := the psi currently resident (after an external initialiser wrote it)
   int commit_psi0();
@@ -179,11 +185,22 @@ struct BlockThread {
 
   int create(int dev, const int B[3], float minValue, float lambda, int id);
   int set_view_kernels(int v, const float* K1, const float* K2, const int K[3]);
+  // whole view image and weights resident on the device (dims VN, shared by all views)
+  std::vector<float*> viewImg, viewW;
+  int VN[3] = {0, 0, 0};
+
+  int set_view_data(int v, const float* img, const float* w, const int n[3]);
   int run_seq(int v, float* psiBlock, const float* imgBlock, const float* wBlock, float maxI, double stats[2]);
+  int run_seq_at(int v, float* psiBlock, const long long off[3], float maxI, double stats[2]);
   int run_mul(int nv, float* psiBlock, const float* const* imgBlocks, const float* const* wBlocks,
               const float* maxIs, double stats[2]);
+  int run_mul_at(int nv, float* psiBlock, const long long off[3], const float* maxIs, double stats[2]);
   void destroy();
   int ensure_partials(int n);
+  int iterate_seq(int v, float maxI);
+  int iterate_mul(int nv, const float* maxIs);
+  int prepare_mul(int nv);
+  int finish(float* psiBlock, double stats[2]);
 };
 
 }  // namespace mvr

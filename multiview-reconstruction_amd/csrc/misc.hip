@@ -233,6 +233,35 @@ void launch_update_mul(float* psi, const float* const* T2, const float* const* w
   *nPartialOut = nb;
 }
 
+// Block.copyBlock from a zero-extended volume (C/Block.java:158-197 with Views.extendZero, as the image and weight
+// blocks of MultiViewDeconvolutionSeq.java:115-126): dst (B voxels) = src (N voxels) at volume coordinate
+// off + local position, 0 outside the volume.  One thread per destination voxel, rows of B.x consecutive
+// threads so that the loads of a row are contiguous.
+__global__ void k_copy_block_zero(const float* __restrict__ src, int3 N, int3 off, float* __restrict__ dst, int3 B) {
+  const size_t n = (size_t)B.x * B.y * B.z;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % (size_t)B.x);
+    const size_t r = i / (size_t)B.x;
+    const int y = (int)(r % (size_t)B.y), z = (int)(r / (size_t)B.y);
+    const long long gx = (long long)off.x + x, gy = (long long)off.y + y, gz = (long long)off.z + z;
+    float v = 0.f;
+    if (gx >= 0 && gy >= 0 && gz >= 0 && gx < N.x && gy < N.y && gz < N.z)
+      v = src[((size_t)gz * N.y + (size_t)gy) * N.x + (size_t)gx];
+    dst[i] = v;
+  }
+}
+
+void launch_copy_block_zero(const float* src, const int N[3], const long long off[3], float* dst, const int B[3],
+                            hipStream_t s) {
+  const size_t n = (size_t)B[0] * B[1] * B[2];
+  const int nb = (int)std::max<size_t>(1, std::min<size_t>((n + kThreads - 1) / kThreads, 16384));
+  // offsets beyond +-2^31 would place the block wholly outside any volume the library holds: clamp to a value
+  // that is still outside (the copy is then all zeros)
+  auto c = [](long long v) { return (int)std::max<long long>(-(1ll << 30), std::min<long long>(v, 1ll << 30)); };
+  hipLaunchKernelGGL(k_copy_block_zero, dim3(nb), dim3(kThreads), 0, s, src, make_int3(N[0], N[1], N[2]),
+                     make_int3(c(off[0]), c(off[1]), c(off[2])), dst, make_int3(B[0], B[1], B[2]));
+}
+
 // DeconvolutionMethods.computeQuotient (D/iteration/sequential/DeconvolutionMethods.java:46-98) over a flat array:
 // blurred[i] = img[i] > 0 ? img[i] / blurred[i] : 1.
 __global__ void k_quotient(const float* __restrict__ img, float* __restrict__ blurred, size_t n) {

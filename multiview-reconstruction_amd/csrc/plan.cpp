@@ -35,6 +35,14 @@ static int upload_twiddles(float2** dst, int n, int period) {
 }
 
 int Plan::create(const int D[3], const int Kd[3], const int* Pmin) {
+  if (create_geometry(D, Kd, Pmin)) return -1;
+  if (upload_twiddles(&twM, M, M) || upload_twiddles(&wr, M + 1, dom.P[0]) || upload_twiddles(&twY, dom.P[1], dom.P[1]) ||
+      upload_twiddles(&twZ, dom.P[2], dom.P[2]))
+    return -1;
+  return 0;
+}
+
+int Plan::create_geometry(const int D[3], const int Kd[3], const int* Pmin) {
   for (int d = 0; d < 3; ++d) {
     if (D[d] < 1 || Kd[d] < 1) { set_error("plan: empty domain or kernel"); return -1; }
     K[d] = Kd[d];
@@ -56,9 +64,6 @@ int Plan::create(const int D[3], const int Kd[3], const int* Pmin) {
   sy = get_strided(dom.P[1]);
   sz = get_strided(dom.P[2]);
   if (!xf || !xr || !xu || !xs || !sy || !sz) { set_error("plan: missing kernel instantiation"); return -1; }
-  if (upload_twiddles(&twM, M, M) || upload_twiddles(&wr, M + 1, dom.P[0]) || upload_twiddles(&twY, dom.P[1], dom.P[1]) ||
-      upload_twiddles(&twZ, dom.P[2], dom.P[2]))
-    return -1;
   return 0;
 }
 

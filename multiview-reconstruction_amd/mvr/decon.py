@@ -192,10 +192,14 @@ class MultiViewDeconvolution:
     driving HIP block threads."""
 
     def __init__(self, views: DeconViews, num_iterations, factory: ComputeBlockThreadHIPFactory, scheme="SEQ",
-                 psi0=None):
+                 psi0=None, resident_views=True):
+        """resident_views: each thread holds every view's image and weights on its device (uploaded once) and cuts
+        the blocks there (mvr_block_run_iteration_*_at); otherwise the host cuts and sends every block, as the
+        reference's drivers hand Views.extendZero intervals to the thread."""
         self.views = views
         self.num_iterations = num_iterations
         self.scheme = scheme
+        self.resident_views = resident_views
         self.threads = [factory.create(i) for i in range(factory.num_parallel_blocks())]
         shape = views.views[0].image.shape
         self.avg, self.max = psi_init_avg_precise(views.views)
@@ -204,6 +208,8 @@ class MultiViewDeconvolution:
         for t in self.threads:
             for i, v in enumerate(views.views):
                 t.set_view_kernels(i, v.kernel1, v.kernel2)
+                if resident_views:
+                    t.set_view_data(i, v.image, v.weight)
         self.it = 0
         self.stats = []
 
@@ -238,14 +244,22 @@ class MultiViewDeconvolution:
         vs = self.views.views
         if self.scheme == "SEQ":
             for i, v in enumerate(vs):
-                def compute(t, b, blk, i=i, v=v):
-                    return t.run_iteration(i, blk, copy_block_zero(v.image, b), copy_block_zero(v.weight, b),
-                                           self.max[i])
+                if self.resident_views:
+                    def compute(t, b, blk, i=i):
+                        return t.run_iteration_at(i, blk, b.offset, self.max[i])
+                else:
+                    def compute(t, b, blk, i=i, v=v):
+                        return t.run_iteration(i, blk, copy_block_zero(v.image, b), copy_block_zero(v.weight, b),
+                                               self.max[i])
                 self.stats.append(self._block_loop(v.batches, v.num_blocks, compute))
         else:
-            def compute(t, b, blk):
-                return t.run_iteration_mul(blk, [copy_block_zero(v.image, b) for v in vs],
-                                           [copy_block_zero(v.weight, b) for v in vs], self.max)
+            if self.resident_views:
+                def compute(t, b, blk):
+                    return t.run_iteration_mul_at(blk, b.offset, self.max)
+            else:
+                def compute(t, b, blk):
+                    return t.run_iteration_mul(blk, [copy_block_zero(v.image, b) for v in vs],
+                                               [copy_block_zero(v.weight, b) for v in vs], self.max)
             self.stats.append(self._block_loop(vs[0].batches, vs[0].num_blocks, compute))
 
     def close(self):

@@ -69,11 +69,18 @@ def lib():
         L.mvr_block_run_iteration_seq.argtypes = [vp, ctypes.c_int, vp, vp, vp, ctypes.c_float, _f64p]
         L.mvr_block_run_iteration_mul.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                                   _f32p, _f64p]
+        L.mvr_block_thread_set_view_data.argtypes = [vp, ctypes.c_int, vp, vp, _i32p]
+        L.mvr_block_run_iteration_seq_at.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_longlong),
+                                                     ctypes.c_float, _f64p]
+        L.mvr_block_run_iteration_mul_at.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_longlong), _f32p,
+                                                     _f64p]
         L.mvr_block_thread_destroy.argtypes = [vp]
         L.mvr_block_thread_destroy.restype = None
         L.mvr_decon_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int]
         L.mvr_decon_set_view.argtypes = [vp, ctypes.c_int, vp, vp, vp, _i32p]
+        L.mvr_decon_set_view_kernels.argtypes = [vp, ctypes.c_int, vp, vp, _i32p]
+        L.mvr_decon_set_max.argtypes = [vp, _f32p]
         L.mvr_decon_init.argtypes = [vp, ctypes.c_int, vp]
         L.mvr_decon_run.argtypes = [vp, ctypes.c_int]
         L.mvr_decon_get_psi.argtypes = [vp, vp]
@@ -264,6 +271,29 @@ class BlockThread:
         _check(lib().mvr_block_run_iteration_mul(self.h, n, _ptr(psi), ip, wp, mx.ctypes.data_as(_f32p), st))
         return psi, (st[0], st[1])
 
+    def set_view_data(self, view, img, weight):
+        """The view's whole image and weights, resident on the device for the thread's lifetime."""
+        img, weight = _f32(img), _f32(weight)
+        assert img.shape == weight.shape
+        _check(lib().mvr_block_thread_set_view_data(self.h, view, _ptr(img), _ptr(weight), _dims(img.shape)))
+
+    def run_iteration_at(self, view, psi_block, offset_xyz, max_intensity):
+        """run_iteration with the image and weight blocks cut on the device from the resident view data;
+        offset_xyz = volume coordinate of the block's first voxel (may be negative)."""
+        psi = _f32(psi_block).copy()
+        st = (ctypes.c_double * 2)()
+        off = (ctypes.c_longlong * 3)(*[int(o) for o in offset_xyz])
+        _check(lib().mvr_block_run_iteration_seq_at(self.h, view, _ptr(psi), off, float(max_intensity), st))
+        return psi, (st[0], st[1])
+
+    def run_iteration_mul_at(self, psi_block, offset_xyz, max_intensities):
+        psi = _f32(psi_block).copy()
+        mx = np.ascontiguousarray(max_intensities, np.float32)
+        st = (ctypes.c_double * 2)()
+        off = (ctypes.c_longlong * 3)(*[int(o) for o in offset_xyz])
+        _check(lib().mvr_block_run_iteration_mul_at(self.h, mx.size, _ptr(psi), off, mx.ctypes.data_as(_f32p), st))
+        return psi, (st[0], st[1])
+
     def close(self):
         if self.h:
             lib().mvr_block_thread_destroy(self.h)
@@ -298,6 +328,17 @@ class Decon:
         kernel = _f32(kernel)
         _check(lib().mvr_decon_set_view(self.h, view, ctypes.c_void_p(img_ptr), ctypes.c_void_p(weight_ptr),
                                         _ptr(kernel), _dims(kernel.shape)))
+
+    def set_view_kernels(self, view, kernel1, kernel2):
+        """Final kernels of a view whose DeconViewPSF.init ran on the caller's side (used as they are by init)."""
+        k1, k2 = _f32(kernel1), _f32(kernel2)
+        _check(lib().mvr_decon_set_view_kernels(self.h, view, _ptr(k1), _ptr(k2), _dims(k1.shape)))
+
+    def set_max(self, max_per_view):
+        """PsiInit.getMax of an initialiser that ran on the caller's side."""
+        mx = np.ascontiguousarray(max_per_view, np.float32)
+        assert mx.size == self.V
+        _check(lib().mvr_decon_set_max(self.h, mx.ctypes.data_as(_f32p)))
 
     PSI_INITS = {"AVG_PRECISE": 0, "FROM_ARRAY": 1, "AVG_APPROX": 2, "BLURRED_FUSED": 3}
 

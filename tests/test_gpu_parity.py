@@ -181,6 +181,37 @@ def test_block_mul_matches_oracle(oracle):
     t.close()
 
 
+def test_block_iteration_at_equals_host_cut_blocks():
+    # the device-side cut of the resident view data (zero outside the volume) gives bit-identical results to the
+    # host's copy_block_zero, for blocks that reach past every face of the volume and for an interior one
+    rng = np.random.default_rng(9)
+    V, N = 3, (26, 30, 34)                     # z, y, x
+    imgs = rng.uniform(1, 60, (V,) + N).astype(np.float32)
+    ws = rng.uniform(0, 1, (V,) + N).astype(np.float32)
+    ks = np.stack([syn.gaussian_psf(5, 1.0, 2.0, a) for a in (0, 100, 230)])
+    k1, k2 = hip.init_psfs(ks, "EFFICIENT_BAYESIAN", 8)
+    B = (20, 18, 16)                            # x, y, z
+    t = hip.BlockThread(B, 1e-4, 6e-4)
+    for v in range(V):
+        t.set_view_kernels(v, k1[v], k2[v])
+        t.set_view_data(v, imgs[v], ws[v])
+    for off in [(-4, -4, -4), (20, 16, 14), (5, 6, 7), (-30, 2, 3), (40, 40, 40)]:
+        b = decon.Block(B, off, B, off, (0, 0, 0))
+        psi = rng.uniform(5, 30, B[::-1]).astype(np.float32)
+        for v in range(V):
+            a, sa = t.run_iteration(v, psi, decon.copy_block_zero(imgs[v], b), decon.copy_block_zero(ws[v], b), 55.0)
+            c, sc = t.run_iteration_at(v, psi, off, 55.0)
+            np.testing.assert_array_equal(a, c)
+            assert sa == sc
+        mx = np.array([60.0, 58.0, 59.5], np.float32)
+        a, sa = t.run_iteration_mul(psi, [decon.copy_block_zero(imgs[v], b) for v in range(V)],
+                                    [decon.copy_block_zero(ws[v], b) for v in range(V)], mx)
+        c, sc = t.run_iteration_mul_at(psi, off, mx)
+        np.testing.assert_array_equal(a, c)
+        assert sa == sc
+    t.close()
+
+
 def _small_problem(n=(28, 30, 26), V=3, K=7, seed=11, weights="uniform"):
     w = syn.small_workload(views=V, k=K, seed_index=seed, n_blobs=25, weights=weights, shape=n)
     return syn.make_inputs(w)
@@ -451,10 +482,39 @@ def test_resident_psi_init_from_array(oracle):
     d.close()
 
 
+@pytest.mark.parametrize("scheme,mode", [("SEQ", "AVG_PRECISE"), ("MUL", "AVG_APPROX")])
+def test_resident_decon_with_callers_kernels_and_init(scheme, mode):
+    # the Java driver's order: DeconViews has run DeconViewPSF.init (final kernels handed over) and the driver's
+    # PsiInit has produced psi and max[] (handed over as an array and the maxima): the same run, bit for bit
+    inp = _small_problem(weights="zramp")
+    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+    a = hip.Decon(imgs.shape[1:], 3, "OPTIMIZATION_I", scheme, 6e-4)
+    for v in range(3):
+        a.set_view(v, imgs[v], ws[v], ks[v])
+    a.init(mode=mode)
+    psi0, (mx, _) = a.psi(), a.max_avg()
+    kern = [a.kernels(v, ks.shape[1:]) for v in range(3)]
+    a.run(4)
+    b = hip.Decon(imgs.shape[1:], 3, "OPTIMIZATION_I", scheme, 6e-4)
+    for v in range(3):
+        b.set_view(v, imgs[v], ws[v], kern[v][0])
+        b.set_view_kernels(v, *kern[v])
+    b.init(psi0)
+    b.set_max(mx)
+    np.testing.assert_array_equal(b.max_avg()[0], mx)
+    b.run(4)
+    np.testing.assert_array_equal(a.psi(), b.psi())
+    np.testing.assert_array_equal(a.stats(), b.stats())
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("scheme", ["SEQ", "MUL"])
-def test_blocked_driver_matches_oracle(oracle, scheme):
+@pytest.mark.parametrize("resident_views", [True, False])
+def test_blocked_driver_matches_oracle(oracle, scheme, resident_views):
     """The Java-side driver (mirrored in mvr.decon) with several blocks per view, every
-    block iteration in the HIP library, against the oracle with the same block size."""
+    block iteration in the HIP library, against the oracle with the same block size; the view data either
+    resident on the device (blocks cut there) or cut on the host and sent per block."""
     inp = _small_problem(n=(30, 26, 28), K=5)
     imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
     bs = (22, 22, 22)
@@ -462,7 +522,8 @@ def test_blocked_driver_matches_oracle(oracle, scheme):
     views = [decon.DeconView(imgs[v], ws[v], ks[v], bs) for v in range(3)]
     assert views[0].num_blocks > 1
     dv = decon.DeconViews(views, "EFFICIENT_BAYESIAN")
-    m = decon.MultiViewDeconvolution(dv, 4, decon.ComputeBlockThreadHIPFactory(bs, 6e-4), scheme)
+    m = decon.MultiViewDeconvolution(dv, 4, decon.ComputeBlockThreadHIPFactory(bs, 6e-4), scheme,
+                                     resident_views=resident_views)
     m.run_iterations()
     e = rel(m.psi, ref["psi"])
     assert e < BAR, f"rel-L2 {e:.2e} (bar {BAR}, spec {SPEC})"
