@@ -7,7 +7,12 @@ Mirrors the exporters the reference's deconvolution plug-in hands its result to
   * save_n5: process/export/ExportN5Api.java (an N5 dataset: attributes.json with dimensions, blockSize,
     dataType and compression; one file per block at <dataset>/<i>/<j>/<k> holding a big-endian header
     {mode, number of dimensions, block size} and the big-endian block data, x fastest; raw or gzip).
-Volumes are numpy arrays indexed [z, y, x].  Readers for both formats are included for round trips.
+  * save_ome_zarr: the OME-ZARR branch of ExportN5Api.exportImage (process/export/ExportN5Api.java:219-270,
+    365-385, metadata OMEZarrAttibutes.createOMEZarrMetadata, fiji/spimdata/imgloaders/OMEZarrAttibutes.java):
+    one zarr v2 group with a 5D array per resolution level (t, c, z, y, x; levels "0", "1", ...), chunks
+    under "/"-separated keys, and OME-NGFF 0.4 "multiscales" metadata (axes time/channel/space, per level a
+    scale of voxel size x downsampling factor and a half-pixel translation).
+Volumes are numpy arrays indexed [z, y, x].  Readers for all formats are included for round trips.
 PSFs are stored like fiji/spimdata/pointspreadfunctions/PointSpreadFunction.java:46-50, 123-177: one N5
 dataset psf_t<timepoint>_v<setup> per view in <base>/psf.n5 (128^3 blocks, gzip level 1); legacy .tif
 files in <base>/psf are read as well (save_psf / load_psf).
@@ -203,3 +208,150 @@ def load_psf(base_dir, file):
     if file.endswith(".tif"):
         return read_tiff_stack(os.path.join(base_dir, PSF_TIF_DIR, file))[0]
     return read_n5(os.path.join(base_dir, PSF_N5), file)[0]
+
+
+# ---------------------------------------------------------------- OME-ZARR (zarr v2 + OME-NGFF 0.4 multiscales)
+def downsample_mean(vol, factors_zyx):
+    """Block mean by integer factors per axis; an edge block shorter than its factor averages what it holds.
+    (The reference's pyramid uses imglib2-algorithm's Downsample, version managed by pom-scijava 43.0.0, for the
+    same 2^k averaging; its rounding at odd edges is not pinned here.)"""
+    out = np.asarray(vol, np.float64)
+    for axis, f in enumerate(factors_zyx):
+        f = int(f)
+        if f == 1:
+            continue
+        n = out.shape[axis]
+        m = -(-n // f)
+        pad = m * f - n
+        if pad:
+            widths = [(0, 0)] * out.ndim
+            widths[axis] = (0, pad)
+            summed = np.pad(out, widths).reshape(out.shape[:axis] + (m, f) + out.shape[axis + 1:]).sum(axis + 1)
+            counts = np.full(m, float(f))
+            counts[-1] = f - pad
+            shape = [1] * out.ndim
+            shape[axis] = m
+            out = summed / counts.reshape(shape)
+        else:
+            out = out.reshape(out.shape[:axis] + (m, f) + out.shape[axis + 1:]).mean(axis + 1)
+    return out.astype(np.float32)
+
+
+def ome_zarr_multiscales(name, voxel_size_zyx, unit, downsampling_zyx, levels_path=None):
+    """The "multiscales" attribute (OME-NGFF 0.4) of a 5D t, c, z, y, x image: per level a scale of voxel size x
+    downsampling factor and a translation of (factor - 1) / 2 voxels (the mipmap transform of a half-pixel
+    downsampling), 1 and 0 for t and c."""
+    axes = [{"name": "t", "type": "time", "unit": "millisecond"}, {"name": "c", "type": "channel"}] + \
+           [{"name": a, "type": "space", "unit": unit} for a in ("z", "y", "x")]
+    datasets = []
+    for s, f in enumerate(downsampling_zyx):
+        scale = [1.0, 1.0] + [float(v) * float(k) for v, k in zip(voxel_size_zyx, f)]
+        trans = [0.0, 0.0] + [float(v) * (float(k) - 1.0) / 2.0 for v, k in zip(voxel_size_zyx, f)]
+        datasets.append({"path": (levels_path or str)(s),
+                         "coordinateTransformations": [{"type": "scale", "scale": scale},
+                                                       {"type": "translation", "translation": trans}]})
+    return [{"version": "0.4", "name": name, "axes": axes, "datasets": datasets,
+             "coordinateTransformations": [{"type": "scale", "scale": [1.0] * 5}]}]
+
+
+def _zarr_compressor(compression, level):
+    if compression == "raw":
+        return None
+    if compression == "gzip":
+        return {"id": "gzip", "level": 6 if level < 0 else int(level)}
+    if compression == "zlib":
+        return {"id": "zlib", "level": 6 if level < 0 else int(level)}
+    raise ValueError(f"unknown compression {compression}")
+
+
+def _zarr_encode(data, comp):
+    import zlib
+    if comp is None:
+        return data
+    if comp["id"] == "gzip":
+        return gzip.compress(data, compresslevel=comp["level"])
+    return zlib.compress(data, comp["level"])
+
+
+def _zarr_decode(raw, comp):
+    import zlib
+    if comp is None:
+        return raw
+    if comp["id"] == "gzip":
+        return gzip.decompress(raw)
+    if comp["id"] == "zlib":
+        return zlib.decompress(raw)
+    raise ValueError(f"unsupported compressor {comp}")
+
+
+def save_ome_zarr(vol, path, voxel_size_zyx=(1.0, 1.0, 1.0), unit="micrometer", name="deconvolved",
+                  chunks_zyx=(64, 128, 128), downsampling_zyx=((1, 1, 1), (2, 2, 2), (4, 4, 4)), compression="gzip",
+                  level=-1):
+    """vol [z, y, x] as an OME-ZARR image at `path`: a zarr v2 group whose arrays "0", "1", ... hold the resolution
+    levels as float32 5D arrays (1, 1, z, y, x), chunked (1, 1, chunks_zyx) with "/" separated chunk keys.
+    downsampling_zyx: the absolute factor of every level (the first must be 1, each a multiple of the previous).
+    The reference's default block size is 128 x 128 x 64 (x, y, z; ExportN5Api.java:110-112)."""
+    vol = np.ascontiguousarray(vol, np.float32)
+    if tuple(downsampling_zyx[0]) != (1, 1, 1):
+        raise ValueError("level 0 is the full resolution")
+    comp = _zarr_compressor(compression, level)
+    os.makedirs(path, exist_ok=True)
+    with open(os.path.join(path, ".zgroup"), "w") as f:
+        json.dump({"zarr_format": 2}, f)
+    with open(os.path.join(path, ".zattrs"), "w") as f:
+        json.dump({"multiscales": ome_zarr_multiscales(name, voxel_size_zyx, unit, downsampling_zyx)}, f, indent=1)
+    cur, prev = vol, (1, 1, 1)
+    for s, fac in enumerate(downsampling_zyx):
+        rel = tuple(int(a) // int(b) for a, b in zip(fac, prev))
+        if any(int(a) % int(b) for a, b in zip(fac, prev)):
+            raise ValueError("each downsampling factor must be a multiple of the previous level's")
+        cur = downsample_mean(cur, rel) if s else cur
+        prev = tuple(fac)
+        _write_zarr_array(cur, os.path.join(path, str(s)), chunks_zyx, comp)
+    return path
+
+
+def _write_zarr_array(vol, adir, chunks_zyx, comp):
+    nz, ny, nx = vol.shape
+    cz, cy, cx = chunks_zyx
+    os.makedirs(adir, exist_ok=True)
+    meta = {"zarr_format": 2, "shape": [1, 1, nz, ny, nx], "chunks": [1, 1, cz, cy, cx], "dtype": "<f4",
+            "compressor": comp, "fill_value": 0.0, "order": "C", "filters": None, "dimension_separator": "/"}
+    with open(os.path.join(adir, ".zarray"), "w") as f:
+        json.dump(meta, f, indent=1)
+    for k in range(0, nz, cz):
+        for j in range(0, ny, cy):
+            for i in range(0, nx, cx):
+                chunk = np.zeros((cz, cy, cx), np.float32)      # edge chunks are stored whole, padded with 0
+                blk = vol[k:k + cz, j:j + cy, i:i + cx]
+                chunk[:blk.shape[0], :blk.shape[1], :blk.shape[2]] = blk
+                d = os.path.join(adir, "0", "0", str(k // cz), str(j // cy))
+                os.makedirs(d, exist_ok=True)
+                with open(os.path.join(d, str(i // cx)), "wb") as f:
+                    f.write(_zarr_encode(chunk.astype("<f4").tobytes(), comp))
+
+
+def read_ome_zarr(path, level=0):
+    """(vol [z, y, x] of resolution level `level`, the group's attributes)."""
+    with open(os.path.join(path, ".zattrs")) as f:
+        attrs = json.load(f)
+    ds = attrs["multiscales"][0]["datasets"][level]["path"]
+    adir = os.path.join(path, ds)
+    with open(os.path.join(adir, ".zarray")) as f:
+        meta = json.load(f)
+    assert meta["zarr_format"] == 2 and meta["dtype"] in ("<f4", "|f4") and meta["order"] == "C"
+    sep = meta.get("dimension_separator", ".")
+    t, c, nz, ny, nx = meta["shape"]
+    _, _, cz, cy, cx = meta["chunks"]
+    out = np.full((nz, ny, nx), meta["fill_value"] or 0.0, np.float32)
+    for k in range(0, nz, cz):
+        for j in range(0, ny, cy):
+            for i in range(0, nx, cx):
+                p = os.path.join(adir, sep.join(["0", "0", str(k // cz), str(j // cy), str(i // cx)]))
+                if not os.path.exists(p):
+                    continue
+                with open(p, "rb") as f:
+                    chunk = np.frombuffer(_zarr_decode(f.read(), meta["compressor"]), "<f4").reshape(cz, cy, cx)
+                blk = out[k:k + cz, j:j + cy, i:i + cx]
+                blk[...] = chunk[:blk.shape[0], :blk.shape[1], :blk.shape[2]]
+    return out, attrs
