@@ -77,6 +77,10 @@ def main():
     ap.add_argument("--reference-iterations", type=int, default=200)
     ap.add_argument("--block-path", type=int, default=0,
                     help="also time this many iterations through the per-block drop-in ABI (host buffers)")
+    ap.add_argument("--block-size", type=int, nargs=3, default=None, metavar=("BX", "BY", "BZ"),
+                    help="run in the reference's block structure (DeconView block size per axis, all blocks of a "
+                         "view in one launch per pass) instead of one frame; e.g. 216 216 216 for 128^3 effective "
+                         "blocks with 45^3 PSFs")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this size (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -128,8 +132,12 @@ def main():
         for v in range(V):
             d.set_view(v, imgs[v], ws[v], ks[v])
     log(f"inputs generated and uploaded in {time.time() - t0:.1f}s")
+    if args.block_size:
+        d.set_block_size(args.block_size)
     t0 = time.time()
     d.init()
+    if args.block_size:
+        log(f"blocks of {tuple(args.block_size)}: {d.blocks()[0]} in the volume, per view {d.blocks()[1]}")
     log(f"psf init + spectra + psi init in {time.time() - t0:.2f}s, fft grid {d.geometry()}")
     if args.autotune:
         P, ms = d.autotune(args.autotune)
@@ -177,6 +185,7 @@ def main():
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("bytes_per_launch")
 
+    d_blocks = d.blocks()
     psi = d.psi()
     finite = bool(np.isfinite(psi).all())
     parity = golden_parity(d, psi, w, args.workload)
@@ -240,7 +249,9 @@ def main():
         "config": {"workload": w.name, "volume_zyx": list(w.shape_zyx), "views": V, "psf": list(K),
                    "psf_type": w.psf_type, "lambda": w.lam, "iterations": w.iterations, "scheme": "SEQ",
                    "fft_grid_xyz": list(geo["P"]), "tile_widths_yz": list(geo["tile_widths_yz"]),
-                   "parallelism": "single GPU", "psi_finite": finite},
+                   "parallelism": "single GPU", "psi_finite": finite,
+                   "blocks": ({"block_size_xyz": list(args.block_size), "blocks": d_blocks[0],
+                               "blocks_with_content_per_view": d_blocks[1]} if args.block_size else "one frame")},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_s * 1e6, "launches": n_dom,

@@ -147,6 +147,16 @@ int mvr_decon_set_view(mvr_decon* d, int view, const float* img, const float* we
  * DeconViewPSF.init on the raw kernels of mvr_decon_set_view. */
 int mvr_decon_set_view_kernels(mvr_decon* d, int view, const float* kernel1, const float* kernel2,
                                const int kXYZ[3]);
+/* The reference's block structure instead of one frame: DeconView's block size (deconvolution/DeconView.java:
+ * 118-184; effective size B - 2K + 2 per axis, margin K - 1, cuda/BlockGeneratorFixedSizePrecise.java:59-131, uneven
+ * tail blocks), blocks without weights dropped per view (DeconView.filterBlocksForContent, :204-274).  All blocks
+ * of one view run in one launch per pass (they share the FFT grid and the kernel spectra).  Seq scheme; the
+ * effective size must be at least K - 1 (the Seq driver's write-back lag then makes every view pass act on the
+ * psi before it, MultiViewDeconvolutionSeq.java:150-157, as the batched launch does).  {0, 0, 0}: one frame
+ * (default).  Takes effect at the next mvr_decon_init. */
+int mvr_decon_set_block_size(mvr_decon* d, const int blockSizeXYZ[3]);
+/* blocks in the volume (returned) and per view the blocks with content (blocksPerView[0..numViews)) */
+int mvr_decon_get_blocks(mvr_decon* d, int* blocksPerView, int cap);
 /* DeconViewPSF.init for all views + PsiInit.runInitialization (MultiViewDeconvolution.java:102-135) */
 int mvr_decon_init(mvr_decon* d, int psiInit, const float* psi0);
 /* PsiInit.getMax of an initialiser that ran on the caller's side (MultiViewDeconvolution.java:124-134), after

@@ -257,6 +257,23 @@ int mvr_decon_set_max(mvr_decon* d, const float* maxPerView) {
   return d->d.set_max(maxPerView);
 }
 
+int mvr_decon_set_block_size(mvr_decon* d, const int blockSizeXYZ[3]) {
+  NULLCHECK(d);
+  NULLCHECK(blockSizeXYZ);
+  return d->d.set_block_size(blockSizeXYZ);
+}
+
+int mvr_decon_get_blocks(mvr_decon* d, int* blocksPerView, int cap) {
+  NULLCHECK(d);
+  if (!d->d.ready) { set_error("init first"); return -1; }
+  if (!d->d.blocked()) {
+    if (blocksPerView) for (int v = 0; v < d->d.V && v < cap; ++v) blocksPerView[v] = 1;
+    return 1;
+  }
+  if (blocksPerView) for (int v = 0; v < d->d.V && v < cap; ++v) blocksPerView[v] = d->d.nbView[v];
+  return d->d.nbAll;
+}
+
 int mvr_decon_init(mvr_decon* d, int psiInit, const float* psi0) {
   NULLCHECK(d);
   if (psiInit != MVR_PSI_INIT_AVG_PRECISE && psiInit != MVR_PSI_INIT_FROM_ARRAY && psiInit != MVR_PSI_INIT_AVG_APPROX &&

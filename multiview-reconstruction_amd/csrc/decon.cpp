@@ -96,6 +96,89 @@ int Decon::set_max(const float* mx) {
   return 0;
 }
 
+int Decon::set_block_size(const int B[3]) {
+  bool on = false;
+  for (int d = 0; d < 3; ++d) {
+    MVR_CHECK(B[d] >= 0, "negative block size");
+    on = on || B[d] > 0;
+  }
+  if (on) {
+    for (int d = 0; d < 3; ++d) MVR_CHECK(B[d] > 0, "give a block size on every axis (or 0 on all for one frame)");
+    MVR_CHECK(scheme == SCHEME_SEQ, "batched blocks run the Seq scheme");
+  }
+  for (int d = 0; d < 3; ++d) blockB[d] = on ? B[d] : 0;
+  tunedP[0] = tunedP[1] = tunedP[2] = 0;
+  ready = false;
+  return 0;
+}
+
+void Decon::free_blocks() {
+  for (auto p : btab) if (p) (void)hipFree(p);
+  btab.clear();
+  nbView.clear();
+  nbAll = 0;
+}
+
+// Blocks of BlockGeneratorFixedSizePrecise (C/BlockGeneratorFixedSizePrecise.java:59-131): effective regions on a
+// grid of E, x fastest, uneven tail blocks; each view keeps the blocks whose interval (effective region plus a
+// margin of K - 1, D/DeconView.java:155-159) holds non-zero weights (DeconView.filterBlocksForContent,
+// D/DeconView.java:204-274).
+int Decon::build_blocks() {
+  free_blocks();
+  int nbx[3];
+  for (int d = 0; d < 3; ++d) nbx[d] = (N[d] + blockE[d] - 1) / blockE[d];
+  nbAll = nbx[0] * nbx[1] * nbx[2];
+  std::vector<int> out, interval;   // {o, D} and {offset, size} per block
+  for (int k = 0; k < nbx[2]; ++k)
+    for (int j = 0; j < nbx[1]; ++j)
+      for (int i = 0; i < nbx[0]; ++i) {
+        const int c[3] = {i, j, k};
+        for (int d = 0; d < 3; ++d) out.push_back(c[d] * blockE[d]);
+        for (int d = 0; d < 3; ++d) out.push_back(std::min(blockE[d], N[d] - c[d] * blockE[d]));
+        for (int d = 0; d < 3; ++d) interval.push_back(c[d] * blockE[d] - (K[d] - 1));
+        for (int d = 0; d < 3; ++d) interval.push_back(blockB[d]);
+      }
+  int* dInt = nullptr;
+  int* dFlags = nullptr;
+  MVR_HIP(hipMalloc(&dInt, interval.size() * sizeof(int)));
+  MVR_HIP(hipMalloc(&dFlags, sizeof(int) * nbAll));
+  MVR_HIP(hipMemcpyAsync(dInt, interval.data(), interval.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+  btab.assign(V, nullptr);
+  nbView.assign(V, 0);
+  std::vector<int> flags(nbAll);
+  int rc = 0;
+  for (int v = 0; v < V && !rc; ++v) {
+    launch_fill(reinterpret_cast<float*>(dFlags), (size_t)nbAll, 0.f, stream);
+    launch_blocks_with_content(w[v], N, dInt, nbAll, dFlags, stream);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(flags.data(), dFlags, sizeof(int) * nbAll, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+      set_error("block content scan failed");
+      rc = -1;
+      break;
+    }
+    std::vector<int> tab;
+    for (int b = 0; b < nbAll; ++b)
+      if (flags[b]) tab.insert(tab.end(), out.begin() + kBlockTabInts * b, out.begin() + kBlockTabInts * (b + 1));
+    nbView[v] = (int)(tab.size() / kBlockTabInts);
+    if (nbView[v] == 0) continue;
+    if (hipMalloc(&btab[v], tab.size() * sizeof(int)) != hipSuccess ||
+        hipMemcpy(btab[v], tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("block table upload failed");
+      rc = -1;
+    }
+  }
+  (void)hipFree(dInt);
+  (void)hipFree(dFlags);
+  return rc;
+}
+
+// partial sums of one x_update launch: per work-group, at most D1 D2 / 4 + 1 per block
+size_t Decon::partials_needed() const {
+  const size_t per = plan.rows_inner() / 4 + 1;
+  return std::max<size_t>(2048, blocked() ? per * std::max(1, nbAll) : per);
+}
+
 int Decon::ensure_partials(int n) {
   if (n <= partialCap) return 0;
   if (partial) (void)hipFree(partial);
@@ -200,7 +283,18 @@ int Decon::init(int psiInit, const float* psiIn) {
     double* dh = nullptr;
     MVR_HIP(hipMalloc(&dh, sizeof(double) * H));
     MVR_HIP(hipMemcpyAsync(dh, half.data(), sizeof(double) * H, hipMemcpyHostToDevice, stream));
-    float* tmp = reinterpret_cast<float*>(S);  // the work spectrum holds >= 4 bytes per voxel
+    float* tmp = reinterpret_cast<float*>(S);  // the work spectra hold >= 4 bytes per voxel, unless blocks are dropped
+    float* own = nullptr;
+    if (blocked()) {
+      size_t have = plan.ncpx() * 8;
+      int nS = 1;
+      for (int v = 0; v < V; ++v) nS = std::max(nS, nbView[v]);
+      have *= (size_t)nS;
+      if (have < nvox() * 4) {
+        MVR_HIP(hipMalloc(&own, nvox() * 4));
+        tmp = own;
+      }
+    }
     std::vector<const float*> ip(img.begin(), img.end()), wp(w.begin(), w.end());
     launch_fused_nonzero(ip.data(), wp.data(), V, nvox(), tmp, stream);
     launch_sym_conv_axis(tmp, psi, N, 0, dh, H, stream);
@@ -209,6 +303,7 @@ int Decon::init(int psiInit, const float* psiIn) {
     MVR_HIP(hipGetLastError());
     MVR_HIP(hipStreamSynchronize(stream));
     (void)hipFree(dh);
+    if (own) (void)hipFree(own);
   } else {
     MVR_CHECK(false, "unknown psi init");
   }
@@ -223,7 +318,26 @@ int Decon::init(int psiInit, const float* psiIn) {
 // work spectra.
 int Decon::build_plan_and_spectra(const int* Pmin, int nviews) {
   plan.destroy();
-  if (plan.create(N, K, Pmin)) return -1;
+  int nS = 1;  // work spectra: one per block of the largest view list
+  if (blocked()) {
+    // a block's domain holds its output region plus both kernels' reach on either side, so that both chained
+    // convolutions are exact on the effective region: P >= E + 2 (K - 1), output region at m = m1 + m2
+    int D[3], Pm[3];
+    for (int d = 0; d < 3; ++d) {
+      blockE[d] = blockB[d] - 2 * K[d] + 2;
+      MVR_CHECK(blockE[d] >= 1, "the block size must exceed 2 x kernel size - 2 (BlockGeneratorFixedSizePrecise)");
+      MVR_CHECK(blockE[d] >= K[d] - 1 || blockE[d] >= N[d],
+                "batched blocks need an effective block size of at least kernel size - 1");
+      D[d] = std::min(blockE[d], N[d]);
+      Pm[d] = std::max(D[d] + 2 * (K[d] - 1), Pmin ? Pmin[d] : 0);
+    }
+    if (plan.create(D, K, Pm)) return -1;
+    for (int d = 0; d < 3; ++d) plan.dom.m[d] = 2 * (K[d] - 1 - K[d] / 2);
+    if (build_blocks()) return -1;
+    for (int v = 0; v < V; ++v) nS = std::max(nS, nbView[v]);
+  } else {
+    if (plan.create(N, K, Pmin)) return -1;
+  }
   plan.wy = tunedW[0];
   plan.wz = tunedW[1];
   for (auto p : spec1) if (p) (void)hipFree(p);
@@ -244,7 +358,7 @@ int Decon::build_plan_and_spectra(const int* Pmin, int nviews) {
   }
   (void)hipFree(dk);
   if (S) (void)hipFree(S);
-  MVR_HIP(hipMalloc(&S, plan.ncpx() * 8));
+  MVR_HIP(hipMalloc(&S, plan.ncpx() * 8 * (size_t)nS));
   if (scheme == SCHEME_MUL) {
     if (Spsi) (void)hipFree(Spsi);
     if (T2) (void)hipFree(T2);
@@ -272,6 +386,7 @@ int Decon::set_graph(bool on) {
 // next init.
 int Decon::autotune(int candidates, int chosen[3], double* msOut) {
   MVR_CHECK(ready, "init first");
+  MVR_CHECK(!blocked(), "autotuning runs on the one-frame schedule (set the block size to 0)");
   MVR_CHECK(candidates >= 1 && candidates <= 8, "candidates per axis must be in [1, 8]");
   MVR_HIP(hipSetDevice(dev));
   drop_graph();
@@ -424,8 +539,24 @@ static StridedGeom timed(Timer& t, StridedGeom g, int c) {
 
 // One view of one Seq iteration over the whole psi volume.
 int Decon::view_seq(int v, double2* statOut) {
-  const Dom& d = plan.dom;
-  const StridedGeom gy = plan.geom_y(), gz = plan.geom_z();
+  Dom d = plan.dom;
+  StridedGeom gy = plan.geom_y(), gz = plan.geom_z();
+  if (blocked()) {
+    // every block of this view in one launch per pass
+    if (nbView[v] == 0) {  // no block with content: the view's statistics stay {0, -1}
+      launch_reduce_stats(partial, 0, statOut, stream);
+      MVR_HIP(hipGetLastError());
+      return 0;
+    }
+    d.btab = btab[v];
+    d.nb = nbView[v];
+    d.blockStride = plan.ncpx();
+    for (StridedGeom* gm : {&gy, &gz}) {
+      gm->blockPlanes = gm->nplanes;
+      gm->nplanes *= d.nb;
+      gm->blockStride = plan.ncpx();
+    }
+  }
   GatherParams g{};
   g.src = psi;
   for (int k = 0; k < 3; ++k) { g.N[k] = N[k]; g.org[k] = -d.m[k]; }
@@ -438,7 +569,14 @@ int Decon::view_seq(int v, double2* statOut) {
   plan.sz(timed(timer, gz, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
   RatioParams q{};
   q.img = img[v];
-  for (int k = 0; k < 3; ++k) { q.N[k] = N[k]; q.o[k] = 0; }
+  for (int k = 0; k < 3; ++k) {
+    q.N[k] = N[k];
+    q.o[k] = 0;
+    if (blocked()) {  // the quotient around a block: the second kernel's reach beyond its output region
+      q.qlo[k] = K[k] - 1 - K[k] / 2;
+      q.qhi[k] = K[k] / 2;
+    }
+  }
   timer.begin(stream, PASS_X_RATIO);
   plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
   timer.end(stream);
@@ -523,8 +661,7 @@ int Decon::run(int iterations) {
     MVR_HIP(hipMalloc(&statsDev, sizeof(double2) * std::max(1, need)));
     statsCap = std::max(1, need);
   }
-  const int rowsPerWG = 4;  // smallest x-pass row group
-  if (ensure_partials((int)std::max<size_t>(2048, plan.rows_inner() / rowsPerWG + 1))) return -1;
+  if (ensure_partials((int)partials_needed())) return -1;
   if (useGraph && !timer.on && iterations > 0) {
     if (!graphStats) MVR_HIP(hipMalloc(&graphStats, sizeof(double2) * V));
     if (!graphExec) {
@@ -587,7 +724,7 @@ int Decon::run_view(int v) {
     MVR_HIP(hipMalloc(&statsDev, sizeof(double2)));
     statsCap = 1;
   }
-  if (ensure_partials((int)std::max<size_t>(2048, plan.rows_inner() / 4 + 1))) return -1;
+  if (ensure_partials((int)partials_needed())) return -1;
   if (view_seq(v, statsDev)) return -1;
   double2 h;
   MVR_HIP(hipMemcpyAsync(&h, statsDev, sizeof(double2), hipMemcpyDeviceToHost, stream));
@@ -651,6 +788,7 @@ void Decon::destroy() {
   for (auto p : w) if (p) (void)hipFree(p);
   for (auto p : spec1) if (p) (void)hipFree(p);
   for (auto p : spec2) if (p) (void)hipFree(p);
+  free_blocks();
   drop_graph();
   for (void* p : {(void*)psi, (void*)psi0, (void*)S, (void*)Spsi, (void*)T2, (void*)partial, (void*)statsDev,
                   (void*)ref, (void*)relDev, (void*)graphStats})

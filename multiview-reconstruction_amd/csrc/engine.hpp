@@ -22,7 +22,16 @@ struct Dom {
   int HxP;
   int D[3];
   int m[3];
+  // Batched blocks: when btab is set, one launch runs nb blocks that share P, HxP and m.  Entry b of btab
+  // (device memory, 6 ints) is {o[3], D[3]}: the volume coordinate of the block's output region and its size
+  // (at most D above); the block's spectrum starts at b * blockStride complex values.  The launchers set
+  // wgPerBlock (work-groups per block of an x pass).
+  const int* btab = nullptr;
+  int nb = 1;
+  int wgPerBlock = 0;
+  size_t blockStride = 0;
 };
+constexpr int kBlockTabInts = 6;
 
 enum GatherMode : int { GATHER_MIRROR = 0, GATHER_VALUE = 1, GATHER_KERNEL = 2 };
 
@@ -40,10 +49,14 @@ struct GatherParams {
 
 // Inverse-transform epilogues.  `o` is the volume coordinate of the output
 // region's first voxel; N are the dims of the volume that img/psi/w index.
+// The quotient is formed on the output region widened by qlo / qhi voxels per side (the reach of the second
+// kernel, for domains whose neighbourhood lies inside the volume: blocks), within the volume; 1 elsewhere.
 struct RatioParams {
   const float* img;
   int N[3];
   int o[3];
+  int qlo[3];
+  int qhi[3];
 };
 
 struct UpdateParams {
@@ -76,6 +89,10 @@ struct StridedGeom {
   int ncols;           // valid columns (hx) per line
   int nplanes;
   int width = 0;       // tile width in columns (8 or 16; 0 = the default for the length)
+  // batched blocks: planes [b * blockPlanes, (b + 1) * blockPlanes) belong to block b, whose data starts at
+  // b * blockStride; the kernel spectrum is shared (indexed by the plane within the block).  0 = one block.
+  int blockPlanes = 0;
+  size_t blockStride = 0;
   // timing: when set, the launch records the kernel's own start and stop in these events
   hipEvent_t evStart = nullptr, evStop = nullptr;
 };
@@ -103,6 +120,8 @@ void launch_update_mul(float* psi, const float* const* T2, const float* const* w
                        int* nPartialOut, hipStream_t s);
 void launch_copy_block_zero(const float* src, const int N[3], const long long off[3], float* dst, const int B[3],
                             hipStream_t s);
+void launch_blocks_with_content(const float* w, const int N[3], const int* tab, int nblocks, int* flags,
+                                hipStream_t s);
 void launch_quotient(const float* img, float* blurred, size_t n, hipStream_t s);
 int launch_final_values(float* psi, const float* integral, const float* weight, size_t n, double lambda,
                         float minValue, float maxI, double2* partial, int maxPartials, hipStream_t s);

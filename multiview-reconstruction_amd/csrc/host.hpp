@@ -131,6 +131,19 @@ struct Decon {
   bool useGraph = false;
   hipGraphExec_t graphExec = nullptr;
   double2* graphStats = nullptr;  // per-view (Seq) or per-iteration (Mul) stats of the replayed iteration
+  // The reference's block structure (DeconView's block size B, D/DeconView.java:118-184: effective size
+  // E = B - 2K + 2, margin K - 1; BlockGeneratorFixedSizePrecise), all blocks of one view in one launch per
+  // pass.  blockB[0] == 0: one frame over the whole volume.
+  int blockB[3] = {0, 0, 0};
+  int blockE[3] = {0, 0, 0};
+  int nbAll = 0;
+  std::vector<int*> btab;   // per view: device table {o[3], D[3]} of its blocks with content
+  std::vector<int> nbView;  // blocks per view after DeconView.filterBlocksForContent
+  bool blocked() const { return blockB[0] != 0; }
+  int set_block_size(const int B[3]);
+  int build_blocks();
+  void free_blocks();
+  size_t partials_needed() const;
   // FFT lengths chosen by autotune (0 = the smallest compiled length)
   int tunedP[3] = {0, 0, 0};
   int tunedW[2] = {0, 0};  // tile widths of the y and z passes chosen by autotune (0 = default)

@@ -262,6 +262,34 @@ void launch_copy_block_zero(const float* src, const int N[3], const long long of
                      make_int3(c(off[0]), c(off[1]), c(off[2])), dst, make_int3(B[0], B[1], B[2]));
 }
 
+// DeconView.blockContainsContent (D/DeconView.java:232-274) for many blocks at once: flags[b] = 1 when the weights
+// hold a non-zero value inside block b's interval (entry b of tab: {offset[3], size[3]}, zero outside the
+// volume); flags must be zero before the launch.  Work-group (x, b) scans a strided share of block b.
+__global__ void k_blocks_with_content(const float* __restrict__ w, int3 N, const int* __restrict__ tab,
+                                      int* __restrict__ flags) {
+  const int* t = tab + 6 * blockIdx.y;
+  const int x0 = max(t[0], 0), y0 = max(t[1], 0), z0 = max(t[2], 0);
+  const int x1 = min(t[0] + t[3], N.x), y1 = min(t[1] + t[4], N.y), z1 = min(t[2] + t[5], N.z);
+  if (x1 <= x0 || y1 <= y0 || z1 <= z0) return;
+  const int nx = x1 - x0, ny = y1 - y0;
+  const size_t n = (size_t)nx * ny * (z1 - z0);
+  bool any = false;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n && !any; i += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % nx);
+    const size_t r = i / nx;
+    const int y = (int)(r % ny), z = (int)(r / ny);
+    any = w[((size_t)(z0 + z) * N.y + (y0 + y)) * N.x + (x0 + x)] != 0.f;
+  }
+  if (any) flags[blockIdx.y] = 1;  // every writer stores the same value
+}
+
+void launch_blocks_with_content(const float* w, const int N[3], const int* tab, int nblocks, int* flags,
+                                hipStream_t s) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(k_blocks_with_content, dim3(64, nblocks), dim3(kThreads), 0, s, w, make_int3(N[0], N[1], N[2]),
+                     tab, flags);
+}
+
 // DeconvolutionMethods.computeQuotient (D/iteration/sequential/DeconvolutionMethods.java:46-98) over a flat array:
 // blurred[i] = img[i] > 0 ? img[i] / blurred[i] : 1.
 __global__ void k_quotient(const float* __restrict__ img, float* __restrict__ blurred, size_t n) {

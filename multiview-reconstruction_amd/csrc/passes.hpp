@@ -146,11 +146,17 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W>())) void k_str
   unsigned tid = blockIdx.y * gridDim.x + blockIdx.x;
   const unsigned ntile = gridDim.x * gridDim.y;
   if (kStridedXcdOrder && ntile % 8 == 0) tid = (tid % 8) * (ntile / 8) + tid / 8;
-  const int plane = (int)(tid / gridDim.x);
+  int plane = (int)(tid / gridDim.x);
   const int col0 = (int)(tid % gridDim.x) * W;
   const bool colOk = w < gm.ncols - col0;
-  const float2* src = in + (size_t)plane * gm.planeStride + col0 + w;
-  float2* dst = out + (size_t)plane * gm.planeStride + col0 + w;
+  size_t blockOff = 0;  // batched blocks: the block's data offset; `plane` becomes the plane within the block
+  if (gm.blockPlanes) {
+    const int b = plane / gm.blockPlanes;
+    plane -= b * gm.blockPlanes;
+    blockOff = (size_t)b * gm.blockStride;
+  }
+  const float2* src = in + blockOff + (size_t)plane * gm.planeStride + col0 + w;
+  float2* dst = out + blockOff + (size_t)plane * gm.planeStride + col0 + w;
   const size_t ls = gm.lineStride;
   float2 v[A];
   if constexpr (MODE != 1) {

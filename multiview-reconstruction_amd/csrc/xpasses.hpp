@@ -256,7 +256,17 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
   __shared__ size_t dstRow[C::RW];
   if (C::TWL_FWD) x2_load_twiddles<M>(twS, twrS, twM, wr);
   const int nrows = d.P[1] * d.P[2];
-  const int row0 = blockIdx.x * C::RW;
+  // batched blocks: this work-group's block, its gather origin (output region origin - m) and spectrum
+  int wg = blockIdx.x;
+  int org[3] = {g.org[0], g.org[1], g.org[2]};
+  if (d.btab) {
+    const int b = blockIdx.x / d.wgPerBlock;
+    wg -= b * d.wgPerBlock;
+    const int* t = d.btab + kBlockTabInts * b;
+    for (int k = 0; k < 3; ++k) org[k] = t[k] - d.m[k];
+    S += (size_t)b * d.blockStride;
+  }
+  const int row0 = wg * C::RW;
   float* fb = reinterpret_cast<float*>(buf);
   if (g.mode == GATHER_KERNEL) {
     // kernel placement (setup only): per element
@@ -273,7 +283,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
     }
   } else {
     for (int x = threadIdx.x; x < C::PX; x += C::NT) {
-      const int p = g.org[0] + x;
+      const int p = org[0] + x;
       xmap[x] = g.mode == GATHER_MIRROR ? mirror_index(p, g.N[0]) : ((unsigned)p < (unsigned)g.N[0] ? p : -1);
     }
     if (threadIdx.x < C::RW) {
@@ -281,7 +291,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
       long long off = -1;
       if (row < nrows) {
         const int y = row % d.P[1], z = row / d.P[1];
-        int gy = g.org[1] + y, gz = g.org[2] + z;
+        int gy = org[1] + y, gz = org[2] + z;
         if (g.mode == GATHER_MIRROR) {
           gy = mirror_index(gy, g.N[1]);
           gz = mirror_index(gz, g.N[2]);
@@ -346,14 +356,27 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
   __shared__ size_t outOff[C::RW];   // spectrum row to write
   if (C::TWL_RATIO) x2_load_twiddles<M>(twS, twrS, twM, wr);
   const int nrows = d.P[1] * d.P[2];
-  const int row0 = blockIdx.x * C::RW;
+  // batched blocks: this work-group's block, its output region and spectra
+  int wg = blockIdx.x;
+  int o[3] = {q_.o[0], q_.o[1], q_.o[2]}, D[3] = {d.D[0], d.D[1], d.D[2]};
+  if (d.btab) {
+    const int b = blockIdx.x / d.wgPerBlock;
+    wg -= b * d.wgPerBlock;
+    const int* t = d.btab + kBlockTabInts * b;
+    for (int k = 0; k < 3; ++k) { o[k] = t[k]; D[k] = t[3 + k]; }
+    Sin += (size_t)b * d.blockStride;
+    Sout += (size_t)b * d.blockStride;
+  }
+  const int row0 = wg * C::RW;
   if (threadIdx.x < C::RW) {
     const int row = row0 + threadIdx.x;
     long long ir = kOuterRow;
     if (row < nrows) {
+      // quotient rows: the output region widened by qlo / qhi, inside the volume
       const int ly = row % d.P[1] - d.m[1], lz = row / d.P[1] - d.m[2];
-      if ((unsigned)ly < (unsigned)d.D[1] && (unsigned)lz < (unsigned)d.D[2])
-        ir = ((long long)(q_.o[2] + lz) * q_.N[1] + (q_.o[1] + ly)) * q_.N[0] + q_.o[0] - d.m[0];
+      if (ly >= -q_.qlo[1] && ly < D[1] + q_.qhi[1] && lz >= -q_.qlo[2] && lz < D[2] + q_.qhi[2] &&
+          (unsigned)(o[1] + ly) < (unsigned)q_.N[1] && (unsigned)(o[2] + lz) < (unsigned)q_.N[2])
+        ir = ((long long)(o[2] + lz) * q_.N[1] + (o[1] + ly)) * q_.N[0] + o[0] - d.m[0];
     }
     imgRow[threadIdx.x] = ir;
     inOff[threadIdx.x] = ir != kOuterRow ? (size_t)row * d.HxP : kNoRow;
@@ -376,7 +399,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
     // values, the flat pass is 2 % faster)
     if (q < C::B) {
       const long long ir = imgRow[r];
-      const int x0 = d.m[0], x1 = d.m[0] + d.D[0];
+      const int x0 = d.m[0] + max(-q_.qlo[0], -o[0]), x1 = d.m[0] + min(D[0] + q_.qhi[0], q_.N[0] - o[0]);
       constexpr int H = (C::A + 1) / 2;  // two halves of loads in flight
 #pragma unroll
       for (int a0 = 0; a0 < C::A; a0 += H) {
@@ -404,7 +427,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
   } else {
     x2_park_inverse<M>(rb, q, z);
     float* fb = reinterpret_cast<float*>(buf);
-    const int x0 = d.m[0], x1 = d.m[0] + d.D[0];
+    const int x0 = d.m[0] + max(-q_.qlo[0], -o[0]), x1 = d.m[0] + min(D[0] + q_.qhi[0], q_.N[0] - o[0]);
     float im[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -446,15 +469,26 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
   __shared__ long long psiRow[C::RW];
   __shared__ size_t rowOff[C::RW];
   if (C::TWL) x2_load_twiddles<M>(twS, twrS, twM, wr);
-  const int nrows = d.D[1] * d.D[2];
-  const int row0 = blockIdx.x * C::RW;
+  // batched blocks: this work-group's block, its output region and spectrum
+  int wg = blockIdx.x;
+  int o[3] = {u.o[0], u.o[1], u.o[2]}, D[3] = {d.D[0], d.D[1], d.D[2]};
+  if (d.btab) {
+    const int b = blockIdx.x / d.wgPerBlock;
+    wg -= b * d.wgPerBlock;
+    const int* t = d.btab + kBlockTabInts * b;
+    for (int k = 0; k < 3; ++k) { o[k] = t[k]; D[k] = t[3 + k]; }
+    S += (size_t)b * d.blockStride;
+    if (D[0] != d.D[0]) divD0 = FastDiv((unsigned)D[0]);  // a tail block's row length
+  }
+  const int nrows = D[1] * D[2];
+  const int row0 = wg * C::RW;
   if (threadIdx.x < C::RW) {
     const int row = row0 + threadIdx.x;
     long long pr = -1;
     size_t so = kNoRow;
     if (row < nrows) {
-      const int ly = row % d.D[1], lz = row / d.D[1];
-      pr = ((long long)(u.o[2] + lz) * u.N[1] + (u.o[1] + ly)) * u.N[0] + u.o[0];
+      const int ly = row % D[1], lz = row / D[1];
+      pr = ((long long)(o[2] + lz) * u.N[1] + (o[1] + ly)) * u.N[0] + o[0];
       so = ((size_t)(d.m[2] + lz) * d.P[1] + (d.m[1] + ly)) * d.HxP;
     }
     psiRow[threadIdx.x] = pr;   // >= 0 for output rows (the output region lies inside the volume)
@@ -468,7 +502,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
   float2 z[C::A];
   x2_inverse_half<M>(rb, tw, twr, q, z);
   double sum = 0.0, mx = -1.0;
-  const int DX = d.D[0];
+  const int DX = D[0];
   if constexpr (x2_long_rows<M>()) {
     // Long rows: thread q < B holds the values z[a] = (x[2n], x[2n + 1]), n = q + B a, of its row, and updates
     // psi there directly (no park to LDS and flat pass)
@@ -588,32 +622,41 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_store(Dom d, StoreParams st,
 }
 
 // ------------------------------------------------------------------ host launchers
+// Batched launches (d.btab set) run d.nb blocks of wgPerBlock work-groups each: work-group w serves block
+// w / wgPerBlock; rows of a block beyond its own extent are skipped inside the kernels.
 template <int M>
 void launch_x_fwd(const Dom& d, const GatherParams& g, float2* S, const float2* twM, const float2* wr,
                   hipStream_t s) {
   static_assert(x2_ok<M>(), "every compiled x length has a two-step split");
   using C = XCfg<M>;
+  Dom dd = d;
   const int rows = d.P[1] * d.P[2];
-  hipLaunchKernelGGL(k_x2_fwd<M>, dim3((rows + C::RW - 1) / C::RW), dim3(C::NT), 0, s, d, g, S, twM, wr);
+  dd.wgPerBlock = (rows + C::RW - 1) / C::RW;
+  const int nb = d.btab ? d.nb : 1;
+  hipLaunchKernelGGL(k_x2_fwd<M>, dim3(dd.wgPerBlock * nb), dim3(C::NT), 0, s, dd, g, S, twM, wr);
 }
 template <int M>
 void launch_x_ratio(const Dom& d, const RatioParams& q, const float2* Sin, float2* Sout, const float2* twM,
                     const float2* wr, hipStream_t s) {
   static_assert(x2_ok<M>(), "every compiled x length has a two-step split");
   using C = XCfg<M>;
+  Dom dd = d;
   const int rows = d.P[1] * d.P[2];
-  hipLaunchKernelGGL(k_x2_ratio<M>, dim3((rows + C::RW - 1) / C::RW), dim3(C::NT), 0, s, d, q, Sin, Sout, twM,
-                     wr);
+  dd.wgPerBlock = (rows + C::RW - 1) / C::RW;
+  const int nb = d.btab ? d.nb : 1;
+  hipLaunchKernelGGL(k_x2_ratio<M>, dim3(dd.wgPerBlock * nb), dim3(C::NT), 0, s, dd, q, Sin, Sout, twM, wr);
 }
 template <int M>
 int launch_x_update(const Dom& d, const UpdateParams& u, const float2* S, const float2* twM, const float2* wr,
                     hipStream_t s) {
   static_assert(x2_ok<M>(), "every compiled x length has a two-step split");
   using C = XCfg<M>;
+  Dom dd = d;
   const int rows = d.D[1] * d.D[2];
-  const int nb = (rows + C::RW - 1) / C::RW;
-  hipLaunchKernelGGL(k_x2_update<M>, dim3(nb), dim3(C::NT), 0, s, d, u, FastDiv((unsigned)d.D[0]), S, twM, wr);
-  return nb;
+  dd.wgPerBlock = (rows + C::RW - 1) / C::RW;
+  const int nwg = dd.wgPerBlock * (d.btab ? d.nb : 1);
+  hipLaunchKernelGGL(k_x2_update<M>, dim3(nwg), dim3(C::NT), 0, s, dd, u, FastDiv((unsigned)d.D[0]), S, twM, wr);
+  return nwg;
 }
 template <int M>
 void launch_x_store(const Dom& d, const StoreParams& st, const float2* S, const float2* twM, const float2* wr,

@@ -81,6 +81,8 @@ def lib():
         L.mvr_decon_set_view.argtypes = [vp, ctypes.c_int, vp, vp, vp, _i32p]
         L.mvr_decon_set_view_kernels.argtypes = [vp, ctypes.c_int, vp, vp, _i32p]
         L.mvr_decon_set_max.argtypes = [vp, _f32p]
+        L.mvr_decon_set_block_size.argtypes = [vp, _i32p]
+        L.mvr_decon_get_blocks.argtypes = [vp, _i32p, ctypes.c_int]
         L.mvr_decon_init.argtypes = [vp, ctypes.c_int, vp]
         L.mvr_decon_run.argtypes = [vp, ctypes.c_int]
         L.mvr_decon_get_psi.argtypes = [vp, vp]
@@ -339,6 +341,16 @@ class Decon:
         mx = np.ascontiguousarray(max_per_view, np.float32)
         assert mx.size == self.V
         _check(lib().mvr_decon_set_max(self.h, mx.ctypes.data_as(_f32p)))
+
+    def set_block_size(self, block_size_xyz):
+        """The reference's block structure (DeconView block size B per axis, x y z); (0, 0, 0): one frame."""
+        _check(lib().mvr_decon_set_block_size(self.h, (ctypes.c_int * 3)(*[int(b) for b in block_size_xyz])))
+
+    def blocks(self):
+        """(blocks in the volume, [blocks with content per view])."""
+        per = (ctypes.c_int * self.V)()
+        n = _check(lib().mvr_decon_get_blocks(self.h, per, self.V))
+        return n, list(per)
 
     PSI_INITS = {"AVG_PRECISE": 0, "FROM_ARRAY": 1, "AVG_APPROX": 2, "BLURRED_FUSED": 3}
 

@@ -212,6 +212,80 @@ def test_block_iteration_at_equals_host_cut_blocks():
     t.close()
 
 
+def _expected_blocks_with_content(ws, K, B):
+    """Per view, the blocks of BlockGeneratorFixedSizePrecise whose interval (effective region at i E, margin
+    K - 1, size B) holds a non-zero weight, counted with numpy on the zero-extended weights."""
+    V, nz, ny, nx = ws.shape
+    E = [b - 2 * k + 2 for b, k in zip(B, K)]
+    nb = [-(-n // e) for n, e in zip((nx, ny, nz), E)]
+    counts = []
+    for v in range(V):
+        c = 0
+        for k in range(nb[2]):
+            for j in range(nb[1]):
+                for i in range(nb[0]):
+                    lo = [max(0, q * e - (kk - 1)) for q, e, kk in zip((i, j, k), E, K)]
+                    hi = [min(n, q * e - (kk - 1) + b) for q, e, kk, b, n in zip((i, j, k), E, K, B, (nx, ny, nz))]
+                    c += bool(ws[v, lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]].any())
+        counts.append(c)
+    return nb[0] * nb[1] * nb[2], counts
+
+
+@pytest.mark.parametrize("psf_type,lam", [("EFFICIENT_BAYESIAN", 6e-4), ("OPTIMIZATION_II", 0.0)])
+def test_batched_blocks_match_blocked_oracle(oracle, psf_type, lam):
+    # the reference's block structure in one launch per pass: 22^3 blocks of a 60 x 26 x 30 volume with 5^3 PSFs
+    # (effective 14^3, uneven tail blocks); view 1 has no weight beyond x = 35, so DeconView's content filter
+    # drops its last two block columns
+    rng = np.random.default_rng(31)
+    V, shape, K = 3, (30, 26, 60), (5, 5, 5)
+    imgs = rng.uniform(1.0, 40.0, (V,) + shape).astype(np.float32)
+    ws = rng.uniform(0.0, 1.0, (V,) + shape).astype(np.float32)
+    ws[1, :, :, 36:] = 0.0
+    ks = np.stack([syn.gaussian_psf(5, 1.0, 2.0, a) for a in (0, 120, 240)])
+    B = (22, 22, 22)
+    ref = oracle.deconvolve(imgs, ws, ks, psf_type=psf_type, lam=lam, iterations=4, block_size=B)
+    d = hip.Decon(shape, V, psf_type, "SEQ", lam)
+    for v in range(V):
+        d.set_view(v, imgs[v], ws[v], ks[v])
+    d.set_block_size(B)
+    d.init()
+    n, per = d.blocks()
+    n_want, per_want = _expected_blocks_with_content(ws, K, B)
+    print(f"{n} blocks, per view {per} (numpy {per_want}), FFT grid {d.geometry()['P']}")
+    assert n == n_want and per == per_want and per[1] < per[0]
+    d.run(4)
+    e = rel(d.psi(), ref["psi"])
+    st = d.stats()
+    d.close()
+    print(f"{psf_type}: batched blocks vs blocked oracle rel-L2 {e:.2e}")
+    assert e < BAR, f"rel-L2 {e:.2e} (bar {BAR}, spec {SPEC})"
+    # statistics count every voxel once (effective regions); the reference's blocked drivers also count the
+    # discarded block margins, so only their form is checked here (one {sum, max} per view update)
+    assert st.shape == (4, V, 2) and np.isfinite(st).all()
+
+
+def test_batched_blocks_equal_frame_where_no_block_is_dropped():
+    # with content in every block, the batched blocks and the one-frame schedule compute the same deconvolution
+    inp = _small_problem(n=(30, 26, 28), K=5)
+    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+    out = []
+    for B in ((0, 0, 0), (22, 22, 22), (24, 20, 30)):
+        d = hip.Decon(imgs.shape[1:], 3, "EFFICIENT_BAYESIAN", "SEQ", 6e-4)
+        for v in range(3):
+            d.set_view(v, imgs[v], ws[v], ks[v])
+        d.set_block_size(B)
+        d.init()
+        d.run(3)
+        out.append((d.psi(), d.stats()))
+        d.close()
+    for o, st in out[1:]:
+        e = rel(o, out[0][0])
+        assert e < 1e-6, e
+        # each voxel updated once: the same statistics as the frame (sumChange to 1e-6 of the flux: it cancels)
+        np.testing.assert_allclose(st[..., 1], out[0][1][..., 1], rtol=1e-4)
+        assert np.abs(st[..., 0] - out[0][1][..., 0]).max() <= 1e-6 * float(out[0][0].sum(dtype=np.float64))
+
+
 def _small_problem(n=(28, 30, 26), V=3, K=7, seed=11, weights="uniform"):
     w = syn.small_workload(views=V, k=K, seed_index=seed, n_blobs=25, weights=weights, shape=n)
     return syn.make_inputs(w)
