@@ -73,6 +73,18 @@ int mvr_fft_convolve(int dev, const float* img, const int nXYZ[3], int ext, floa
  * The reference picks its padded sizes inside imglib2's FFTMethods; this list has no counterpart there. */
 int mvr_fft_lengths(int* out, int cap);
 
+/* ---- mvr_fft_convolve without allocation.  No counterpart in the reference: for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  img, kernel and out must be device memory
+ * of `dev` (refused otherwise); the twiddle tables, the kernel spectrum and the work spectrum live in one static
+ * device array of MVR_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, so FFT grids up to about 180^3
+ * fit (e.g. a 128^3 block with a 41^3 kernel); larger ones are refused with an error before any device work.
+ * The kernels are enqueued on `stream` (a hipStream_t; NULL = the default stream) and the call returns without
+ * waiting: synchronise the stream before reading out.  All calls on one GPU share its array, so calls that
+ * overlap on one GPU race: serialise them (one stream). ---- */
+#define MVR_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int mvr_fft_convolve_noalloc(int dev, const float* img, const int nXYZ[3], int ext, float extValue,
+                             const float* kernel, const int kXYZ[3], float* out, void* stream);
+
 /* ---- DeconvolutionMethods on whole arrays (deconvolution/iteration/sequential/DeconvolutionMethods.java): the
  * elementwise steps that the deconvolution passes fuse into their transforms, run alone over n values in flat order
  * (same float and double rounding as the Java methods, no FMA contraction).  stats = {sumChange, maxChange} of

@@ -55,6 +55,8 @@ def lib():
         L.mvr_convolution3d_fft_inplace.argtypes = [vp, _i32p, vp, _i32p, ctypes.c_int]
         L.mvr_fft_convolve.argtypes = [ctypes.c_int, vp, _i32p, ctypes.c_int, ctypes.c_float, vp, _i32p, vp]
         L.mvr_fft_lengths.argtypes = [vp, ctypes.c_int]
+        L.mvr_fft_convolve_noalloc.argtypes = [ctypes.c_int, vp, _i32p, ctypes.c_int, ctypes.c_float, vp, _i32p, vp,
+                                               vp]
         L.mvr_compute_quotient.argtypes = [ctypes.c_int, vp, vp, ctypes.c_longlong]
         L.mvr_compute_final_values.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_longlong, ctypes.c_double,
                                                ctypes.c_float, ctypes.c_float, _f64p]
@@ -190,6 +192,16 @@ def fft_convolve(img, kernel, ext="mirror", value=0.0, dev=0):
     _check(lib().mvr_fft_convolve(dev, _ptr(img), _dims(img.shape), e, value, _ptr(kernel), _dims(kernel.shape),
                                   _ptr(out)))
     return out
+
+
+def fft_convolve_noalloc(img_ptr, shape_zyx, kernel_ptr, kshape_zyx, out_ptr, ext="mirror", value=0.0, stream=None,
+                         dev=0):
+    """mvr_fft_convolve_noalloc on device pointers (e.g. torch tensors' data_ptr()); enqueued on `stream`
+    (None = the default stream), no device memory allocated.  The caller synchronises before reading out."""
+    e = 0 if ext == "mirror" else 1
+    _check(lib().mvr_fft_convolve_noalloc(dev, ctypes.c_void_p(img_ptr), _dims(shape_zyx), e, value,
+                                          ctypes.c_void_p(kernel_ptr), _dims(kshape_zyx), ctypes.c_void_p(out_ptr),
+                                          ctypes.c_void_p(stream) if stream else None))
 
 
 def convolution3d_fft_inplace(img, kernel, dev=0):

@@ -83,3 +83,13 @@ def test_argument_errors_are_reported_before_any_device_work():
         hip.fuse_group([], [], (0, 0, 0), (4, 4, 4))
     with pytest.raises(hip.MvrError, match="empty bounding box"):
         hip.fuse_group([img], [eye], (0, 0, 0), (0, 4, 4))
+
+
+def test_noalloc_refuses_oversized_grids_before_device_work():
+    # the scratch layout is checked on the host: a 200^3 volume with a 41^3 kernel needs a 240^3 grid
+    fake = ctypes.c_void_p(256)     # never dereferenced: the call ends at the size check
+    rc = hip.lib().mvr_fft_convolve_noalloc(0, fake, hip._dims((200, 200, 200)), 0, 0.0, fake, hip._dims((41, 41, 41)),
+                                            fake, None)
+    assert rc < 0
+    msg = hip.lib().mvr_last_error().decode()
+    assert "scratch" in msg and "240x240x240" in msg
