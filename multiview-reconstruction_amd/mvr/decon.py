@@ -93,7 +93,19 @@ def _mirror_idx(p, n):
 
 
 def copy_block_mirror(psi, b: Block):
-    """Block.copyBlock from Views.extendMirrorSingle(psi) (C/Block.java:158-197, 277-315)."""
+    """Block.copyBlock from Views.extendMirrorSingle(psi) (C/Block.java:158-197, 277-315).  Blocks that reach at
+    most one period beyond the volume are one reflect-pad (mirror-single) and a slice; others index per axis."""
+    lo = [max(0, -b.offset[d]) for d in range(3)]
+    hi = [max(0, b.offset[d] + b.block_size[d] - psi.shape[2 - d]) for d in range(3)]
+    src = [slice(max(0, b.offset[d]), min(psi.shape[2 - d], b.offset[d] + b.block_size[d])) for d in range(3)]
+    size = [s_.stop - s_.start for s_ in src]
+    # a padded side starts (ends) at the volume's first (last) voxel, so reflect-padding the slice mirrors the
+    # volume as long as the slice is longer than the padding on each side
+    if all(lo[d] < size[d] and hi[d] < size[d] for d in range(3)):
+        core = psi[src[2], src[1], src[0]]
+        if not any(lo) and not any(hi):
+            return np.ascontiguousarray(core)
+        return np.ascontiguousarray(np.pad(core, [(lo[2], hi[2]), (lo[1], hi[1]), (lo[0], hi[0])], mode="reflect"))
     idx = [_mirror_idx(np.arange(b.block_size[d]) + b.offset[d], psi.shape[2 - d]) for d in range(3)]
     return np.ascontiguousarray(psi[np.ix_(idx[2], idx[1], idx[0])])
 

@@ -93,3 +93,16 @@ def test_noalloc_refuses_oversized_grids_before_device_work():
     assert rc < 0
     msg = hip.lib().mvr_last_error().decode()
     assert "scratch" in msg and "240x240x240" in msg
+
+
+def test_mirror_copy_fast_path_equals_index_form():
+    # the reflect-pad form of Block.copyBlock equals mirror-single indexing of the whole volume, also where a
+    # block reaches further than the volume is long (index form) and for one-voxel axes
+    rng = np.random.default_rng(3)
+    for shp in [(17, 19, 23), (1, 5, 9), (2, 2, 2)]:
+        psi = rng.normal(size=shp).astype(np.float32)
+        for off in [(-4, -4, -4), (0, 0, 0), (3, 2, 1), (-10, 5, -3), (10, 10, 10), (-30, 0, 0), (20, -2, 15)]:
+            for bs in [(12, 12, 12), (31, 27, 25), (5, 5, 5), (40, 8, 9), (1, 1, 1)]:
+                b = decon.Block(bs, off, bs, off, (0, 0, 0))
+                idx = [decon._mirror_idx(np.arange(bs[d]) + off[d], shp[2 - d]) for d in range(3)]
+                np.testing.assert_array_equal(decon.copy_block_mirror(psi, b), psi[np.ix_(idx[2], idx[1], idx[0])])
