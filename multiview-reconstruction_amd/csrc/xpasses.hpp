@@ -1,16 +1,15 @@
 // xpasses.hpp -- x (real axis) passes with the two-step register FFT.
 //
-// A work-group owns RW rows of one x-pass.  The length-M complex transform of
-// a row (M = P0/2, real FFT by the even/odd split) runs as M = A x B with one
-// LDS exchange, and the C2R pre-processing is folded into the first inverse
-// step.  Spectra are staged through LDS with flat, coalesced mappings (row
-// data is contiguous in HBM).  Real-valued data meets the transforms in the
-// registers where that is possible: the forward pass reads psi straight into
-// its first transform step, and on long rows (B > 16) the quotient and the
-// psi update work on the values the inverse leaves in registers; otherwise
-// they run as flat passes over the row in LDS.  Index arithmetic that depends
-// on runtime sizes is done once per row (row tables in LDS) or with a
-// multiply-shift divider, never per element with '/' or '%'.
+// A work-group owns RW rows of one x-pass; each row is served by T lanes, T the power of two >= A, so that a
+// row never straddles a wave.  The length-M complex transform of a row (M = P0/2, real FFT by the even/odd
+// split) runs as M = A x B with one LDS exchange.  The R2C post-processing and the C2R pre-processing combine
+// the values k and M - k of a row; after the second forward step (before the first inverse step) lane q holds
+// index q + A k2 in register k2, and M - (q + A k2) = (A - q) + A (B - 1 - k2), so the partner is lane A - q,
+// register B - 1 - k2 of the same row: the pairs are swapped with wave shuffles, and spectra go between HBM
+// and registers directly, coalesced across the row's lanes.  Real-valued data meets the transforms in
+// registers too: psi is read into the first forward step, and the quotient and the psi update work on the
+// values the inverse leaves in lane q < B (x[2n], x[2n + 1] for n = q + B a).  Index arithmetic that depends on
+// runtime sizes is done once per row (row tables in LDS), never per element with '/' or '%'.
 #pragma once
 #include "passes.hpp"
 
@@ -55,97 +54,60 @@ __device__ __forceinline__ void block_reduce_stats_nt(double s, double mx, doubl
 }
 
 constexpr long long kOuterRow = -(1ll << 62);
+constexpr size_t kNoRow = ~(size_t)0;
 
 template <int M>
 struct XCfg {
   static constexpr Split2 S = split2(M);
   static constexpr int A = S.A, B = S.B;
-  static constexpr int T = A;                      // threads per row
-  static constexpr int RW = 256 / A;               // rows per work-group
-  static constexpr int NT = RW * T;                // threads per work-group
-  static constexpr int RS = M + 5;                 // LDS row stride (complex): M + 1 used, +4 against bank conflicts
+  static constexpr int T = A <= 16 ? 16 : A <= 32 ? 32 : 64;  // lanes per row (a power of two, divides 64)
+  static constexpr int RW = 256 / T;                          // rows per work-group
+  static constexpr int NT = RW * T;                           // threads per work-group
+  static constexpr int RS = M + 5;                 // LDS row stride (complex): M used, +5 against bank conflicts
   static constexpr int PX = 2 * M;
-  // x_update: twiddle tables (2M + 1 complex) in LDS unless leaving them in global memory (read through
-  // L1) fits one more work-group per CU into the 160 KB of LDS; its other LDS is the rows and 16 B of row
-  // tables per row (the stats reduction reuses the rows).
-  static constexpr int LDS_ROWS = RW * RS * 8 + RW * 16;
+  // the exchange rows, at least as large as the stats reduction that reuses them (2 x 256 doubles)
+  static constexpr int BUF = RW * RS > 512 ? RW * RS : 512;
+  // Twiddle tables (2M + 1 complex) stay in LDS unless leaving them in global memory (read through L1) fits one
+  // more work-group per CU into the 160 KB of LDS; the other LDS is the exchange rows and 16-24 B of row tables
+  // per row (the stats reduction reuses the rows), plus the x gather map in the forward pass.
+  static constexpr int LDS_ROWS = RW * RS * 8 + RW * 24;
   static constexpr int LDS_TW = (2 * M + 1) * 8;
   static constexpr bool TWL = (160 * 1024) / (LDS_ROWS + LDS_TW) >= (160 * 1024) / LDS_ROWS;
-  // The forward passes follow the same rule on long rows (B > 16; their other LDS: the x gather map, 8 more
-  // bytes of row tables in the quotient pass): at M = 576 both gain a third work-group per CU and run 2-5 %
-  // faster.  On shorter rows they keep LDS twiddles: at M = 150 global twiddles were slower (x_fwd 5.9 ->
-  // 6.2, x_ratio 8.8 -> 9.9 ms per 256^3 run) despite the extra work-group.
-  static constexpr bool long_rows = S.B > 16;
+  // The forward and quotient passes follow the same rule on long rows (B > 16; at M = 576 global twiddles gave
+  // both a third work-group per CU and 2-5 %) and keep LDS twiddles on shorter rows (at M = 150 global twiddles
+  // were slower despite the extra work-group).
+  static constexpr bool long_rows = B > 16;
   static constexpr bool TWL_FWD =
       !long_rows || (160 * 1024) / (LDS_ROWS + PX * 4 + LDS_TW) >= (160 * 1024) / (LDS_ROWS + PX * 4);
-  static constexpr bool TWL_RATIO =
-      !long_rows || (160 * 1024) / (LDS_ROWS + RW * 8 + LDS_TW) >= (160 * 1024) / (LDS_ROWS + RW * 8);
+  static constexpr bool TWL_RATIO = !long_rows || TWL;
 };
-
-// Rows whose second split factor B exceeds 16 (M >= 360 except 384 and 512): every lane of a row holds
-// values after the transforms, and x_ratio / x_update work on them in registers instead of a flat LDS pass
-template <int M>
-__host__ __device__ constexpr bool x2_long_rows() {
-  return split2(M).B > 16;
-}
 
 template <int M>
 __host__ __device__ constexpr bool x2_ok() {
   return split2(M).A > 0 && split2(M).B > 1;
 }
 
-// forward length-M transform of LDS row `rb` (natural order) -> out[k2] = Z[q + A k2] (thread q < A)
-template <int M>
-__device__ __forceinline__ void x2_forward(float2* rb, const float2* tw, int q, float2* out) {
-  using C = XCfg<M>;
-  constexpr int A = C::A, B = C::B;
-  float2 v[A];
-  if (q < B) {
-#pragma unroll
-    for (int a = 0; a < A; ++a) v[a] = rb[q + B * a];
-    rdft<A, -1>(v);
-  }
-  __syncthreads();
-  if (q < B) {
-#pragma unroll
-    for (int k1 = 0; k1 < A; ++k1) rb[q + B * k1] = k1 == 0 ? v[k1] : cmul(v[k1], tw[q * k1]);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int b = 0; b < B; ++b) out[b] = rb[b + B * q];
-  rdft<B, -1>(out);
+__device__ __forceinline__ float2 shfl2(float2 v, int src) {
+  return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
 }
 
-// inverse of a half spectrum X[0..M] held in LDS row `rb`: the C2R pre-processing
-// Z[k] = E[k] + i O[k] is applied on load; returns z[q + B n1] (thread q < B) in out[n1]
+// For values V[q + A k2] held as v[k2] by lane q < A of a row (and V[M] as vM in lane 0): out[k2] = V[M - (q + A k2)].
+// Lane q > 0 gets them from lane A - q (register B - 1 - k2), lane 0 from itself (M - A k2 = A (B - k2)).
+// Every lane of the wave must call it (wave shuffle); idle lanes (q >= A) read themselves.
 template <int M>
-__device__ __forceinline__ void x2_inverse_half(float2* rb, const float2* tw, const float2* twr, int q, float2* out) {
+__device__ __forceinline__ void x2_mirror_pairs(const float2* v, float2 vM, int q, float2* out) {
   using C = XCfg<M>;
-  constexpr int A = C::A, B = C::B;
-  float2 u[B];
+  const int lane = (int)(threadIdx.x & 63);
+  const int src = (lane & ~(C::T - 1)) + (q > 0 && q < C::A ? C::A - q : q);
 #pragma unroll
-  for (int k2 = 0; k2 < B; ++k2) {
-    const int k = q + A * k2;
-    const float2 a = rb[k];
-    const float2 b = cconj(rb[M - k]);
-    const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
-    const float2 o = cmul(make_float2(0.5f * (a.x - b.x), 0.5f * (a.y - b.y)), cconj(twr[k]));
-    u[k2] = make_float2(e.x - o.y, e.y + o.x);
-  }
-  rdft<B, 1>(u);
-  __syncthreads();
-#pragma unroll
-  for (int n2 = 0; n2 < B; ++n2) rb[q + A * n2] = n2 == 0 ? u[n2] : cmul(u[n2], cconj(tw[n2 * q]));
-  __syncthreads();
-  if (q < B) {
-#pragma unroll
-    for (int k1 = 0; k1 < A; ++k1) out[k1] = rb[k1 + A * q];
-    rdft<A, 1>(out);
+  for (int k2 = 0; k2 < C::B; ++k2) {
+    const float2 t = shfl2(v[C::B - 1 - k2], src);
+    out[k2] = q != 0 ? t : (k2 == 0 ? vM : v[(C::B - k2) % C::B]);
   }
 }
 
-// forward transform whose first-step inputs v[a] = z[q + B a] (thread q < B) are already in registers;
-// LDS row `rb` is only written -> out[k2] = Z[q + A k2] (thread q < A)
+// forward transform whose first-step inputs v[a] = z[q + B a] (lane q < B) are in registers; LDS row `rb` is
+// only written -> out[k2] = Z[q + A k2] (lane q < A)
 template <int M>
 __device__ __forceinline__ void x2_forward_regs(float2* rb, const float2* tw, int q, float2* v, float2* out) {
   using C = XCfg<M>;
@@ -156,48 +118,96 @@ __device__ __forceinline__ void x2_forward_regs(float2* rb, const float2* tw, in
     for (int k1 = 0; k1 < A; ++k1) rb[q + B * k1] = k1 == 0 ? v[k1] : cmul(v[k1], tw[q * k1]);
   }
   __syncthreads();
+  if (q < A) {
 #pragma unroll
-  for (int b = 0; b < B; ++b) out[b] = rb[b + B * q];
-  rdft<B, -1>(out);
+    for (int b = 0; b < B; ++b) out[b] = rb[b + B * q];
+    rdft<B, -1>(out);
+  } else {
+#pragma unroll
+    for (int b = 0; b < B; ++b) out[b] = make_float2(0.f, 0.f);
+  }
 }
 
-// registers -> natural order in LDS.  Forward result: Z[q + A k2] from out[k2].
+// forward transform of LDS row `rb` (natural order) -> out[k2] = Z[q + A k2] (lane q < A)
 template <int M>
-__device__ __forceinline__ void x2_park_forward(float2* rb, int q, const float2* z) {
+__device__ __forceinline__ void x2_forward(float2* rb, const float2* tw, int q, float2* out) {
   using C = XCfg<M>;
-  __syncthreads();
-#pragma unroll
-  for (int k2 = 0; k2 < C::B; ++k2) rb[q + C::A * k2] = z[k2];
-  __syncthreads();
-}
-// inverse result: z[q + B n1] from out[n1] (thread q < B)
-template <int M>
-__device__ __forceinline__ void x2_park_inverse(float2* rb, int q, const float2* z) {
-  using C = XCfg<M>;
-  __syncthreads();
+  float2 v[C::A];
   if (q < C::B) {
 #pragma unroll
-    for (int n1 = 0; n1 < C::A; ++n1) rb[q + C::B * n1] = z[n1];
+    for (int a = 0; a < C::A; ++a) v[a] = rb[q + C::B * a];
   }
-  __syncthreads();
+  __syncthreads();  // every lane has read its values before the row is overwritten
+  x2_forward_regs<M>(rb, tw, q, v, out);
 }
 
-// R2C post-processing of every row in LDS, written to HBM with a flat mapping
+// R2C post-processing of a row whose complex transform Z lane q holds as z[k2] = Z[q + A k2]:
+// X[k] = (Z[k] + Z*[M - k]) / 2 + W_P0^k (Z[k] - Z*[M - k]) / 2i for k = q + A k2, and X[M] (lane 0), written
+// to dst (the row's start in HBM) when rowOk
 template <int M>
-__device__ __forceinline__ void x2_post_store(const float2* buf, const float2* twr, const size_t* dstRow,
-                                              float2* __restrict__ S) {
+__device__ __forceinline__ void x2_post_store_regs(const float2* z, int q, const float2* twr, float2* __restrict__ dst,
+                                                   bool rowOk) {
   using C = XCfg<M>;
-  for (int idx = threadIdx.x; idx < C::RW * (M + 1); idx += C::NT) {
-    const int r = idx / (M + 1), k = idx - r * (M + 1);
-    const size_t o = dstRow[r];
-    if (o == ~(size_t)0) continue;  // no output row
-    const float2* rb = buf + r * C::RS;
-    const float2 zk = rb[k == M ? 0 : k];
-    const float2 zc = cconj(rb[k == 0 ? 0 : M - k]);
-    const float2 e = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
-    const float2 dd = csub(zk, zc);
+  float2 zc[C::B];
+  x2_mirror_pairs<M>(z, z[0], q, zc);  // Z[M] = Z[0]
+  if (!rowOk || q >= C::A) return;
+#pragma unroll
+  for (int k2 = 0; k2 < C::B; ++k2) {
+    const int k = q + C::A * k2;
+    const float2 zk = z[k2];
+    const float2 c = cconj(zc[k2]);
+    const float2 e = make_float2(0.5f * (zk.x + c.x), 0.5f * (zk.y + c.y));
+    const float2 dd = csub(zk, c);
     const float2 od = make_float2(0.5f * dd.y, -0.5f * dd.x);
-    S[o + k] = cadd(e, cmul(twr[k], od));
+    dst[k] = cadd(e, cmul(twr[k], od));
+  }
+  if (q == 0) {
+    const float2 zk = z[0], c = cconj(z[0]);
+    const float2 e = make_float2(0.5f * (zk.x + c.x), 0.5f * (zk.y + c.y));
+    const float2 dd = csub(zk, c);
+    const float2 od = make_float2(0.5f * dd.y, -0.5f * dd.x);
+    dst[M] = cadd(e, cmul(twr[M], od));
+  }
+}
+
+// inverse of the half spectrum X[0..M] of one row (src: its start in HBM; null: a row of zeros), read straight
+// into registers: the C2R pre-processing Z[k] = E[k] + i O[k] combines X[k] and X*[M - k]; returns
+// z[q + B n1] in out[n1] (lane q < B)
+template <int M>
+__device__ __forceinline__ void x2_inverse_half_regs(const float2* __restrict__ src, float2* rb, const float2* tw,
+                                                     const float2* twr, int q, float2* out) {
+  using C = XCfg<M>;
+  constexpr int A = C::A, B = C::B;
+  float2 x[B];
+  float2 xM = make_float2(0.f, 0.f);
+  const bool ld = src != nullptr && q < A;
+#pragma unroll
+  for (int k2 = 0; k2 < B; ++k2) x[k2] = ld ? src[q + A * k2] : make_float2(0.f, 0.f);
+  if (ld && q == 0) xM = src[M];
+  float2 xc[B];
+  x2_mirror_pairs<M>(x, xM, q, xc);
+  float2 u[B];
+  const int qq = q < A ? q : 0;  // idle lanes compute on a valid twiddle index and store nothing
+#pragma unroll
+  for (int k2 = 0; k2 < B; ++k2) {
+    const int k = qq + A * k2;
+    const float2 a = x[k2];
+    const float2 b = cconj(xc[k2]);
+    const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+    const float2 o = cmul(make_float2(0.5f * (a.x - b.x), 0.5f * (a.y - b.y)), cconj(twr[k]));
+    u[k2] = make_float2(e.x - o.y, e.y + o.x);
+  }
+  rdft<B, 1>(u);
+  __syncthreads();  // the row buffer is free (previous readers are done)
+  if (q < A) {
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) rb[q + A * n2] = n2 == 0 ? u[n2] : cmul(u[n2], cconj(tw[n2 * q]));
+  }
+  __syncthreads();
+  if (q < B) {
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) out[k1] = rb[k1 + A * q];
+    rdft<A, 1>(out);
   }
 }
 
@@ -208,36 +218,16 @@ __device__ __forceinline__ void x2_load_twiddles(float2* tw, float2* twr, const 
   for (int i = threadIdx.x; i <= M; i += C::NT) twr[i] = wr[i];
 }
 
-// ------------------------------------------------------------------ pipelined HBM <-> LDS staging
-// Every loop below has a compile-time trip count and issues all of its global
-// loads before it consumes any of them (lanes without data read a valid dummy
-// address and discard the value), so each wave keeps many loads in flight.
-constexpr size_t kNoRow = ~(size_t)0;
-
-// rows of M+1 complex from S into LDS rows of stride RS; rows with offset kNoRow become zero
+// the inverse result z[q + B n1] (lane q < B, register n1) into LDS row `rb` in natural order
 template <int M>
-__device__ __forceinline__ void x2_stage_spectrum(float2* buf, const float2* __restrict__ S, const size_t* rowOff) {
+__device__ __forceinline__ void x2_park_inverse(float2* rb, int q, const float2* z) {
   using C = XCfg<M>;
-  constexpr int TOT = C::RW * (M + 1);
-  constexpr int IT = (TOT + C::NT - 1) / C::NT;
-  float2 v[IT];
+  __syncthreads();  // every lane has read the row's exchange values
+  if (q < C::B) {
 #pragma unroll
-  for (int it = 0; it < IT; ++it) {
-    const int idx = threadIdx.x + it * C::NT;
-    const int r = min(idx / (M + 1), C::RW - 1), k = idx - r * (M + 1);
-    const size_t o = rowOff[r];
-    const bool ok = idx < TOT && o != kNoRow;
-    v[it] = S[ok ? o + k : 0];
-    if (!ok) v[it] = make_float2(0.f, 0.f);
+    for (int n1 = 0; n1 < C::A; ++n1) rb[q + C::B * n1] = z[n1];
   }
-#pragma unroll
-  for (int it = 0; it < IT; ++it) {
-    const int idx = threadIdx.x + it * C::NT;
-    if (idx < TOT) {
-      const int r = idx / (M + 1), k = idx - r * (M + 1);
-      buf[r * C::RS + k] = v[it];
-    }
-  }
+  __syncthreads();
 }
 
 // ------------------------------------------------------------------ forward with gather
@@ -246,7 +236,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
                                                          const float2* __restrict__ twM,
                                                          const float2* __restrict__ wr) {
   using C = XCfg<M>;
-  __shared__ float2 buf[C::RW * C::RS];
+  __shared__ float2 buf[C::BUF];
   __shared__ float2 twS[C::TWL_FWD ? M : 1];
   __shared__ float2 twrS[C::TWL_FWD ? M + 1 : 1];
   const float2* tw = C::TWL_FWD ? twS : twM;
@@ -267,20 +257,25 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
     S += (size_t)b * d.blockStride;
   }
   const int row0 = wg * C::RW;
-  float* fb = reinterpret_cast<float*>(buf);
+  const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
+  float2* rb = buf + r * C::RS;
+  float2 z[C::B];
   if (g.mode == GATHER_KERNEL) {
-    // kernel placement (setup only): per element
+    // kernel placement (setup only): per element into LDS, then the transform from LDS
+    float* fb = reinterpret_cast<float*>(buf);
     for (int idx = threadIdx.x; idx < C::RW * C::PX; idx += C::NT) {
-      const int r = idx / C::PX, x = idx - r * C::PX;
-      const int row = row0 + r;
+      const int rr = idx / C::PX, x = idx - rr * C::PX;
+      const int row = row0 + rr;
       float v = 0.f;
       if (row < nrows) v = gather_value(g, d, x, row % d.P[1], row / d.P[1]);
-      fb[r * 2 * C::RS + x] = v;
+      fb[rr * 2 * C::RS + x] = v;
     }
     if (threadIdx.x < C::RW) {
       const int row = row0 + threadIdx.x;
       dstRow[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : kNoRow;
     }
+    __syncthreads();
+    x2_forward<M>(rb, tw, q, z);
   } else {
     for (int x = threadIdx.x; x < C::PX; x += C::NT) {
       const int p = org[0] + x;
@@ -290,8 +285,8 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
       const int row = row0 + threadIdx.x;
       long long off = -1;
       if (row < nrows) {
-        const int y = row % d.P[1], z = row / d.P[1];
-        int gy = org[1] + y, gz = org[2] + z;
+        const int y = row % d.P[1], zz = row / d.P[1];
+        int gy = org[1] + y, gz = org[2] + zz;
         if (g.mode == GATHER_MIRROR) {
           gy = mirror_index(gy, g.N[1]);
           gz = mirror_index(gz, g.N[2]);
@@ -305,7 +300,6 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
     }
     __syncthreads();
     // the first transform step reads its values z[q + B a] = (x[2n], x[2n + 1]) straight from psi
-    const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
     float2 v[C::A];
     if (q < C::B) {
       const long long off = srcRow[r];
@@ -320,24 +314,14 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
         if (!ok1) v[a].y = g.value;
       }
     }
-    float2* rb = buf + r * C::RS;
-    float2 z[C::B];
     x2_forward_regs<M>(rb, tw, q, v, z);
-    x2_park_forward<M>(rb, q, z);
-    x2_post_store<M>(buf, twr, dstRow, S);
-    return;
   }
-  __syncthreads();
-  const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
-  float2* rb = buf + r * C::RS;
-  float2 z[C::B];
-  x2_forward<M>(rb, tw, q, z);
-  x2_park_forward<M>(rb, q, z);
-  x2_post_store<M>(buf, twr, dstRow, S);
+  const size_t o = dstRow[r];
+  x2_post_store_regs<M>(z, q, twr, S + (o == kNoRow ? 0 : o), o != kNoRow);
 }
 
 // ------------------------------------------------------------------ inverse, quotient, forward
-// ratio = img > 0 ? img / blurred : 1 on the output region, 1 elsewhere
+// ratio = img > 0 ? img / blurred : 1 on the quotient region, 1 elsewhere
 // (DeconvolutionMethods.computeQuotient, D/iteration/sequential/DeconvolutionMethods.java:46-98)
 template <int M>
 __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_, const float2* __restrict__ Sin,
@@ -345,15 +329,14 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
                                                            const float2* __restrict__ twM,
                                                            const float2* __restrict__ wr) {
   using C = XCfg<M>;
-  __shared__ float2 buf[C::RW * C::RS];
+  __shared__ float2 buf[C::BUF];
   __shared__ float2 twS[C::TWL_RATIO ? M : 1];
   __shared__ float2 twrS[C::TWL_RATIO ? M + 1 : 1];
   const float2* tw = C::TWL_RATIO ? twS : twM;
   const float2* twr = C::TWL_RATIO ? twrS : wr;
   // img index of FFT position x is imgRow + x (may be negative: x starts at m0); kOuterRow: ratio row of ones
   __shared__ long long imgRow[C::RW];
-  __shared__ size_t inOff[C::RW];    // spectrum row to read (kNoRow: outer row, not read)
-  __shared__ size_t outOff[C::RW];   // spectrum row to write
+  __shared__ size_t outOff[C::RW];   // spectrum row (read and written; kNoRow: beyond the grid)
   if (C::TWL_RATIO) x2_load_twiddles<M>(twS, twrS, twM, wr);
   const int nrows = d.P[1] * d.P[2];
   // batched blocks: this work-group's block, its output region and spectra
@@ -379,27 +362,52 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
         ir = ((long long)(o[2] + lz) * q_.N[1] + (o[1] + ly)) * q_.N[0] + o[0] - d.m[0];
     }
     imgRow[threadIdx.x] = ir;
-    inOff[threadIdx.x] = ir != kOuterRow ? (size_t)row * d.HxP : kNoRow;
     outOff[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : kNoRow;
   }
   __syncthreads();
-  x2_stage_spectrum<M>(buf, Sin, inOff);
-  __syncthreads();
   const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
   float2* rb = buf + r * C::RS;
+  const long long ir = imgRow[r];
+  const size_t so = outOff[r];
+  // FFT positions x of the row whose image value is read: the quotient region, inside the volume
+  const int x0f = d.m[0] + max(-q_.qlo[0], -o[0]), x1f = d.m[0] + min(D[0] + q_.qhi[0], q_.N[0] - o[0]);
+  // rows outside the quotient region are a row of ones after the quotient: their blurred values are not read
   float2 z[C::A];
-  x2_inverse_half<M>(rb, tw, twr, q, z);
-  constexpr int TOT = C::RW * C::PX;
-  constexpr int IT = (TOT + C::NT - 1) / C::NT;
+  x2_inverse_half_regs<M>(ir != kOuterRow ? Sin + so : nullptr, rb, tw, twr, q, z);
   float2 zf[C::B];
-  if constexpr (x2_long_rows<M>()) {
-    // Long rows: thread q < B holds blurred values z[a] = (x[2n], x[2n + 1]), n = q + B a, exactly the
-    // inputs of its first forward step, so the quotient is formed in registers with the image read straight
-    // from HBM (at M = 576 11 % faster than the flat pass below; at M = 150, where only B of A lanes hold
-    // values, the flat pass is 2 % faster)
+  if constexpr (!C::long_rows) {
+    // short rows (only B of T lanes hold values): park the blurred rows in LDS, form the quotient with a flat
+    // mapping over every thread, and run the forward transform from LDS (at M = 150 9 % faster than the
+    // quotient in registers)
+    x2_park_inverse<M>(rb, q, z);
+    float* fb = reinterpret_cast<float*>(buf);
+    constexpr int TOT = C::RW * C::PX;
+    constexpr int IT = (TOT + C::NT - 1) / C::NT;
+    float im[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int idx = threadIdx.x + it * C::NT;
+      const int rr = min(idx / C::PX, C::RW - 1), x = idx - rr * C::PX;
+      const long long irr = imgRow[rr];
+      const bool ok = idx < TOT && irr != kOuterRow && x >= x0f && x < x1f;
+      im[it] = q_.img[ok ? irr + x : 0];
+      if (!ok) im[it] = 0.f;
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int idx = threadIdx.x + it * C::NT;
+      if (idx < TOT) {
+        const int rr = idx / C::PX, x = idx - rr * C::PX;
+        float* p = fb + rr * 2 * C::RS + x;
+        *p = im[it] > 0.f ? im[it] / *p : 1.f;
+      }
+    }
+    __syncthreads();
+    x2_forward<M>(rb, tw, q, zf);
+  } else {
+    // long rows: lane q < B holds blurred values z[a] = (x[2n], x[2n + 1]), n = q + B a, exactly the inputs of its
+    // first forward step: the quotient is formed there, with the image read straight from HBM
     if (q < C::B) {
-      const long long ir = imgRow[r];
-      const int x0 = d.m[0] + max(-q_.qlo[0], -o[0]), x1 = d.m[0] + min(D[0] + q_.qhi[0], q_.N[0] - o[0]);
       constexpr int H = (C::A + 1) / 2;  // two halves of loads in flight
 #pragma unroll
       for (int a0 = 0; a0 < C::A; a0 += H) {
@@ -408,7 +416,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
         for (int j = 0; j < 2 * H; ++j) {
           const int a = a0 + (j >> 1);
           const int x = 2 * (q + C::B * a) + (j & 1);
-          const bool ok = a < C::A && ir != kOuterRow && x >= x0 && x < x1;
+          const bool ok = a < C::A && ir != kOuterRow && x >= x0f && x < x1f;
           im[j] = q_.img[ok ? ir + x : 0];
           if (!ok) im[j] = 0.f;
         }
@@ -424,34 +432,8 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
     }
     __syncthreads();  // the inverse's reads of the row are done before the forward writes it
     x2_forward_regs<M>(rb, tw, q, z, zf);
-  } else {
-    x2_park_inverse<M>(rb, q, z);
-    float* fb = reinterpret_cast<float*>(buf);
-    const int x0 = d.m[0] + max(-q_.qlo[0], -o[0]), x1 = d.m[0] + min(D[0] + q_.qhi[0], q_.N[0] - o[0]);
-    float im[IT];
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int idx = threadIdx.x + it * C::NT;
-      const int rr = min(idx / C::PX, C::RW - 1), x = idx - rr * C::PX;
-      const long long ir = imgRow[rr];
-      const bool ok = idx < TOT && ir != kOuterRow && x >= x0 && x < x1;
-      im[it] = q_.img[ok ? ir + x : 0];
-      if (!ok) im[it] = 0.f;
-    }
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int idx = threadIdx.x + it * C::NT;
-      if (idx < TOT) {
-        const int rr = idx / C::PX, x = idx - rr * C::PX;
-        float* p = fb + rr * 2 * C::RS + x;
-        *p = im[it] > 0.f ? im[it] / *p : 1.f;
-      }
-    }
-    __syncthreads();
-    x2_forward<M>(rb, tw, q, zf);
   }
-  x2_park_forward<M>(rb, q, zf);
-  x2_post_store<M>(buf, twr, outOff, Sout);
+  x2_post_store_regs<M>(zf, q, twr, Sout + (so == kNoRow ? 0 : so), so != kNoRow);
 }
 
 // ------------------------------------------------------------------ inverse + psi update (output rows only)
@@ -461,7 +443,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
                                                             const float2* __restrict__ twM,
                                                             const float2* __restrict__ wr) {
   using C = XCfg<M>;
-  __shared__ float2 buf[C::RW * C::RS];
+  __shared__ float2 buf[C::BUF];
   __shared__ float2 twS[C::TWL ? M : 1];
   __shared__ float2 twrS[C::TWL ? M + 1 : 1];
   const float2* tw = C::TWL ? twS : twM;
@@ -495,17 +477,16 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
     rowOff[threadIdx.x] = so;
   }
   __syncthreads();
-  x2_stage_spectrum<M>(buf, S, rowOff);
-  __syncthreads();
   const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
   float2* rb = buf + r * C::RS;
+  const size_t so = rowOff[r];
   float2 z[C::A];
-  x2_inverse_half<M>(rb, tw, twr, q, z);
+  x2_inverse_half_regs<M>(so != kNoRow ? S + so : nullptr, rb, tw, twr, q, z);
   double sum = 0.0, mx = -1.0;
   const int DX = D[0];
-  if constexpr (x2_long_rows<M>()) {
-    // Long rows: thread q < B holds the values z[a] = (x[2n], x[2n + 1]), n = q + B a, of its row, and updates
-    // psi there directly (no park to LDS and flat pass)
+  if constexpr (C::long_rows) {
+    // long rows: lane q < B holds the values z[a] = (x[2n], x[2n + 1]), n = q + B a, of its row and updates psi
+    // there (at M = 576 faster than a flat pass)
     const long long pr = psiRow[r];
     if (q < C::B && pr >= 0) {
       const int m0 = d.m[0];
@@ -530,13 +511,15 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
           const int a = min(a0 + (j >> 1), C::A - 1);
           const float next = rl_next_value(last[j], (j & 1) ? z[a].y : z[a].x, wt[j], u.lambda, u.minValue, u.maxI);
           psiRowPtr[lx[j]] = next;
-          const float ch = __fsub_rn(next, last[j]);
+          const float ch = next - last[j];
           sum += (double)ch;
           mx = java_max(mx, (double)ch);
         }
       }
     }
   } else {
+    // short rows (only B of T lanes hold values): park the rows in LDS and update them with a flat mapping over
+    // every thread, six psi / weight pairs in flight (at M = 150 35 % faster than the register update)
     x2_park_inverse<M>(rb, q, z);
     const float* fb = reinterpret_cast<const float*>(buf);
     const unsigned total = (unsigned)(C::RW * DX);
@@ -551,9 +534,9 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
         const unsigned idx = base + j * C::NT;
         rr[j] = min(divD0.div(idx), (unsigned)C::RW - 1);
         const int lx = (int)(idx - rr[j] * (unsigned)DX);
-        const long long pr = psiRow[rr[j]];
-        ok[j] = idx < total && pr >= 0;
-        gi[j] = ok[j] ? (size_t)(pr + lx) : 0;
+        const long long prr = psiRow[rr[j]];
+        ok[j] = idx < total && prr >= 0;
+        gi[j] = ok[j] ? (size_t)(prr + lx) : 0;
         last[j] = u.psi[gi[j]];
         wt[j] = u.w[gi[j]];
       }
@@ -565,14 +548,14 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
         const float next = rl_next_value(last[j], fb[rr[j] * 2 * C::RS + d.m[0] + lx], wt[j], u.lambda, u.minValue,
                                          u.maxI);
         u.psi[gi[j]] = next;
-        const float ch = __fsub_rn(next, last[j]);
+        const float ch = next - last[j];
         sum += (double)ch;
         mx = java_max(mx, (double)ch);
       }
     }
   }
   // the stats reduction reuses the row buffer once every thread has read its rows
-  static_assert(C::RW * C::RS * sizeof(float2) >= 2 * reduce_width<C::NT>() * sizeof(double),
+  static_assert(C::BUF * sizeof(float2) >= 2 * reduce_width<C::NT>() * sizeof(double),
                 "the row buffer holds the reduction");
   __syncthreads();
   double* red = reinterpret_cast<double*>(buf);
@@ -581,16 +564,17 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
 
 // ------------------------------------------------------------------ inverse + store of the output region
 template <int M>
-__global__ __launch_bounds__(XCfg<M>::NT) void k_x2_store(Dom d, StoreParams st, FastDiv divD0,
-                                                           const float2* __restrict__ S,
+__global__ __launch_bounds__(XCfg<M>::NT) void k_x2_store(Dom d, StoreParams st, const float2* __restrict__ S,
                                                            const float2* __restrict__ twM,
                                                            const float2* __restrict__ wr) {
   using C = XCfg<M>;
-  __shared__ float2 buf[C::RW * C::RS];
-  __shared__ float2 tw[M];
-  __shared__ float2 twr[M + 1];
+  __shared__ float2 buf[C::BUF];
+  __shared__ float2 twS[C::TWL ? M : 1];
+  __shared__ float2 twrS[C::TWL ? M + 1 : 1];
+  const float2* tw = C::TWL ? twS : twM;
+  const float2* twr = C::TWL ? twrS : wr;
   __shared__ size_t rowOff[C::RW];
-  x2_load_twiddles<M>(tw, twr, twM, wr);
+  if (C::TWL) x2_load_twiddles<M>(twS, twrS, twM, wr);
   const int nrows = d.D[1] * d.D[2];
   const int row0 = blockIdx.x * C::RW;
   if (threadIdx.x < C::RW) {
@@ -603,21 +587,21 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_store(Dom d, StoreParams st,
     rowOff[threadIdx.x] = so;
   }
   __syncthreads();
-  x2_stage_spectrum<M>(buf, S, rowOff);
-  __syncthreads();
   const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
   float2* rb = buf + r * C::RS;
+  const size_t so = rowOff[r];
   float2 z[C::A];
-  x2_inverse_half<M>(rb, tw, twr, q, z);
-  x2_park_inverse<M>(rb, q, z);
-  const float* fb = reinterpret_cast<const float*>(buf);
-  const int DX = d.D[0];
-  const unsigned total = (unsigned)(C::RW * DX);
-  for (unsigned idx = threadIdx.x; idx < total; idx += C::NT) {
-    const unsigned rr = divD0.div(idx);
-    const int lx = (int)(idx - rr * (unsigned)DX);
-    const int row = row0 + (int)rr;
-    if (row < nrows) st.out[(size_t)row * DX + lx] = fb[rr * 2 * C::RS + d.m[0] + lx];
+  x2_inverse_half_regs<M>(so != kNoRow ? S + so : nullptr, rb, tw, twr, q, z);
+  const int row = row0 + r;
+  if (q < C::B && row < nrows) {
+    const int DX = d.D[0], m0 = d.m[0];
+    float* out = st.out + (size_t)row * DX;
+#pragma unroll
+    for (int a = 0; a < C::A; ++a) {
+      const int lx = 2 * (q + C::B * a) - m0;
+      if (lx >= 0 && lx < DX) out[lx] = z[a].x;
+      if (lx + 1 >= 0 && lx + 1 < DX) out[lx + 1] = z[a].y;
+    }
   }
 }
 
@@ -664,8 +648,7 @@ void launch_x_store(const Dom& d, const StoreParams& st, const float2* S, const 
   static_assert(x2_ok<M>(), "every compiled x length has a two-step split");
   using C = XCfg<M>;
   const int rows = d.D[1] * d.D[2];
-  hipLaunchKernelGGL(k_x2_store<M>, dim3((rows + C::RW - 1) / C::RW), dim3(C::NT), 0, s, d, st,
-                     FastDiv((unsigned)d.D[0]), S, twM, wr);
+  hipLaunchKernelGGL(k_x2_store<M>, dim3((rows + C::RW - 1) / C::RW), dim3(C::NT), 0, s, d, st, S, twM, wr);
 }
 
 }  // namespace mvr
