@@ -538,14 +538,19 @@ static StridedGeom timed(Timer& t, StridedGeom g, int c) {
 }
 
 // One view of one Seq iteration over the whole psi volume.
-int Decon::view_seq(int v, double2* statOut) {
+int Decon::view_seq(int v, double2* statOut, int slot) {
   Dom d = plan.dom;
   StridedGeom gy = plan.geom_y(), gz = plan.geom_z();
+  double2* part = slot >= 0 ? partial + (size_t)slot * partials_needed() : partial;
   if (blocked()) {
     // every block of this view in one launch per pass
     if (nbView[v] == 0) {  // no block with content: the view's statistics stay {0, -1}
-      launch_reduce_stats(partial, 0, statOut, stream);
-      MVR_HIP(hipGetLastError());
+      if (slot >= 0) {
+        slotCount[slot] = 0;
+      } else {
+        launch_reduce_stats(partial, 0, statOut, stream);
+        MVR_HIP(hipGetLastError());
+      }
       return 0;
     }
     d.btab = btab[v];
@@ -590,15 +595,26 @@ int Decon::view_seq(int v, double2* statOut) {
   u.lambda = (double)lambda;
   u.minValue = minValue;
   u.maxI = maxI[v];
-  u.partial = partial;
+  u.partial = part;
   timer.begin(stream, PASS_X_UPDATE);
   const int np = plan.xu(d, u, S, plan.twM, plan.wr, stream);
   timer.end(stream);
-  timer.begin(stream, PASS_REDUCE);
-  launch_reduce_stats(partial, np, statOut, stream);
-  timer.end(stream);
+  if (slot >= 0) {
+    slotCount[slot] = np;
+  } else {
+    timer.begin(stream, PASS_REDUCE);
+    launch_reduce_stats(partial, np, statOut, stream);
+    timer.end(stream);
+  }
   MVR_HIP(hipGetLastError());
   return 0;
+}
+
+// the statistics of the view updates in slots 0..n-1, in one launch
+void Decon::reduce_slots(int n, double2* out) {
+  timer.begin(stream, PASS_REDUCE);
+  launch_reduce_stats_batched(partial, partials_needed(), slotCount, n, out, stream);
+  timer.end(stream);
 }
 
 // One Mul iteration (ComputeBlockMulThreadCPU.runIteration semantics on the whole volume).
@@ -661,7 +677,8 @@ int Decon::run(int iterations) {
     MVR_HIP(hipMalloc(&statsDev, sizeof(double2) * std::max(1, need)));
     statsCap = std::max(1, need);
   }
-  if (ensure_partials((int)partials_needed())) return -1;
+  // one slot of partials per view of an iteration, reduced in one launch per iteration
+  if (ensure_partials((int)(partials_needed() * (size_t)(scheme == SCHEME_SEQ ? V : 1)))) return -1;
   if (useGraph && !timer.on && iterations > 0) {
     if (!graphStats) MVR_HIP(hipMalloc(&graphStats, sizeof(double2) * V));
     if (!graphExec) {
@@ -669,7 +686,8 @@ int Decon::run(int iterations) {
       MVR_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       int rc = 0;
       if (scheme == SCHEME_SEQ) {
-        for (int v = 0; v < V && !rc; ++v) rc = view_seq(v, graphStats + v);
+        for (int v = 0; v < V && !rc; ++v) rc = view_seq(v, nullptr, v);
+        if (!rc) reduce_slots(V, graphStats);
       } else {
         rc = iteration_mul(graphStats);
       }
@@ -697,7 +715,9 @@ int Decon::run(int iterations) {
     for (int it = 0; it < iterations; ++it) {
       if (scheme == SCHEME_SEQ) {
         for (int v = 0; v < V; ++v)
-          if (view_seq(v, statsDev + (size_t)it * V + v)) return -1;
+          if (view_seq(v, nullptr, v)) return -1;
+        reduce_slots(V, statsDev + (size_t)it * V);
+        MVR_HIP(hipGetLastError());
       } else {
         if (iteration_mul(statsDev + it)) return -1;
       }

@@ -113,6 +113,9 @@ const int* half_lengths();
 // misc kernels (misc.hip)
 void launch_fill(float* dst, size_t n, float v, hipStream_t s);
 void launch_reduce_stats(const double2* partial, int n, double2* out, hipStream_t s);
+// nUpdates <= 32 reductions in one launch: update v's partials at partial + v * stride, counts[v] of them
+void launch_reduce_stats_batched(const double2* partial, size_t stride, const int* counts, int nUpdates, double2* out,
+                                 hipStream_t s);
 int launch_psi_init_partials(const float* const* imgs, int V, size_t n, double* psum, unsigned long long* pcount,
                              float* pmax, hipStream_t s);  // returns #partials
 void launch_update_mul(float* psi, const float* const* T2, const float* const* w, int V, const int N[3],

@@ -171,7 +171,11 @@ struct Decon {
   int ensure_partials(int n);
   int build_plan_and_spectra(const int* Pmin, int nviews);
   void drop_graph();
-  int view_seq(int v, double2* statOut);
+  // one Seq view update; slot < 0: its statistics reduced into statOut now; slot >= 0: its partials left at
+  // partial + slot * partials_needed() (count in slotCount[slot]) for reduce_slots
+  int view_seq(int v, double2* statOut, int slot = -1);
+  int slotCount[32] = {0};
+  void reduce_slots(int n, double2* out);
   int iteration_mul(double2* statOut);
 };
 

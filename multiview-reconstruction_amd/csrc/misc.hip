@@ -35,6 +35,36 @@ void launch_reduce_stats(const double2* partial, int n, double2* out, hipStream_
   hipLaunchKernelGGL(k_reduce_stats, dim3(1), dim3(kThreads), 0, s, partial, n, out);
 }
 
+// The same reduction for up to 32 view updates at once (work-group v: partials [v * stride, v * stride + n[v]),
+// result out[v]): one launch per iteration instead of one per view update.
+struct PartialCounts { int n[32]; };
+__global__ void k_reduce_stats_batched(const double2* __restrict__ partial, size_t stride, PartialCounts c,
+                                       double2* __restrict__ out) {
+  __shared__ double ss[kThreads], sm[kThreads];
+  const double2* p = partial + (size_t)blockIdx.x * stride;
+  const int n = c.n[blockIdx.x];
+  double s = 0.0, m = -1.0;
+  for (int i = threadIdx.x; i < n; i += kThreads) { s += p[i].x; m = java_max(m, p[i].y); }
+  ss[threadIdx.x] = s;
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = kThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      ss[threadIdx.x] += ss[threadIdx.x + off];
+      sm[threadIdx.x] = java_max(sm[threadIdx.x], sm[threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = make_double2(ss[0], sm[0]);
+}
+
+void launch_reduce_stats_batched(const double2* partial, size_t stride, const int* counts, int nUpdates, double2* out,
+                                 hipStream_t s) {
+  PartialCounts c{};
+  for (int v = 0; v < nUpdates && v < 32; ++v) c.n[v] = counts[v];
+  hipLaunchKernelGGL(k_reduce_stats_batched, dim3(nUpdates), dim3(kThreads), 0, s, partial, stride, c, out);
+}
+
 // PsiInitAvgPrecise per voxel (D/init/PsiInitAvgPreciseThread.java:127-155): the
 // mean of the positive view values is summed (double) over voxels with any
 // positive value; max of the positive values per view.  Per-work-group

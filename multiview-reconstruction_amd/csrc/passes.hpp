@@ -103,16 +103,21 @@ __device__ __forceinline__ void block_reduce_stats(double s, double mx, double2*
 // in the opposite order, so in the convolution pass the spectrum product and
 // the first inverse step stay in the registers of the forward's second step:
 // one LDS round trip per transform.
+template <int L, int W>
+__host__ __device__ constexpr bool s2_fits();
 template <int L>
 __host__ __device__ constexpr int s2_width() {
-  // 16 columns = 128-byte runs (32 was slower at L = 300, 8 slower at L = 400); 8 above L = 640 (LDS)
-  return L <= 640 ? 16 : 8;
+  // 16 columns = 128-byte runs (32 was slower at L = 300, 8 slower at L = 400); 8 for 720..1024 (two work-groups
+  // per CU); 16 again for 1080 and 1152, whose 16-column tile fills the LDS of one CU and still runs 12 % faster
+  // than two 8-column work-groups (y pass of 1152-long pencils: 3.24 -> 2.86 ms per 4-iteration run)
+  return L <= 640 ? 16 : (L > 1024 && s2_fits<L, 16>()) ? 16 : 8;
 }
 constexpr bool kStridedXcdOrder = true;
-// tile widths compiled per length: 8 and 16 columns where the tile fits 96 KB of LDS
+// tile widths compiled per length: 8 and 16 columns where the tile fits 96 KB of LDS; above L = 1024 also
+// 16-column tiles of up to 152 KB (one work-group per CU, twiddles in LDS beside the tile)
 template <int L, int W>
 __host__ __device__ constexpr bool s2_fits() {
-  return split2(L).A > 0 && L * W * 8 <= 96 * 1024;
+  return split2(L).A > 0 && L * W * 8 <= (L > 1024 ? 152 : 96) * 1024;
 }
 
 // The twiddles stay in global memory (read through L1) when that lets a second work-group share the CU's
