@@ -30,12 +30,17 @@ METRIC = "RL voxel-iters/s; wall time to rel-L2≤1e-2, learned-init vs plain, 1
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic (unique) HBM bytes of one launch of each pass.  The complex half spectrum counts 4 B per voxel of
-# the FFT extent (N + K - 1 per axis; padding to the FFT length is never credited), the real volumes (psi, the
-# view image, the weights) 4 B per volume voxel (N per axis), each read or written once:
+# the FFT extent E (N + K - 1 per axis; padding to the FFT length is never credited), the real volumes (psi, the
+# view image, the weights) 4 B per volume voxel (N per axis), each read or written once.  On one frame a pass
+# moves only the rows its neighbours use: a convolution's inverse passes produce the rows the next x pass reads
+# (the output region's y and z; the quotient region is the output region here), and the quotient pass writes
+# only the quotient rows, the rows of ones around them being put in by the z pass that reads them:
 #   x_fwd   : psi 4 N + spectrum write 4 E
-#   z_fwd/z_inv: spectrum read 4 E + write 4 E
-#   y_conv  : spectrum read 4 E + kernel spectrum read 4 E + write 4 E
-#   x_ratio : spectrum read 4 E + view image 4 N + spectrum write 4 E
+#   z_fwd   : after x_fwd spectrum read 4 E + write 4 E; after x_ratio read of the quotient rows 4 Ex Ny Nz +
+#             write 4 E (the two launches of a view update averaged)
+#   y_conv  : spectrum read 4 E + kernel spectrum read 4 E + write of the output region's y, 4 Ex Ny Ez
+#   z_inv   : read of the output region's y, 4 Ex Ny Ez + write of its y and z, 4 Ex Ny Nz
+#   x_ratio : spectrum rows of the output region 4 Ex Ny Nz + view image 4 N + write of those rows 4 Ex Ny Nz
 #   x_update: spectrum rows of the output region 4 Ex Ny Nz + psi read 4 N + weights 4 N + psi write 4 N
 #   x_store : spectrum rows of the output region 4 Ex Ny Nz + store 4 N
 # The whole view update is also priced on the survey's budget of 60 B per extent voxel (five passes with the
@@ -45,13 +50,14 @@ RUN_BYTES_PER_EXTENT_VOXEL = 60
 
 def pass_bytes(name, n_zyx, k_zyx):
     nz, ny, nx = n_zyx
-    ext = 1
-    for n, k in zip(n_zyx, k_zyx):
-        ext *= n + k - 1
+    ez, ey, ex = (n + k - 1 for n, k in zip(n_zyx, k_zyx))
+    ext = ex * ey * ez
     vol = nz * ny * nx
-    rows = (nx + k_zyx[2] - 1) * ny * nz
-    return {"x_fwd": 4 * vol + 4 * ext, "z_fwd": 8 * ext, "y_conv": 12 * ext, "z_inv": 8 * ext,
-            "x_ratio": 8 * ext + 4 * vol, "x_update": 4 * rows + 12 * vol, "x_store": 4 * rows + 4 * vol}[name]
+    rows = ex * ny * nz     # spectrum rows of the output region
+    ycols = ex * ny * ez    # the output region's y on every z plane
+    return {"x_fwd": 4 * vol + 4 * ext, "z_fwd": (8 * ext + 4 * rows + 4 * ext) // 2, "y_conv": 8 * ext + 4 * ycols,
+            "z_inv": 4 * ycols + 4 * rows, "x_ratio": 8 * rows + 4 * vol,
+            "x_update": 4 * rows + 12 * vol, "x_store": 4 * rows + 4 * vol}[name]
 
 
 def log(*a):
@@ -73,8 +79,12 @@ def main():
     ap.add_argument("--iterations", type=int, default=0,
                     help="override the workload's iteration count (short runs under a profiler)")
     ap.add_argument("--no-learned", action="store_true", help="skip the time-to-convergence comparison")
+    ap.add_argument("--weights", default=None, help="weights file of the learned start (default: the workload's)")
     ap.add_argument("--tol", type=float, default=1e-2)
-    ap.add_argument("--reference-iterations", type=int, default=200)
+    ap.add_argument("--reference-iterations", type=int, default=0,
+                    help="reference of the time-to-convergence comparison: plain RL run this many iterations; 0 "
+                         "(default): converged plain RL, iterated until the rel-L2 change over 100 iterations is "
+                         "at most 1e-3 (the 256^3 workload); the 1024^2 x 512 workload defaults to 200")
     ap.add_argument("--block-path", type=int, default=0,
                     help="also time this many iterations through the per-block drop-in ABI (host buffers)")
     ap.add_argument("--block-size", type=int, nargs=3, default=None, metavar=("BX", "BY", "BZ"),
@@ -192,7 +202,8 @@ def main():
 
     learned = None
     if not args.no_learned and args.workload in ("c2", "c5"):
-        learned = time_to_converge(d, w, args.tol, args.reference_iterations, args.workload)
+        ref_its = args.reference_iterations or (200 if args.workload == "c5" else 0)
+        learned = time_to_converge(d, w, args.tol, ref_its, args.workload, args.weights)
     block_path = None
     if args.block_path > 0 and imgs is not None:
         block_path = time_block_path(imgs, ws, ks, w, args.block_path)
@@ -386,26 +397,31 @@ def cpu_baseline_block(c, w, V):
                       f"structure, oracle restatement in FP32 on {threads} OpenMP threads; {secs:.2f}s"}, out
 
 
-def unet_flops(shape_zyx):
-    """Multiply-adds x 2 of one U-Net inference (3^3 layers at their level's voxel count, padded channels)."""
+def unet_flops(shape_zyx, n_views):
+    """Multiply-adds x 2 of one U-Net inference: the 3^3 layers at their level's voxel count, the first layer
+    counted on its real input channels (the views and their weighted mean; the kernel pads them to 16, and
+    the padding is not credited), plus the 1x1 head."""
     from mvr import hip
     nz, ny, nx = ((s + 3) // 4 * 4 for s in shape_zyx)
     v0 = nz * ny * nx
     level = (0, 0, 1, 1, 2, 2, 1, 1, 0, 0)
+    cin = (n_views + 1,) + tuple(hip.UNet.LAYERS_CIN[1:])
     f = 0
     for l in range(10):
-        f += 2 * 27 * hip.UNet.LAYERS_CIN[l] * hip.UNet.LAYERS_COUT[l] * v0 / 8 ** level[l]
+        f += 2 * 27 * cin[l] * hip.UNet.LAYERS_COUT[l] * v0 / 8 ** level[l]
     return f + 2 * 16 * v0
 
 
-def _trajectory(d, start_seconds, max_iterations):
-    """(seconds since the start, rel-L2) after each of up to max_iterations iterations; the distance is
-    reduced on the GPU after every iteration, its cost is inside the times."""
+def _trajectory(d, start_seconds, max_iterations, stop_tol=0.0):
+    """(seconds since the start, rel-L2) after each of up to max_iterations iterations, stopping once rel-L2 <=
+    stop_tol; the distance is reduced on the GPU after every iteration, its cost is inside the times."""
     out = []
     t0 = time.perf_counter() - start_seconds
     for _ in range(max_iterations):
         d.run(1)
         out.append((time.perf_counter() - t0, d.rel_l2()))
+        if out[-1][1] <= stop_tol:
+            break
     return out
 
 
@@ -418,25 +434,60 @@ def _first_below(traj, start, tol):
     return None, None
 
 
-TOLERANCES = (1e-2, 0.05, 0.1, 0.15, 0.2, 0.3)
+TOLERANCES = (1e-2, 0.02, 0.05, 0.1, 0.15, 0.2, 0.3)
+CONVERGED_CHANGE = 1e-3     # rel-L2 change over CONVERGED_BLOCK iterations that ends the reference run
+CONVERGED_BLOCK = 100
+CONVERGED_CAP = 20000
 
 
-def time_to_converge(d, w, tol, ref_iterations, workload="c2"):
-    """Wall time to rel-L2 <= tol to plain RL run to ref_iterations: plain PsiInitAvgPrecise start vs the
-    learned start (U-Net inference included).  The common set-up (PSFs, spectra, AvgPrecise pass) is
-    outside both timed regions.  Also reported for the looser tolerances of TOLERANCES."""
-    from mvr import learned as L
-    weights = L.TRAINING[workload][1]
+def converged_reference(d):
+    """Plain RL from the handle's initial estimate until the rel-L2 change over CONVERGED_BLOCK iterations is at
+    most CONVERGED_CHANGE (at most CONVERGED_CAP iterations): (psi, iterations, last change)."""
     d.reset_psi()
-    d.run(ref_iterations)
-    d.set_reference(d.psi())
-    log(f"reference: {ref_iterations} plain iterations")
+    prev = d.psi()
+    it, ch = 0, float("inf")
+    while it < CONVERGED_CAP:
+        d.set_reference(prev)
+        d.run(CONVERGED_BLOCK)
+        it += CONVERGED_BLOCK
+        ch = d.rel_l2()
+        prev = d.psi()
+        if ch <= CONVERGED_CHANGE:
+            break
+    return prev, it, ch
+
+
+def time_to_converge(d, w, tol, ref_iterations, workload="c2", weights=None):
+    """Wall time to rel-L2 <= tol to a reference: plain PsiInitAvgPrecise start vs PsiInitBlurredFused vs the
+    learned start (U-Net inference included).  The reference is converged plain RL (ref_iterations 0: until the
+    rel-L2 change over 100 iterations is <= 1e-3) or plain RL run to ref_iterations.  The common set-up (PSFs,
+    spectra, AvgPrecise pass) is outside the timed regions.  Also reported for the looser tolerances of
+    TOLERANCES.  Every trajectory runs until it reaches tol or a cap: the converged reference's iteration count
+    + 3000, or a fixed-length reference's own iteration count (plain RL meets that reference there)."""
+    from mvr import learned as L
+    if weights is None:
+        # against a converged reference, the network trained on deep targets when there is one
+        deep = L.DEEP_WEIGHTS.get(workload) if ref_iterations == 0 else None
+        weights = deep if deep and os.path.exists(deep) else L.TRAINING[workload][1]
+    t0 = time.perf_counter()
+    if ref_iterations > 0:
+        d.reset_psi()
+        d.run(ref_iterations)
+        ref, ref_its, ref_change = d.psi(), ref_iterations, None
+        desc = f"plain RL, {ref_iterations} iterations from PsiInitAvgPrecise"
+    else:
+        ref, ref_its, ref_change = converged_reference(d)
+        desc = (f"converged plain RL from PsiInitAvgPrecise: {ref_its} iterations, until the rel-L2 change over "
+                f"{CONVERGED_BLOCK} iterations was <= {CONVERGED_CHANGE} (last {ref_change:.2e})")
+    d.set_reference(ref)
+    log(f"reference: {desc} ({time.perf_counter() - t0:.1f}s)")
+    cap = ref_its if ref_iterations > 0 else ref_its + 3000
     d.reset_psi()
     plain0 = (0.0, d.rel_l2())
-    plain = _trajectory(d, 0.0, ref_iterations)
-    log("plain trajectory done")
-    res = {"tol": tol, "reference": f"plain RL, {ref_iterations} iterations from PsiInitAvgPrecise",
-           "plain": {"rel_l2_start": plain0[1], "rel_l2_after_20": plain[19][1]}}
+    plain = _trajectory(d, 0.0, cap, tol)
+    log(f"plain trajectory done: {len(plain)} iterations")
+    res = {"tol": tol, "reference": desc, "reference_iterations": ref_its, "reference_last_change": ref_change,
+           "plain": {"rel_l2_start": plain0[1], "rel_l2_after_20": plain[min(19, len(plain) - 1)][1]}}
     # PsiInitBlurredFused, the reference GUI's default initialiser: its fusion + blur time is the init
     # difference to AvgPrecise (both inits also rebuild PSFs and spectra)
     t0 = time.perf_counter()
@@ -446,10 +497,10 @@ def time_to_converge(d, w, tol, ref_iterations, workload="c2"):
     d.init(mode="BLURRED_FUSED")
     t_blur = max(0.0, time.perf_counter() - t0 - t_avg)
     blurred0 = (t_blur, d.rel_l2())
-    blurred = _trajectory(d, t_blur, ref_iterations)
-    log("BlurredFused trajectory done")
+    blurred = _trajectory(d, t_blur, cap, tol)
+    log(f"BlurredFused trajectory done: {len(blurred)} iterations")
     res["blurred_fused"] = {"init_seconds_over_avg_precise": t_blur, "rel_l2_start": blurred0[1],
-                            "rel_l2_after_20": blurred[19][1]}
+                            "rel_l2_after_20": blurred[min(19, len(blurred) - 1)][1]}
     if not os.path.exists(weights):
         res["learned"] = None
         res["note"] = "no trained weights file"
@@ -464,13 +515,15 @@ def time_to_converge(d, w, tol, ref_iterations, workload="c2"):
     net.close()
     inf = min(t_inf)
     learned0 = (inf, d.rel_l2())
-    learned = _trajectory(d, inf, ref_iterations)
-    log("learned trajectory done")
+    learned = _trajectory(d, inf, cap, tol)
+    log(f"learned trajectory done: {len(learned)} iterations")
     best = min(range(len(learned)), key=lambda i: learned[i][1])
-    fl = unet_flops(w.shape_zyx)
+    fl = unet_flops(w.shape_zyx, w.n_views)
     res["learned"] = {"inference_seconds": inf, "rel_l2_start": learned0[1], "rel_l2_min": learned[best][1],
                       "rel_l2_min_at_iteration": best + 1, "inference_tflops": fl / inf / 1e12,
-                      "inference_mfma_frac_bf16_dense": fl / inf / 1e12 / 2500.0}
+                      "inference_mfma_frac_bf16_dense": fl / inf / 1e12 / 2500.0,
+                      "weights_meta": {k: v for k, v in L.meta(weights).items()
+                                       if k in ("target_iterations", "seeds", "patches", "epochs")}}
     res["weights"] = os.path.relpath(weights, REPO)
     table = []
     for t in sorted(set((tol,) + TOLERANCES)):
@@ -485,7 +538,9 @@ def time_to_converge(d, w, tol, ref_iterations, workload="c2"):
     head = next(r for r in table if r["tol"] == tol)
     res["plain"].update(iterations=head["plain_iterations"], seconds=head["plain_seconds"])
     res["learned"].update(iterations=head["learned_iterations"], seconds=head["learned_seconds"])
+    res["blurred_fused"].update(iterations=head["blurred_fused_iterations"], seconds=head["blurred_fused_seconds"])
     res["speedup"] = head["speedup"]
+    res["speedup_vs_blurred_fused"] = head["speedup_vs_blurred_fused"]
     return res
 
 

@@ -566,12 +566,6 @@ int Decon::view_seq(int v, double2* statOut, int slot) {
   g.src = psi;
   for (int k = 0; k < 3; ++k) { g.N[k] = N[k]; g.org[k] = -d.m[k]; }
   g.mode = GATHER_MIRROR;
-  timer.begin(stream, PASS_X_FWD);
-  plan.xf(d, g, S, plan.twM, plan.wr, stream);
-  timer.end(stream);
-  plan.sz(timed(timer, gz, PASS_Z_FWD), 0, S, S, nullptr, 1.f, plan.twZ, stream);
-  plan.sy(timed(timer, gy, PASS_Y_CONV), 2, S, S, spec1[v], 1.f, plan.twY, stream);
-  plan.sz(timed(timer, gz, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
   RatioParams q{};
   q.img = img[v];
   for (int k = 0; k < 3; ++k) {
@@ -582,12 +576,25 @@ int Decon::view_seq(int v, double2* statOut, int slot) {
       q.qhi[k] = K[k] / 2;
     }
   }
+  // the first convolution is read on the quotient rows, the second on the output rows; on one frame the
+  // quotient pass leaves the rows of ones to the z pass after it
+  const int zero[3] = {0, 0, 0};
+  StridedGeom gy1 = gy, gz1 = gz, gy2 = gy, gz2 = gz, gzq = gz;
+  plan.restrict_rows(q.qlo, q.qhi, gy1, gz1);
+  plan.restrict_rows(zero, zero, gy2, gz2);
+  if (!blocked()) plan.skip_ones_rows(q, gzq);
+  timer.begin(stream, PASS_X_FWD);
+  plan.xf(d, g, S, plan.twM, plan.wr, stream);
+  timer.end(stream);
+  plan.sz(timed(timer, gz, PASS_Z_FWD), 0, S, S, nullptr, 1.f, plan.twZ, stream);
+  plan.sy(timed(timer, gy1, PASS_Y_CONV), 2, S, S, spec1[v], 1.f, plan.twY, stream);
+  plan.sz(timed(timer, gz1, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
   timer.begin(stream, PASS_X_RATIO);
   plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
   timer.end(stream);
-  plan.sz(timed(timer, gz, PASS_Z_FWD), 0, S, S, nullptr, 1.f, plan.twZ, stream);
-  plan.sy(timed(timer, gy, PASS_Y_CONV), 2, S, S, spec2[v], 1.f, plan.twY, stream);
-  plan.sz(timed(timer, gz, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
+  plan.sz(timed(timer, gzq, PASS_Z_FWD), 0, S, S, nullptr, 1.f, plan.twZ, stream);
+  plan.sy(timed(timer, gy2, PASS_Y_CONV), 2, S, S, spec2[v], 1.f, plan.twY, stream);
+  plan.sz(timed(timer, gz2, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
   UpdateParams u{};
   u.psi = psi;
   u.w = w[v];
@@ -621,6 +628,10 @@ void Decon::reduce_slots(int n, double2* out) {
 int Decon::iteration_mul(double2* statOut) {
   const Dom& d = plan.dom;
   const StridedGeom gy = plan.geom_y(), gz = plan.geom_z();
+  // both convolutions are read on the output rows (the quotient and the stored result)
+  const int zero[3] = {0, 0, 0};
+  StridedGeom gyo = gy, gzo = gz;
+  plan.restrict_rows(zero, zero, gyo, gzo);
   GatherParams g{};
   g.src = psi;
   for (int k = 0; k < 3; ++k) { g.N[k] = N[k]; g.org[k] = -d.m[k]; }
@@ -632,17 +643,19 @@ int Decon::iteration_mul(double2* statOut) {
   plan.sz(timed(timer, gz, PASS_Z_FWD), 0, Spsi, Spsi, nullptr, 1.f, plan.twZ, stream);
   std::vector<const float*> t2p(V), wp(V);
   for (int v = 0; v < V; ++v) {
-    plan.sy(timed(timer, gy, PASS_Y_CONV), 2, Spsi, S, spec1[v], 1.f, plan.twY, stream);
-    plan.sz(timed(timer, gz, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
+    plan.sy(timed(timer, gyo, PASS_Y_CONV), 2, Spsi, S, spec1[v], 1.f, plan.twY, stream);
+    plan.sz(timed(timer, gzo, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
     RatioParams q{};
     q.img = img[v];
     for (int k = 0; k < 3; ++k) { q.N[k] = N[k]; q.o[k] = 0; }
+    StridedGeom gzq = gz;
+    plan.skip_ones_rows(q, gzq);
     timer.begin(stream, PASS_X_RATIO);
     plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
     timer.end(stream);
-    plan.sz(timed(timer, gz, PASS_Z_FWD), 0, S, S, nullptr, 1.f, plan.twZ, stream);
-    plan.sy(timed(timer, gy, PASS_Y_CONV), 2, S, S, spec2[v], 1.f, plan.twY, stream);
-    plan.sz(timed(timer, gz, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
+    plan.sz(timed(timer, gzq, PASS_Z_FWD), 0, S, S, nullptr, 1.f, plan.twZ, stream);
+    plan.sy(timed(timer, gyo, PASS_Y_CONV), 2, S, S, spec2[v], 1.f, plan.twY, stream);
+    plan.sz(timed(timer, gzo, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
     StoreParams st{T2 + nvox() * v};
     timer.begin(stream, PASS_X_STORE);
     plan.xs(d, st, S, plan.twM, plan.wr, stream);
@@ -909,6 +922,10 @@ int BlockThread::set_view_data(int v, const float* im, const float* wt, const in
 int BlockThread::iterate_seq(int v, float maxI) {
   const Dom& d = plan.dom;
   const StridedGeom gy = plan.geom_y(), gz = plan.geom_z();
+  // both convolutions are read on the block's rows (the quotient over the whole block, the update)
+  const int zero[3] = {0, 0, 0};
+  StridedGeom gyo = gy, gzo = gz;
+  plan.restrict_rows(zero, zero, gyo, gzo);
   // convolve1: psi block with mirror-single extension of the block itself (ComputeBlockSeqThreadCPU.java:171-189)
   GatherParams g{};
   g.src = psiB;
@@ -916,15 +933,17 @@ int BlockThread::iterate_seq(int v, float maxI) {
   g.mode = GATHER_MIRROR;
   plan.xf(d, g, S, plan.twM, plan.wr, stream);
   plan.sz(gz, 0, S, S, nullptr, 1.f, plan.twZ, stream);
-  plan.sy(gy, 2, S, S, spec1[v], 1.f, plan.twY, stream);
-  plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
+  plan.sy(gyo, 2, S, S, spec1[v], 1.f, plan.twY, stream);
+  plan.sz(gzo, 1, S, S, nullptr, 1.f, plan.twZ, stream);
   RatioParams q{};
   q.img = imgB;
   for (int k = 0; k < 3; ++k) { q.N[k] = B[k]; q.o[k] = 0; }
+  StridedGeom gzq = gz;
+  plan.skip_ones_rows(q, gzq);
   plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
-  plan.sz(gz, 0, S, S, nullptr, 1.f, plan.twZ, stream);
-  plan.sy(gy, 2, S, S, spec2[v], 1.f, plan.twY, stream);
-  plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
+  plan.sz(gzq, 0, S, S, nullptr, 1.f, plan.twZ, stream);
+  plan.sy(gyo, 2, S, S, spec2[v], 1.f, plan.twY, stream);
+  plan.sz(gzo, 1, S, S, nullptr, 1.f, plan.twZ, stream);
   if (ensure_partials((int)(plan.rows_inner() / 4 + 1))) return -1;
   UpdateParams u{};
   u.psi = psiB;
@@ -983,6 +1002,9 @@ int BlockThread::iterate_mul(int nv, const float* maxIs) {
   const size_t nb = (size_t)B[0] * B[1] * B[2];
   const Dom& d = plan.dom;
   const StridedGeom gy = plan.geom_y(), gz = plan.geom_z();
+  const int zero[3] = {0, 0, 0};
+  StridedGeom gyo = gy, gzo = gz;
+  plan.restrict_rows(zero, zero, gyo, gzo);
   GatherParams g{};
   g.src = psiB;
   for (int k = 0; k < 3; ++k) { g.N[k] = B[k]; g.org[k] = -d.m[k]; }
@@ -991,15 +1013,17 @@ int BlockThread::iterate_mul(int nv, const float* maxIs) {
   plan.sz(gz, 0, Spsi, Spsi, nullptr, 1.f, plan.twZ, stream);
   std::vector<const float*> t2p(nv), wp(nv);
   for (int v = 0; v < nv; ++v) {
-    plan.sy(gy, 2, Spsi, S, spec1[v], 1.f, plan.twY, stream);
-    plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
+    plan.sy(gyo, 2, Spsi, S, spec1[v], 1.f, plan.twY, stream);
+    plan.sz(gzo, 1, S, S, nullptr, 1.f, plan.twZ, stream);
     RatioParams q{};
     q.img = imgV[v];
     for (int k = 0; k < 3; ++k) { q.N[k] = B[k]; q.o[k] = 0; }
+    StridedGeom gzq = gz;
+    plan.skip_ones_rows(q, gzq);
     plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
-    plan.sz(gz, 0, S, S, nullptr, 1.f, plan.twZ, stream);
-    plan.sy(gy, 2, S, S, spec2[v], 1.f, plan.twY, stream);
-    plan.sz(gz, 1, S, S, nullptr, 1.f, plan.twZ, stream);
+    plan.sz(gzq, 0, S, S, nullptr, 1.f, plan.twZ, stream);
+    plan.sy(gyo, 2, S, S, spec2[v], 1.f, plan.twY, stream);
+    plan.sz(gzo, 1, S, S, nullptr, 1.f, plan.twZ, stream);
     StoreParams st{T2 + nb * v};
     plan.xs(d, st, S, plan.twM, plan.wr, stream);
     t2p[v] = T2 + nb * v;

@@ -51,12 +51,15 @@ struct GatherParams {
 // region's first voxel; N are the dims of the volume that img/psi/w index.
 // The quotient is formed on the output region widened by qlo / qhi voxels per side (the reach of the second
 // kernel, for domains whose neighbourhood lies inside the volume: blocks), within the volume; 1 elsewhere.
+// skipOnes: rows outside the quotient region (a row of ones, whose x spectrum is P0 at frequency 0 and 0
+// elsewhere) are not written; the z pass that reads the result puts them in itself (StridedGeom ones*).
 struct RatioParams {
   const float* img;
   int N[3];
   int o[3];
   int qlo[3];
   int qhi[3];
+  int skipOnes;
 };
 
 struct UpdateParams {
@@ -93,6 +96,15 @@ struct StridedGeom {
   // b * blockStride; the kernel spectrum is shared (indexed by the plane within the block).  0 = one block.
   int blockPlanes = 0;
   size_t blockStride = 0;
+  // Only what the next pass reads: planes [plane0, plane0 + nplanes) (per block) are transformed, and the inverse
+  // modes (1, 2) store only the pencil positions [storeLo, storeHi).  The defaults cover the whole grid.
+  int plane0 = 0;
+  int storeLo = 0, storeHi = 1 << 30;
+  // Forward mode over data whose rows outside planes [onesPlaneLo, onesPlaneHi) x positions [onesLo, onesHi) are
+  // x spectra of rows of ones, never written: the pass reads none of them and uses onesDC at column 0, 0 at the
+  // other columns.  Off when onesDC is 0.
+  float onesDC = 0.f;
+  int onesPlaneLo = 0, onesPlaneHi = 0, onesLo = 0, onesHi = 0;
   // timing: when set, the launch records the kernel's own start and stop in these events
   hipEvent_t evStart = nullptr, evStop = nullptr;
 };

@@ -48,6 +48,14 @@ struct Plan {
   size_t rows_inner() const { return (size_t)dom.D[1] * dom.D[2]; }
   StridedGeom geom_y() const;
   StridedGeom geom_z() const;
+  // Narrows the geometries gy, gz of a convolution's y_conv and z_inv passes to what the next x pass reads: the
+  // grid rows (y, z) in [m - lo, m + D + hi) per axis (within the grid).  y_conv then stores only those y, and
+  // z_inv transforms only those y planes and stores only those z.  Batched geometries stay batched.
+  void restrict_rows(const int lo[3], const int hi[3], StridedGeom& gy, StridedGeom& gz) const;
+  // Sets q.skipOnes and the z geometry gz of the forward pass that reads the quotient pass's result: the rows
+  // outside the quotient region are rows of ones (not written by the quotient pass, put in by the z pass).
+  // Not for batched blocks, whose quotient regions differ per block.
+  void skip_ones_rows(RatioParams& q, StridedGeom& gz) const;
   float inv_scale() const { return 2.0f / ((float)dom.P[0] * (float)dom.P[1] * (float)dom.P[2]); }
 
   // spectrum of a device kernel (K layout), scaled by the inverse-FFT factor

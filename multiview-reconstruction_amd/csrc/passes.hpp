@@ -2,8 +2,8 @@
 //
 // One view of one iteration (Seq scheme, ComputeBlockSeqThreadCPU.runIteration,
 // D/iteration/sequential/ComputeBlockSeqThreadCPU.java:79-169) is nine launches:
-//   x_fwd(gather psi, mirror)  y_fwd  z(*K1^)  y_inv  x_ratio(C2R, img/blurred, R2C)
-//   y_fwd  z(*K2^)  y_inv  x_update(C2R, Tikhonov update of psi, stats)
+//   x_fwd(gather psi, mirror)  z_fwd  y(*K1^)  z_inv  x_ratio(C2R, img/blurred, R2C)
+//   z_fwd  y(*K2^)  z_inv  x_update(C2R, Tikhonov update of psi, stats)
 // Every pass reads and writes its data once; the pointwise work of the
 // reference (quotient, update, spectrum multiply) is fused into the passes.
 //
@@ -160,6 +160,7 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W>())) void k_str
     plane -= b * gm.blockPlanes;
     blockOff = (size_t)b * gm.blockStride;
   }
+  plane += gm.plane0;
   const float2* src = in + blockOff + (size_t)plane * gm.planeStride + col0 + w;
   float2* dst = out + blockOff + (size_t)plane * gm.planeStride + col0 + w;
   const size_t ls = gm.lineStride;
@@ -167,8 +168,20 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W>())) void k_str
   if constexpr (MODE != 1) {
     // ---- forward step 1: thread q = b < B
     if (q < B) {
+      if (gm.onesDC != 0.f) {
+        // rows of ones outside the written region: their x spectrum is known and not read
+        const bool planeIn = plane >= gm.onesPlaneLo && plane < gm.onesPlaneHi;
+        const float2 one = make_float2(col0 + w == 0 ? gm.onesDC : 0.f, 0.f);
 #pragma unroll
-      for (int a = 0; a < A; ++a) v[a] = colOk ? src[(size_t)(q + B * a) * ls] : make_float2(0.f, 0.f);
+        for (int a = 0; a < A; ++a) {
+          const int n = q + B * a;
+          const bool rd = colOk && planeIn && n >= gm.onesLo && n < gm.onesHi;
+          v[a] = rd ? src[(size_t)n * ls] : (colOk ? one : make_float2(0.f, 0.f));
+        }
+      } else {
+#pragma unroll
+        for (int a = 0; a < A; ++a) v[a] = colOk ? src[(size_t)(q + B * a) * ls] : make_float2(0.f, 0.f);
+      }
     }
     __syncthreads();  // twiddle table
     if (q < B) {
@@ -236,8 +249,12 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W>())) void k_str
       for (int k1 = 0; k1 < A; ++k1) v[k1] = tile[(k1 + A * q) * W + w];
       rdft<A, 1>(v);
       if (colOk) {
+        // positions the next pass reads (storeLo <= q + B n1 < storeHi)
 #pragma unroll
-        for (int n1 = 0; n1 < A; ++n1) dst[(size_t)(q + B * n1) * ls] = v[n1];
+        for (int n1 = 0; n1 < A; ++n1) {
+          const int n = q + B * n1;
+          if (n >= gm.storeLo && n < gm.storeHi) dst[(size_t)n * ls] = v[n1];
+        }
       }
     }
   }

@@ -1,4 +1,5 @@
 // plan.cpp -- FFT-convolution plans, error state and pass timers.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -92,6 +93,37 @@ StridedGeom Plan::geom_z() const {
   return g;
 }
 
+void Plan::restrict_rows(const int lo[3], const int hi[3], StridedGeom& gy, StridedGeom& gz) const {
+  int a[3], b[3];
+  for (int k = 1; k < 3; ++k) {
+    a[k] = std::max(0, dom.m[k] - lo[k]);
+    b[k] = std::min(dom.P[k], dom.m[k] + dom.D[k] + hi[k]);
+  }
+  gy.storeLo = a[1];
+  gy.storeHi = b[1];
+  const int nb = gz.blockPlanes ? gz.nplanes / gz.blockPlanes : 1;
+  gz.plane0 = a[1];
+  gz.nplanes = (b[1] - a[1]) * nb;
+  if (gz.blockPlanes) gz.blockPlanes = b[1] - a[1];
+  gz.storeLo = a[2];
+  gz.storeHi = b[2];
+}
+
+void Plan::skip_ones_rows(RatioParams& q, StridedGeom& gz) const {
+  // quotient rows (y, z) in grid coordinates, as the quotient pass selects them
+  int lo[3], hi[3];
+  for (int k = 1; k < 3; ++k) {
+    lo[k] = std::max(dom.m[k] - q.qlo[k], dom.m[k] - q.o[k]);
+    hi[k] = std::min(dom.m[k] + dom.D[k] + q.qhi[k], dom.m[k] - q.o[k] + q.N[k]);
+  }
+  q.skipOnes = 1;
+  gz.onesDC = (float)dom.P[0];  // the x transform of a row of ones
+  gz.onesPlaneLo = lo[1];
+  gz.onesPlaneHi = hi[1];
+  gz.onesLo = lo[2];
+  gz.onesHi = hi[2];
+}
+
 int Plan::kernel_spectrum(const float* dKernel, float2* spec, hipStream_t s) const {
   GatherParams g{};
   g.src = dKernel;
@@ -105,10 +137,14 @@ int Plan::kernel_spectrum(const float* dKernel, float2* spec, hipStream_t s) con
 }
 
 int Plan::convolve(const GatherParams& g, const float2* spec, float2* work, float* out, hipStream_t s) const {
+  // the store reads the output rows only
+  const int zero[3] = {0, 0, 0};
+  StridedGeom gy = geom_y(), gz = geom_z(), gzo = gz;
+  restrict_rows(zero, zero, gy, gzo);
   xf(dom, g, work, twM, wr, s);
-  sz(geom_z(), 0, work, work, nullptr, 1.f, twZ, s);
-  sy(geom_y(), 2, work, work, spec, 1.f, twY, s);
-  sz(geom_z(), 1, work, work, nullptr, 1.f, twZ, s);
+  sz(gz, 0, work, work, nullptr, 1.f, twZ, s);
+  sy(gy, 2, work, work, spec, 1.f, twY, s);
+  sz(gzo, 1, work, work, nullptr, 1.f, twZ, s);
   StoreParams st{out};
   xs(dom, st, work, twM, wr, s);
   MVR_HIP(hipGetLastError());

@@ -308,10 +308,13 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
         const int x = 2 * (q + C::B * a);
         const int g0 = xmap[x], g1 = xmap[x + 1];
         const bool ok0 = off >= 0 && g0 >= 0, ok1 = off >= 0 && g1 >= 0;
-        v[a].x = g.src[ok0 ? off + g0 : 0];
-        v[a].y = g.src[ok1 ? off + g1 : 0];
-        if (!ok0) v[a].x = g.value;
-        if (!ok1) v[a].y = g.value;
+        const float* p0 = g.src + (ok0 ? off + g0 : 0);
+        if (ok0 && g1 == g0 + 1 && !(reinterpret_cast<uintptr_t>(p0) & 7)) {
+          v[a] = *reinterpret_cast<const float2*>(p0);  // an aligned contiguous pair: one 8-byte load
+        } else {
+          v[a].x = ok0 ? *p0 : g.value;
+          v[a].y = ok1 ? g.src[off + g1] : g.value;
+        }
       }
     }
     x2_forward_regs<M>(rb, tw, q, v, z);
@@ -365,6 +368,13 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
     outOff[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : kNoRow;
   }
   __syncthreads();
+  if (q_.skipOnes) {
+    // a work-group whose rows are all rows of ones has nothing to write (uniform: every thread reads the table)
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < C::RW; ++i) any |= imgRow[i] != kOuterRow;
+    if (!any) return;
+  }
   const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
   float2* rb = buf + r * C::RS;
   const long long ir = imgRow[r];
@@ -411,21 +421,29 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
       constexpr int H = (C::A + 1) / 2;  // two halves of loads in flight
 #pragma unroll
       for (int a0 = 0; a0 < C::A; a0 += H) {
-        float im[2 * H];
+        // pair h: image values of FFT positions x, x + 1 (x = 2 (q + B a)); one 8-byte load where both are read
+        // and the address is aligned
+        float2 im[H];
 #pragma unroll
-        for (int j = 0; j < 2 * H; ++j) {
-          const int a = a0 + (j >> 1);
-          const int x = 2 * (q + C::B * a) + (j & 1);
-          const bool ok = a < C::A && ir != kOuterRow && x >= x0f && x < x1f;
-          im[j] = q_.img[ok ? ir + x : 0];
-          if (!ok) im[j] = 0.f;
+        for (int h = 0; h < H; ++h) {
+          const int a = a0 + h;
+          const int x = 2 * (q + C::B * a);
+          const bool ok0 = a < C::A && ir != kOuterRow && x >= x0f && x < x1f;
+          const bool ok1 = a < C::A && ir != kOuterRow && x + 1 >= x0f && x + 1 < x1f;
+          const float* p = q_.img + (ok0 ? ir + x : 0);
+          if (ok0 && ok1 && !(reinterpret_cast<uintptr_t>(p) & 7)) {
+            im[h] = *reinterpret_cast<const float2*>(p);
+          } else {
+            im[h].x = ok0 ? *p : 0.f;
+            im[h].y = ok1 ? q_.img[ir + x + 1] : 0.f;
+          }
         }
 #pragma unroll
-        for (int j = 0; j < 2 * H; j += 2) {
-          const int a = a0 + (j >> 1);
+        for (int h = 0; h < H; ++h) {
+          const int a = a0 + h;
           if (a < C::A) {
-            z[a].x = im[j] > 0.f ? im[j] / z[a].x : 1.f;
-            z[a].y = im[j + 1] > 0.f ? im[j + 1] / z[a].y : 1.f;
+            z[a].x = im[h].x > 0.f ? im[h].x / z[a].x : 1.f;
+            z[a].y = im[h].y > 0.f ? im[h].y / z[a].y : 1.f;
           }
         }
       }
@@ -433,7 +451,8 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_,
     __syncthreads();  // the inverse's reads of the row are done before the forward writes it
     x2_forward_regs<M>(rb, tw, q, z, zf);
   }
-  x2_post_store_regs<M>(zf, q, twr, Sout + (so == kNoRow ? 0 : so), so != kNoRow);
+  x2_post_store_regs<M>(zf, q, twr, Sout + (so == kNoRow ? 0 : so),
+                        so != kNoRow && !(q_.skipOnes && ir == kOuterRow));
 }
 
 // ------------------------------------------------------------------ inverse + psi update (output rows only)

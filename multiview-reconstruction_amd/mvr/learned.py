@@ -9,7 +9,7 @@ distance to a converged reference in FP64 on the device.
 Training data come from this method itself: phantoms of the blob generator of the 256^3 / 6-view
 Optimization-II workload with seeds disjoint from the benchmark's, views blurred on the GPU
 (mvr_fft_convolve, mirror extension) and Poisson-noised, z-ramp blending weights; targets are plain
-RL run to 200 iterations from PsiInitAvgPrecise by the HIP engine.  Training draws a fixed seeded set
+RL run to 200 iterations (4000 for the deep-target network) from PsiInitAvgPrecise by the HIP engine.  Training draws a fixed seeded set
 of 64^3 patches and runs Adam on the GPU through the library's own forward and backward kernels.
 
 The 1024^2 x 512 / 8-view Efficient-Bayesian workload has its own network (8 view channels), trained the
@@ -19,6 +19,8 @@ Command line (writes the weights file)::
 
     python -m mvr.learned --workload c2 --out mvr/weights/unet_c2_6views.npz
     python -m mvr.learned --workload c5 --out mvr/weights/unet_c5_8views.npz
+    python -m mvr.learned --workload c2 --volumes 24 --patches 8000 --epochs 6 --lr 2e-3 \
+        --target-iterations 4000 --out mvr/weights/unet_c2_6views_rl4000.npz
 """
 from __future__ import annotations
 
@@ -40,6 +42,9 @@ DEFAULT_WEIGHTS = os.path.join(WEIGHTS_DIR, "unet_c2_6views.npz")
 # training workload and weights file per benchmark workload
 TRAINING = {"c2": (syn.C2, DEFAULT_WEIGHTS),
             "c5": (syn.C5_TRAIN, os.path.join(WEIGHTS_DIR, "unet_c5_8views.npz"))}
+# networks trained on deep targets (plain RL run to 4000 iterations, 24 training volumes), the start used against
+# a converged reference
+DEEP_WEIGHTS = {"c2": os.path.join(WEIGHTS_DIR, "unet_c2_6views_rl4000.npz")}
 TRAIN_SEEDS = tuple(range(100, 108))   # the benchmark uses seed index 2, the held-out check 90
 HELD_OUT_SEED = 90
 
