@@ -120,26 +120,76 @@ __host__ __device__ constexpr bool s2_fits() {
   return split2(L).A > 0 && L * W * 8 <= (L > 1024 ? 152 : 96) * 1024;
 }
 
+// The exchange between the two register steps goes through LDS in R rounds: round r carries the values of
+// one part of the second step's threads (a half for R = 2), so the tile holds about L / R values per column
+// and more work-groups share a CU's 160 KB of LDS, for two more barriers per exchange.  The plain forward and
+// inverse passes (the z passes) use two rounds up to L = 1024: at L = 300 z_fwd goes from 3 to 6 work-groups
+// per CU (2 % faster), at L = 576 from 2 to 3 (z_fwd and z_inv 5-16 % faster).  The convolution pass keeps one
+// (its kernel spectrum values, requested before the exchange, would stay live across the rounds and take back
+// the occupancy in registers), and so do the longest pencils, whose registers hold one work-group per CU
+// either way; their code is the one-round code.
+template <int L, int W, int MODE>
+__host__ __device__ constexpr int s2_rounds() {
+  return MODE != 2 && L <= 1024 ? 2 : 1;
+}
+// complex values per column the exchange tile holds
+template <int L, int W, int MODE>
+__host__ __device__ constexpr int s2_tile_rows() {
+  constexpr Split2 S = split2(L);
+  constexpr int R = s2_rounds<L, W, MODE>();
+  constexpr int fa = (S.A + R - 1) / R * S.B, fb = S.A * ((S.B + R - 1) / R);
+  return R == 1 ? L : (fa > fb ? fa : fb);
+}
+
 // The twiddles stay in global memory (read through L1) when that lets a second work-group share the CU's
 // 160 KB of LDS; such kernels are then compiled for the waves per SIMD that two work-groups need.
-template <int L, int W>
+template <int L, int W, int MODE>
 __host__ __device__ constexpr bool s2_global_tw() {
-  return 2 * (L * W * 8 + L * 8) > 160 * 1024 && 2 * L * W * 8 <= 160 * 1024;
+  constexpr int T = s2_tile_rows<L, W, MODE>() * W * 8;
+  return 2 * (T + L * 8) > 160 * 1024 && 2 * T <= 160 * 1024;
 }
-template <int L, int W>
+template <int L, int W, int MODE>
 __host__ __device__ constexpr int s2_min_waves() {
-  return s2_global_tw<L, W>() ? (2 * ((W * split2(L).A + 63) / 64) + 3) / 4 : 1;
+  return s2_global_tw<L, W, MODE>() ? (2 * ((W * split2(L).A + 63) / 64) + 3) / 4 : 1;
+}
+
+// Inverse exchange in R >= 2 rounds: thread q = k1 < A holds u[n2] for n2 < B, to be multiplied by
+// conj(W_L^{n2 k1}); afterwards thread q = n2 < B holds v[k1] = the twiddled u[n2] of thread k1, for k1 < A.
+// Round r carries n2 in [r BH, r BH + BH).  Starts with a barrier: every thread is done with the tile before it.
+template <int A, int B, int W, int R>
+__device__ __forceinline__ void s2_inverse_exchange(float2* tile, float2* u, float2* v, int q, int w,
+                                                    const float2* tw) {
+  {
+    constexpr int BH = (B + R - 1) / R;
+#pragma unroll
+    for (int n2 = 1; n2 < B; ++n2) u[n2] = cmul(u[n2], cconj(tw[n2 * q]));
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int n0 = r * BH, nn = (r + 1) * BH < B ? BH : B - r * BH;
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < BH; ++j)
+        if (j < nn) tile[(q + A * j) * W + w] = u[n0 + j];
+      __syncthreads();
+      if (q >= n0 && q < n0 + nn) {
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) v[k1] = tile[(k1 + A * (q - n0)) * W + w];
+      }
+    }
+  }
 }
 
 template <int L, int W, int MODE>
-__global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W>())) void k_strided2(
+__global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W, MODE>())) void k_strided2(
     StridedGeom gm, const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ spec,
     float scale, const float2* __restrict__ twg) {
   constexpr Split2 S = split2(L);
   constexpr int A = S.A, B = S.B;
   constexpr int NT = W * A;
-  constexpr bool TWG = s2_global_tw<L, W>();
-  __shared__ float2 tile[L * W];
+  constexpr bool TWG = s2_global_tw<L, W, MODE>();
+  constexpr int R = s2_rounds<L, W, MODE>();
+  constexpr int AH = (A + R - 1) / R, BH = (B + R - 1) / R;  // second-step threads (k1, resp. n2) per round
+  __shared__ float2 tile[s2_tile_rows<L, W, MODE>() * W];
   __shared__ float2 twS[TWG ? 1 : L];
   const float2* tw = TWG ? twg : twS;
   if (!TWG)
@@ -186,10 +236,15 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W>())) void k_str
     __syncthreads();  // twiddle table
     if (q < B) {
       rdft<A, -1>(v);
+      if constexpr (R == 1) {
 #pragma unroll
-      for (int k1 = 0; k1 < A; ++k1) {
-        const float2 t = k1 == 0 ? v[k1] : cmul(v[k1], tw[q * k1]);
-        tile[(q + B * k1) * W + w] = t;
+        for (int k1 = 0; k1 < A; ++k1) {
+          const float2 t = k1 == 0 ? v[k1] : cmul(v[k1], tw[q * k1]);
+          tile[(q + B * k1) * W + w] = t;
+        }
+      } else {
+#pragma unroll
+        for (int k1 = 1; k1 < A; ++k1) v[k1] = cmul(v[k1], tw[q * k1]);
       }
     }
     // convolution: the kernel spectrum values of this thread's outputs are requested before the exchange,
@@ -200,11 +255,29 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W>())) void k_str
 #pragma unroll
       for (int k2 = 0; k2 < B; ++k2) kv[k2] = colOk ? sp[(size_t)(q + A * k2) * ls] : make_float2(0.f, 0.f);
     }
-    __syncthreads();
-    // ---- forward step 2: thread q = k1 < A
+    // ---- exchange (round r: the values of k1 in [r AH, r AH + AH)), then forward step 2: thread q = k1 < A
     float2 u[B];
+    if constexpr (R == 1) {
+      __syncthreads();
 #pragma unroll
-    for (int b = 0; b < B; ++b) u[b] = tile[(b + B * q) * W + w];
+      for (int b = 0; b < B; ++b) u[b] = tile[(b + B * q) * W + w];
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int k0 = r * AH, kn = (r + 1) * AH < A ? AH : A - r * AH;
+        if (r > 0) __syncthreads();  // the previous round's readers are done
+        if (q < B) {
+#pragma unroll
+          for (int j = 0; j < AH; ++j)
+            if (j < kn) tile[(q + B * j) * W + w] = v[k0 + j];
+        }
+        __syncthreads();
+        if (q >= k0 && q < k0 + kn) {
+#pragma unroll
+          for (int b = 0; b < B; ++b) u[b] = tile[(b + B * (q - k0)) * W + w];
+        }
+      }
+    }
     rdft<B, -1>(u);  // u[k2] = X[q + A k2]
     if constexpr (MODE == 0 || MODE == 3) {
       if (colOk) {
@@ -221,11 +294,15 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W>())) void k_str
 #pragma unroll
       for (int k2 = 0; k2 < B; ++k2) u[k2] = cmul(u[k2], kv[k2]);
       rdft<B, 1>(u);  // u[n2] = sum_k2 X[q + A k2] W_B^{-n2 k2}
-      __syncthreads();  // every thread has read its forward values from the tile
+      if constexpr (R == 1) {
+        __syncthreads();  // every thread has read its forward values from the tile
 #pragma unroll
-      for (int n2 = 0; n2 < B; ++n2) {
-        const float2 t = n2 == 0 ? u[n2] : cmul(u[n2], cconj(tw[n2 * q]));
-        tile[(q + A * n2) * W + w] = t;
+        for (int n2 = 0; n2 < B; ++n2) {
+          const float2 t = n2 == 0 ? u[n2] : cmul(u[n2], cconj(tw[n2 * q]));
+          tile[(q + A * n2) * W + w] = t;
+        }
+      } else {
+        s2_inverse_exchange<A, B, W, R>(tile, u, v, q, w, tw);
       }
     }
   } else {
@@ -235,18 +312,24 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W>())) void k_str
     for (int k2 = 0; k2 < B; ++k2) u[k2] = colOk ? src[(size_t)(q + A * k2) * ls] : make_float2(0.f, 0.f);
     __syncthreads();  // twiddle table
     rdft<B, 1>(u);
+    if constexpr (R == 1) {
 #pragma unroll
-    for (int n2 = 0; n2 < B; ++n2) {
-      const float2 t = n2 == 0 ? u[n2] : cmul(u[n2], cconj(tw[n2 * q]));
-      tile[(q + A * n2) * W + w] = t;
+      for (int n2 = 0; n2 < B; ++n2) {
+        const float2 t = n2 == 0 ? u[n2] : cmul(u[n2], cconj(tw[n2 * q]));
+        tile[(q + A * n2) * W + w] = t;
+      }
+    } else {
+      s2_inverse_exchange<A, B, W, R>(tile, u, v, q, w, tw);
     }
   }
   if constexpr (MODE == 1 || MODE == 2) {
-    __syncthreads();
+    if constexpr (R == 1) __syncthreads();
     // ---- inverse step 2: thread q = n2 < B does the A-point DFT over k1 -> x[n2 + B n1]
     if (q < B) {
+      if constexpr (R == 1) {
 #pragma unroll
-      for (int k1 = 0; k1 < A; ++k1) v[k1] = tile[(k1 + A * q) * W + w];
+        for (int k1 = 0; k1 < A; ++k1) v[k1] = tile[(k1 + A * q) * W + w];
+      }
       rdft<A, 1>(v);
       if (colOk) {
         // positions the next pass reads (storeLo <= q + B n1 < storeHi)
