@@ -77,9 +77,19 @@ struct XCfg {
   // both a third work-group per CU and 2-5 %) and keep LDS twiddles on shorter rows (at M = 150 global twiddles
   // were slower despite the extra work-group).
   static constexpr bool long_rows = B > 16;
-  static constexpr bool TWL_FWD =
-      !long_rows || (160 * 1024) / (LDS_ROWS + PX * 4 + LDS_TW) >= (160 * 1024) / (LDS_ROWS + PX * 4);
+  // The forward pass exchanges in two rounds (half of the second step's lanes per round), so its rows hold
+  // half a transform: at M = 150 8 instead of 6 work-groups per CU, at M = 576 4 instead of 3.
+  static constexpr int AH = (A + 1) / 2;
+  static constexpr int RS_FWD = AH * B + 5;
+  static constexpr int BUF_FWD = RW * RS_FWD;
+  static constexpr int LDS_ROWS_FWD = RW * RS_FWD * 8 + RW * 24;
+  static constexpr bool TWL_FWD = !long_rows || (160 * 1024) / (LDS_ROWS_FWD + PX * 4 + LDS_TW) >=
+                                                   (160 * 1024) / (LDS_ROWS_FWD + PX * 4);
   static constexpr bool TWL_RATIO = !long_rows || TWL;
+  // The quotient pass at M = 100, 120, 150, 200 compiles to 81 VGPRs (this ROCm), one over what 6 waves per
+  // SIMD allow; bounded to 6 waves it takes 80 without spilling (x_ratio 4 % faster at M = 150).  Other lengths
+  // either fit already or would spill under the bound.
+  static constexpr int MINW_RATIO = (M == 100 || M == 120 || M == 150 || M == 200) ? 6 : 1;
 };
 
 template <int M>
@@ -126,6 +136,37 @@ __device__ __forceinline__ void x2_forward_regs(float2* rb, const float2* tw, in
 #pragma unroll
     for (int b = 0; b < B; ++b) out[b] = make_float2(0.f, 0.f);
   }
+}
+
+// x2_forward_regs with the exchange in two rounds through a row `rb` of AH * B values: round r carries the
+// first-step results k1 in [r AH, r AH + AH), read by the lanes q = k1 of that range
+template <int M>
+__device__ __forceinline__ void x2_forward_regs_half(float2* rb, const float2* tw, int q, float2* v, float2* out) {
+  using C = XCfg<M>;
+  constexpr int A = C::A, B = C::B, AH = C::AH;
+  if (q < B) {
+    rdft<A, -1>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < A; ++k1) v[k1] = cmul(v[k1], tw[q * k1]);
+  }
+#pragma unroll
+  for (int b = 0; b < B; ++b) out[b] = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int k0 = r * AH, kn = r == 0 ? AH : A - AH;
+    if (r > 0) __syncthreads();  // the first round's readers are done
+    if (q < B) {
+#pragma unroll
+      for (int j = 0; j < AH; ++j)
+        if (j < kn) rb[q + B * j] = v[k0 + j];
+    }
+    __syncthreads();
+    if (q >= k0 && q < k0 + kn) {
+#pragma unroll
+      for (int b = 0; b < B; ++b) out[b] = rb[b + B * (q - k0)];
+    }
+  }
+  if (q < A) rdft<B, -1>(out);
 }
 
 // forward transform of LDS row `rb` (natural order) -> out[k2] = Z[q + A k2] (lane q < A)
@@ -236,7 +277,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
                                                          const float2* __restrict__ twM,
                                                          const float2* __restrict__ wr) {
   using C = XCfg<M>;
-  __shared__ float2 buf[C::BUF];
+  __shared__ float2 buf[C::BUF_FWD];
   __shared__ float2 twS[C::TWL_FWD ? M : 1];
   __shared__ float2 twrS[C::TWL_FWD ? M + 1 : 1];
   const float2* tw = C::TWL_FWD ? twS : twM;
@@ -258,34 +299,32 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
   }
   const int row0 = wg * C::RW;
   const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
-  float2* rb = buf + r * C::RS;
-  float2 z[C::B];
-  if (g.mode == GATHER_KERNEL) {
-    // kernel placement (setup only): per element into LDS, then the transform from LDS
-    float* fb = reinterpret_cast<float*>(buf);
-    for (int idx = threadIdx.x; idx < C::RW * C::PX; idx += C::NT) {
-      const int rr = idx / C::PX, x = idx - rr * C::PX;
-      const int row = row0 + rr;
-      float v = 0.f;
-      if (row < nrows) v = gather_value(g, d, x, row % d.P[1], row / d.P[1]);
-      fb[rr * 2 * C::RS + x] = v;
-    }
-    if (threadIdx.x < C::RW) {
-      const int row = row0 + threadIdx.x;
-      dstRow[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : kNoRow;
-    }
-    __syncthreads();
-    x2_forward<M>(rb, tw, q, z);
-  } else {
-    for (int x = threadIdx.x; x < C::PX; x += C::NT) {
+  float2* rb = buf + r * C::RS_FWD;
+  // source index of FFT position x of a row: row start (srcRow, -1: no source row) + xmap[x] (-1: outside),
+  // else g.value.  Mirror: the volume mirrored at its faces; value: the volume, `value` outside; kernel
+  // placement: K[(p + K/2) mod P], 0 beyond K (GATHER_KERNEL, value 0).
+  const bool kern = g.mode == GATHER_KERNEL;
+  const float fill = kern ? 0.f : g.value;
+  for (int x = threadIdx.x; x < C::PX; x += C::NT) {
+    int gx;
+    if (kern) {
+      gx = (x + g.N[0] / 2) % d.P[0];
+      if (gx >= g.N[0]) gx = -1;
+    } else {
       const int p = org[0] + x;
-      xmap[x] = g.mode == GATHER_MIRROR ? mirror_index(p, g.N[0]) : ((unsigned)p < (unsigned)g.N[0] ? p : -1);
+      gx = g.mode == GATHER_MIRROR ? mirror_index(p, g.N[0]) : ((unsigned)p < (unsigned)g.N[0] ? p : -1);
     }
-    if (threadIdx.x < C::RW) {
-      const int row = row0 + threadIdx.x;
-      long long off = -1;
-      if (row < nrows) {
-        const int y = row % d.P[1], zz = row / d.P[1];
+    xmap[x] = gx;
+  }
+  if (threadIdx.x < C::RW) {
+    const int row = row0 + threadIdx.x;
+    long long off = -1;
+    if (row < nrows) {
+      const int y = row % d.P[1], zz = row / d.P[1];
+      if (kern) {
+        const int jy = (y + g.N[1] / 2) % d.P[1], jz = (zz + g.N[2] / 2) % d.P[2];
+        if (jy < g.N[1] && jz < g.N[2]) off = ((long long)jz * g.N[1] + jy) * g.N[0];
+      } else {
         int gy = org[1] + y, gz = org[2] + zz;
         if (g.mode == GATHER_MIRROR) {
           gy = mirror_index(gy, g.N[1]);
@@ -295,30 +334,31 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
           off = ((long long)gz * g.N[1] + gy) * g.N[0];
         }
       }
-      srcRow[threadIdx.x] = off;   // row start in src (>= 0), -1: row outside the volume
-      dstRow[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : kNoRow;
     }
-    __syncthreads();
-    // the first transform step reads its values z[q + B a] = (x[2n], x[2n + 1]) straight from psi
-    float2 v[C::A];
-    if (q < C::B) {
-      const long long off = srcRow[r];
+    srcRow[threadIdx.x] = off;   // row start in src (>= 0), -1: no source row
+    dstRow[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : kNoRow;
+  }
+  __syncthreads();
+  // the first transform step reads its values z[q + B a] = (x[2n], x[2n + 1]) straight from the source
+  float2 v[C::A];
+  if (q < C::B) {
+    const long long off = srcRow[r];
 #pragma unroll
-      for (int a = 0; a < C::A; ++a) {
-        const int x = 2 * (q + C::B * a);
-        const int g0 = xmap[x], g1 = xmap[x + 1];
-        const bool ok0 = off >= 0 && g0 >= 0, ok1 = off >= 0 && g1 >= 0;
-        const float* p0 = g.src + (ok0 ? off + g0 : 0);
-        if (ok0 && g1 == g0 + 1 && !(reinterpret_cast<uintptr_t>(p0) & 7)) {
-          v[a] = *reinterpret_cast<const float2*>(p0);  // an aligned contiguous pair: one 8-byte load
-        } else {
-          v[a].x = ok0 ? *p0 : g.value;
-          v[a].y = ok1 ? g.src[off + g1] : g.value;
-        }
+    for (int a = 0; a < C::A; ++a) {
+      const int x = 2 * (q + C::B * a);
+      const int g0 = xmap[x], g1 = xmap[x + 1];
+      const bool ok0 = off >= 0 && g0 >= 0, ok1 = off >= 0 && g1 >= 0;
+      const float* p0 = g.src + (ok0 ? off + g0 : 0);
+      if (ok0 && g1 == g0 + 1 && !(reinterpret_cast<uintptr_t>(p0) & 7)) {
+        v[a] = *reinterpret_cast<const float2*>(p0);  // an aligned contiguous pair: one 8-byte load
+      } else {
+        v[a].x = ok0 ? *p0 : fill;
+        v[a].y = ok1 ? g.src[off + g1] : fill;
       }
     }
-    x2_forward_regs<M>(rb, tw, q, v, z);
   }
+  float2 z[C::B];
+  x2_forward_regs_half<M>(rb, tw, q, v, z);
   const size_t o = dstRow[r];
   x2_post_store_regs<M>(z, q, twr, S + (o == kNoRow ? 0 : o), o != kNoRow);
 }
@@ -327,7 +367,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
 // ratio = img > 0 ? img / blurred : 1 on the quotient region, 1 elsewhere
 // (DeconvolutionMethods.computeQuotient, D/iteration/sequential/DeconvolutionMethods.java:46-98)
 template <int M>
-__global__ __launch_bounds__(XCfg<M>::NT) void k_x2_ratio(Dom d, RatioParams q_, const float2* __restrict__ Sin,
+__global__ __launch_bounds__(XCfg<M>::NT, XCfg<M>::MINW_RATIO) void k_x2_ratio(Dom d, RatioParams q_, const float2* __restrict__ Sin,
                                                            float2* __restrict__ Sout,
                                                            const float2* __restrict__ twM,
                                                            const float2* __restrict__ wr) {
