@@ -579,14 +579,17 @@ int Decon::view_seq(int v, double2* statOut, int slot) {
   // the first convolution is read on the quotient rows, the second on the output rows; on one frame the
   // quotient pass leaves the rows of ones to the z pass after it
   const int zero[3] = {0, 0, 0};
-  StridedGeom gy1 = gy, gz1 = gz, gy2 = gy, gz2 = gz, gzq = gz;
+  StridedGeom gy1 = gy, gz1 = gz, gy2 = gy, gz2 = gz, gzq = gz, gzf = gz;
   plan.restrict_rows(q.qlo, q.qhi, gy1, gz1);
   plan.restrict_rows(zero, zero, gy2, gz2);
-  if (!blocked()) plan.skip_ones_rows(q, gzq);
+  if (!blocked()) {
+    plan.skip_ones_rows(q, gzq);
+    plan.mirror_rows(g, gzf, gy1);  // psi's mirror copies are read from their sources
+  }
   timer.begin(stream, PASS_X_FWD);
   plan.xf(d, g, S, plan.twM, plan.wr, stream);
   timer.end(stream);
-  plan.sz(timed(timer, gz, PASS_Z_FWD), 0, S, S, nullptr, 1.f, plan.twZ, stream);
+  plan.sz(timed(timer, gzf, PASS_Z_FWD), 0, S, S, nullptr, 1.f, plan.twZ, stream);
   plan.sy(timed(timer, gy1, PASS_Y_CONV), 2, S, S, spec1[v], 1.f, plan.twY, stream);
   plan.sz(timed(timer, gz1, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
   timer.begin(stream, PASS_X_RATIO);
@@ -636,14 +639,16 @@ int Decon::iteration_mul(double2* statOut) {
   g.src = psi;
   for (int k = 0; k < 3; ++k) { g.N[k] = N[k]; g.org[k] = -d.m[k]; }
   g.mode = GATHER_MIRROR;
+  StridedGeom gzf = gz, gyf = gyo;  // psi's mirror copies are read from their sources
+  plan.mirror_rows(g, gzf, gyf);
   // the forward transform of psi is shared by all views
   timer.begin(stream, PASS_X_FWD);
   plan.xf(d, g, Spsi, plan.twM, plan.wr, stream);
   timer.end(stream);
-  plan.sz(timed(timer, gz, PASS_Z_FWD), 0, Spsi, Spsi, nullptr, 1.f, plan.twZ, stream);
+  plan.sz(timed(timer, gzf, PASS_Z_FWD), 0, Spsi, Spsi, nullptr, 1.f, plan.twZ, stream);
   std::vector<const float*> t2p(V), wp(V);
   for (int v = 0; v < V; ++v) {
-    plan.sy(timed(timer, gyo, PASS_Y_CONV), 2, Spsi, S, spec1[v], 1.f, plan.twY, stream);
+    plan.sy(timed(timer, gyf, PASS_Y_CONV), 2, Spsi, S, spec1[v], 1.f, plan.twY, stream);
     plan.sz(timed(timer, gzo, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
     RatioParams q{};
     q.img = img[v];
@@ -931,9 +936,11 @@ int BlockThread::iterate_seq(int v, float maxI) {
   g.src = psiB;
   for (int k = 0; k < 3; ++k) { g.N[k] = B[k]; g.org[k] = -d.m[k]; }
   g.mode = GATHER_MIRROR;
+  StridedGeom gzf = gz, gyf = gyo;  // the block's mirror copies are read from their sources
+  plan.mirror_rows(g, gzf, gyf);
   plan.xf(d, g, S, plan.twM, plan.wr, stream);
-  plan.sz(gz, 0, S, S, nullptr, 1.f, plan.twZ, stream);
-  plan.sy(gyo, 2, S, S, spec1[v], 1.f, plan.twY, stream);
+  plan.sz(gzf, 0, S, S, nullptr, 1.f, plan.twZ, stream);
+  plan.sy(gyf, 2, S, S, spec1[v], 1.f, plan.twY, stream);
   plan.sz(gzo, 1, S, S, nullptr, 1.f, plan.twZ, stream);
   RatioParams q{};
   q.img = imgB;
@@ -1009,11 +1016,13 @@ int BlockThread::iterate_mul(int nv, const float* maxIs) {
   g.src = psiB;
   for (int k = 0; k < 3; ++k) { g.N[k] = B[k]; g.org[k] = -d.m[k]; }
   g.mode = GATHER_MIRROR;
+  StridedGeom gzf = gz, gyf = gyo;  // the block's mirror copies are read from their sources
+  plan.mirror_rows(g, gzf, gyf);
   plan.xf(d, g, Spsi, plan.twM, plan.wr, stream);
-  plan.sz(gz, 0, Spsi, Spsi, nullptr, 1.f, plan.twZ, stream);
+  plan.sz(gzf, 0, Spsi, Spsi, nullptr, 1.f, plan.twZ, stream);
   std::vector<const float*> t2p(nv), wp(nv);
   for (int v = 0; v < nv; ++v) {
-    plan.sy(gyo, 2, Spsi, S, spec1[v], 1.f, plan.twY, stream);
+    plan.sy(gyf, 2, Spsi, S, spec1[v], 1.f, plan.twY, stream);
     plan.sz(gzo, 1, S, S, nullptr, 1.f, plan.twZ, stream);
     RatioParams q{};
     q.img = imgV[v];

@@ -45,6 +45,9 @@ struct GatherParams {
   int org[3];
   int mode;
   float value;
+  // MIRROR only: transform only the rows (y, z) that are not mirror copies, FFT positions [-org, -org + N)
+  // per axis (the passes after it read the copies from their sources)
+  int uniqueRows;
 };
 
 // Inverse-transform epilogues.  `o` is the volume coordinate of the output
@@ -105,6 +108,9 @@ struct StridedGeom {
   // other columns.  Off when onesDC is 0.
   float onesDC = 0.f;
   int onesPlaneLo = 0, onesPlaneHi = 0, onesLo = 0, onesHi = 0;
+  // Forward modes over data that is a mirror extension along the pencil (mirN > 0): position n holds what
+  // position mirM + mirror(n - mirM, mirN) holds, and only those positions were written; the loads read them.
+  int mirM = 0, mirN = 0;
   // timing: when set, the launch records the kernel's own start and stop in these events
   hipEvent_t evStart = nullptr, evStop = nullptr;
 };

@@ -228,6 +228,24 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W, MODE>())) void
           const bool rd = colOk && planeIn && n >= gm.onesLo && n < gm.onesHi;
           v[a] = rd ? src[(size_t)n * ls] : (colOk ? one : make_float2(0.f, 0.f));
         }
+      } else if (gm.mirN > 0) {
+        // a mirror extension along the pencil: the copies are read from their sources.  One reflection at
+        // each face suffices when the pencil reaches less than mirN - 1 beyond either face (the usual case).
+        if (gm.mirM < gm.mirN && L - gm.mirM <= 2 * gm.mirN - 1) {
+#pragma unroll
+          for (int a = 0; a < A; ++a) {
+            int n = q + B * a - gm.mirM;
+            n = n < 0 ? -n : n;
+            n = n >= gm.mirN ? 2 * gm.mirN - 2 - n : n;
+            v[a] = colOk ? src[(size_t)(gm.mirM + n) * ls] : make_float2(0.f, 0.f);
+          }
+        } else {
+#pragma unroll
+          for (int a = 0; a < A; ++a) {
+            const int n = mirror_index(q + B * a - gm.mirM, gm.mirN);
+            v[a] = colOk ? src[(size_t)(gm.mirM + n) * ls] : make_float2(0.f, 0.f);
+          }
+        }
       } else {
 #pragma unroll
         for (int a = 0; a < A; ++a) v[a] = colOk ? src[(size_t)(q + B * a) * ls] : make_float2(0.f, 0.f);

@@ -124,6 +124,17 @@ void Plan::skip_ones_rows(RatioParams& q, StridedGeom& gz) const {
   gz.onesHi = hi[2];
 }
 
+void Plan::mirror_rows(GatherParams& g, StridedGeom& gzf, StridedGeom& gyc) const {
+  g.uniqueRows = 1;
+  // unique rows: FFT positions [-org, -org + N) per axis, inside the grid
+  gzf.plane0 = -g.org[1];
+  gzf.nplanes = g.N[1];
+  gzf.mirM = -g.org[2];
+  gzf.mirN = g.N[2];
+  gyc.mirM = -g.org[1];
+  gyc.mirN = g.N[1];
+}
+
 int Plan::kernel_spectrum(const float* dKernel, float2* spec, hipStream_t s) const {
   GatherParams g{};
   g.src = dKernel;
@@ -141,7 +152,9 @@ int Plan::convolve(const GatherParams& g, const float2* spec, float2* work, floa
   const int zero[3] = {0, 0, 0};
   StridedGeom gy = geom_y(), gz = geom_z(), gzo = gz;
   restrict_rows(zero, zero, gy, gzo);
-  xf(dom, g, work, twM, wr, s);
+  GatherParams gg = g;
+  if (g.mode == GATHER_MIRROR) mirror_rows(gg, gz, gy);
+  xf(dom, gg, work, twM, wr, s);
   sz(gz, 0, work, work, nullptr, 1.f, twZ, s);
   sy(gy, 2, work, work, spec, 1.f, twY, s);
   sz(gzo, 1, work, work, nullptr, 1.f, twZ, s);

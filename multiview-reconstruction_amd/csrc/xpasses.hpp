@@ -300,6 +300,9 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
   const int row0 = wg * C::RW;
   const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
   float2* rb = buf + r * C::RS_FWD;
+  // unique rows only (mirror gather): row index -> grid row (-org1 + i % N1, -org2 + i / N1)
+  const bool uniq = g.uniqueRows && g.mode == GATHER_MIRROR;
+  const int nrowsRun = uniq ? g.N[1] * g.N[2] : nrows;
   // source index of FFT position x of a row: row start (srcRow, -1: no source row) + xmap[x] (-1: outside),
   // else g.value.  Mirror: the volume mirrored at its faces; value: the volume, `value` outside; kernel
   // placement: K[(p + K/2) mod P], 0 beyond K (GATHER_KERNEL, value 0).
@@ -317,9 +320,11 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
     xmap[x] = gx;
   }
   if (threadIdx.x < C::RW) {
-    const int row = row0 + threadIdx.x;
+    int row = row0 + threadIdx.x;
     long long off = -1;
-    if (row < nrows) {
+    const bool live = row < nrowsRun;
+    if (uniq && live) row = (-org[2] + row / g.N[1]) * d.P[1] + (-org[1] + row % g.N[1]);
+    if (live) {
       const int y = row % d.P[1], zz = row / d.P[1];
       if (kern) {
         const int jy = (y + g.N[1] / 2) % d.P[1], jz = (zz + g.N[2] / 2) % d.P[2];
@@ -336,7 +341,7 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_fwd(Dom d, GatherParams g, f
       }
     }
     srcRow[threadIdx.x] = off;   // row start in src (>= 0), -1: no source row
-    dstRow[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : kNoRow;
+    dstRow[threadIdx.x] = live ? (size_t)row * d.HxP : kNoRow;
   }
   __syncthreads();
   // the first transform step reads its values z[q + B a] = (x[2n], x[2n + 1]) straight from the source
@@ -673,7 +678,7 @@ void launch_x_fwd(const Dom& d, const GatherParams& g, float2* S, const float2* 
   static_assert(x2_ok<M>(), "every compiled x length has a two-step split");
   using C = XCfg<M>;
   Dom dd = d;
-  const int rows = d.P[1] * d.P[2];
+  const int rows = g.uniqueRows && g.mode == GATHER_MIRROR ? g.N[1] * g.N[2] : d.P[1] * d.P[2];
   dd.wgPerBlock = (rows + C::RW - 1) / C::RW;
   const int nb = d.btab ? d.nb : 1;
   hipLaunchKernelGGL(k_x2_fwd<M>, dim3(dd.wgPerBlock * nb), dim3(C::NT), 0, s, dd, g, S, twM, wr);

@@ -41,6 +41,8 @@ def test_device_query():
     ((1, 8, 8), (1, 3, 3), "mirror", 0.0),
     ((264, 6, 444), (7, 3, 7), "mirror", 0.0),   # FFT lengths 270 (z) and 450 (x: 225-point halves)
     ((10, 264, 84), (5, 7, 7), "value", 1.0),    # 270 along y
+    ((4, 5, 3), (13, 11, 9), "mirror", 0.0),     # kernel wider than the volume: the mirror wraps repeatedly
+    ((6, 2, 7), (5, 9, 5), "mirror", 0.0),
 ])
 def test_fft_convolve_matches_oracle(oracle, shape, kshape, ext, value):
     img = RNG.uniform(0.5, 2.0, shape).astype(np.float32)
