@@ -33,21 +33,19 @@ HBM_PEAK_GBS = 8000.0
 # the FFT extent E (N + K - 1 per axis; padding to the FFT length is never credited), the real volumes (psi, the
 # view image, the weights) 4 B per volume voxel (N per axis), each read or written once.  On one frame a pass
 # moves only the rows its neighbours use: a convolution's inverse passes produce the rows the next x pass reads
-# (the output region's y and z; the quotient region is the output region here), the quotient pass writes only
-# the quotient rows, the rows of ones around them being put in by the z pass that reads them, and the first
-# convolution transforms only psi's own rows (y, z in N), its mirror copies being read from their sources:
+# (the output region's y and z; the quotient region is the output region here); the first convolution
+# transforms only psi's own rows (y, z in N), its mirror copies being read from their sources; the quotient pass
+# writes only the quotient rows, and the rows and y planes of ones around them are put in by the passes that
+# read them (their spectra are known):
 #   x_fwd   : psi 4 N + write of psi's rows 4 Ex Ny Nz
-#   z_fwd   : after x_fwd read of psi's rows 4 Ex Ny Nz + write of psi's y, 4 Ex Ny Ez; after x_ratio read of the
-#             quotient rows 4 Ex Ny Nz + write 4 E (the two launches of a view update averaged)
-#   y_conv  : first psi's y 4 Ex Ny Ez, second the spectrum 4 E; kernel spectrum read 4 E; write of the output
-#             region's y, 4 Ex Ny Ez (the two launches averaged)
+#   z_fwd   : read of psi's (or the quotient's) rows 4 Ex Ny Nz + write of their y planes, 4 Ex Ny Ez
+#   y_conv  : read of those y, 4 Ex Ny Ez + kernel spectrum 4 E + write of the output region's y, 4 Ex Ny Ez
 #   z_inv   : read of the output region's y, 4 Ex Ny Ez + write of its y and z, 4 Ex Ny Nz
 #   x_ratio : spectrum rows of the output region 4 Ex Ny Nz + view image 4 N + write of those rows 4 Ex Ny Nz
 #   x_update: spectrum rows of the output region 4 Ex Ny Nz + psi read 4 N + weights 4 N + psi write 4 N
 #   x_store : spectrum rows of the output region 4 Ex Ny Nz + store 4 N
-# "z_fwd/a", "z_fwd/b", "y_conv/a", "y_conv/b" are the first and second launches on their own.  The whole view
-# update is also priced on the survey's budget of 60 B per extent voxel (five passes with the xy transforms
-# fused), RUN_BYTES_PER_EXTENT_VOXEL.
+# The whole view update is also priced on the survey's budget of 60 B per extent voxel (five passes with the xy
+# transforms fused), RUN_BYTES_PER_EXTENT_VOXEL.
 RUN_BYTES_PER_EXTENT_VOXEL = 60
 
 
@@ -58,12 +56,9 @@ def pass_bytes(name, n_zyx, k_zyx):
     vol = nz * ny * nx
     rows = ex * ny * nz     # spectrum rows of the output region
     ycols = ex * ny * ez    # the output region's y on every z plane
-    b = {"x_fwd": 4 * vol + 4 * rows, "z_fwd/a": 4 * rows + 4 * ycols, "z_fwd/b": 4 * rows + 4 * ext,
-         "y_conv/a": 4 * ext + 8 * ycols, "y_conv/b": 8 * ext + 4 * ycols, "z_inv": 4 * ycols + 4 * rows,
-         "x_ratio": 8 * rows + 4 * vol, "x_update": 4 * rows + 12 * vol, "x_store": 4 * rows + 4 * vol}
-    b["z_fwd"] = (b["z_fwd/a"] + b["z_fwd/b"]) // 2
-    b["y_conv"] = (b["y_conv/a"] + b["y_conv/b"]) // 2
-    return b[name]
+    return {"x_fwd": 4 * vol + 4 * rows, "z_fwd": 4 * rows + 4 * ycols, "y_conv": 4 * ext + 8 * ycols,
+            "z_inv": 4 * ycols + 4 * rows, "x_ratio": 8 * rows + 4 * vol,
+            "x_update": 4 * rows + 12 * vol, "x_store": 4 * rows + 4 * vol}[name]
 
 
 def log(*a):

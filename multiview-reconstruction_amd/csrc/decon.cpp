@@ -583,7 +583,7 @@ int Decon::view_seq(int v, double2* statOut, int slot) {
   plan.restrict_rows(q.qlo, q.qhi, gy1, gz1);
   plan.restrict_rows(zero, zero, gy2, gz2);
   if (!blocked()) {
-    plan.skip_ones_rows(q, gzq);
+    plan.skip_ones_rows(q, gzq, &gy2);
     plan.mirror_rows(g, gzf, gy1);  // psi's mirror copies are read from their sources
   }
   timer.begin(stream, PASS_X_FWD);
@@ -653,13 +653,13 @@ int Decon::iteration_mul(double2* statOut) {
     RatioParams q{};
     q.img = img[v];
     for (int k = 0; k < 3; ++k) { q.N[k] = N[k]; q.o[k] = 0; }
-    StridedGeom gzq = gz;
-    plan.skip_ones_rows(q, gzq);
+    StridedGeom gzq = gz, gyq = gyo;
+    plan.skip_ones_rows(q, gzq, &gyq);
     timer.begin(stream, PASS_X_RATIO);
     plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
     timer.end(stream);
     plan.sz(timed(timer, gzq, PASS_Z_FWD), 0, S, S, nullptr, 1.f, plan.twZ, stream);
-    plan.sy(timed(timer, gyo, PASS_Y_CONV), 2, S, S, spec2[v], 1.f, plan.twY, stream);
+    plan.sy(timed(timer, gyq, PASS_Y_CONV), 2, S, S, spec2[v], 1.f, plan.twY, stream);
     plan.sz(timed(timer, gzo, PASS_Z_INV), 1, S, S, nullptr, 1.f, plan.twZ, stream);
     StoreParams st{T2 + nvox() * v};
     timer.begin(stream, PASS_X_STORE);
@@ -945,11 +945,11 @@ int BlockThread::iterate_seq(int v, float maxI) {
   RatioParams q{};
   q.img = imgB;
   for (int k = 0; k < 3; ++k) { q.N[k] = B[k]; q.o[k] = 0; }
-  StridedGeom gzq = gz;
-  plan.skip_ones_rows(q, gzq);
+  StridedGeom gzq = gz, gyq = gyo;
+  plan.skip_ones_rows(q, gzq, &gyq);
   plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
   plan.sz(gzq, 0, S, S, nullptr, 1.f, plan.twZ, stream);
-  plan.sy(gyo, 2, S, S, spec2[v], 1.f, plan.twY, stream);
+  plan.sy(gyq, 2, S, S, spec2[v], 1.f, plan.twY, stream);
   plan.sz(gzo, 1, S, S, nullptr, 1.f, plan.twZ, stream);
   if (ensure_partials((int)(plan.rows_inner() / 4 + 1))) return -1;
   UpdateParams u{};
@@ -1027,11 +1027,11 @@ int BlockThread::iterate_mul(int nv, const float* maxIs) {
     RatioParams q{};
     q.img = imgV[v];
     for (int k = 0; k < 3; ++k) { q.N[k] = B[k]; q.o[k] = 0; }
-    StridedGeom gzq = gz;
-    plan.skip_ones_rows(q, gzq);
+    StridedGeom gzq = gz, gyq = gyo;
+    plan.skip_ones_rows(q, gzq, &gyq);
     plan.xr(d, q, S, S, plan.twM, plan.wr, stream);
     plan.sz(gzq, 0, S, S, nullptr, 1.f, plan.twZ, stream);
-    plan.sy(gyo, 2, S, S, spec2[v], 1.f, plan.twY, stream);
+    plan.sy(gyq, 2, S, S, spec2[v], 1.f, plan.twY, stream);
     plan.sz(gzo, 1, S, S, nullptr, 1.f, plan.twZ, stream);
     StoreParams st{T2 + nb * v};
     plan.xs(d, st, S, plan.twM, plan.wr, stream);

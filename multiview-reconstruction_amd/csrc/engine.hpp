@@ -55,7 +55,9 @@ struct GatherParams {
 // The quotient is formed on the output region widened by qlo / qhi voxels per side (the reach of the second
 // kernel, for domains whose neighbourhood lies inside the volume: blocks), within the volume; 1 elsewhere.
 // skipOnes: rows outside the quotient region (a row of ones, whose x spectrum is P0 at frequency 0 and 0
-// elsewhere) are not written; the z pass that reads the result puts them in itself (StridedGeom ones*).
+// elsewhere) are not written; the z pass that reads the result puts them in itself (StridedGeom ones*), and
+// the work-groups are dealt over the quotient rows only: grid rows (y, z) in [rowLo, rowLo + rowN) per axis
+// (k = 1, 2), which are exactly the rows the quotient region selects.
 struct RatioParams {
   const float* img;
   int N[3];
@@ -63,6 +65,7 @@ struct RatioParams {
   int qlo[3];
   int qhi[3];
   int skipOnes;
+  int rowLo[3], rowN[3];
 };
 
 struct UpdateParams {
@@ -104,10 +107,11 @@ struct StridedGeom {
   int plane0 = 0;
   int storeLo = 0, storeHi = 1 << 30;
   // Forward mode over data whose rows outside planes [onesPlaneLo, onesPlaneHi) x positions [onesLo, onesHi) are
-  // x spectra of rows of ones, never written: the pass reads none of them and uses onesDC at column 0, 0 at the
-  // other columns.  Off when onesDC is 0.
+  // spectra of rows (or planes) of ones, never written: the pass reads none of them and uses onesDC at column 0,
+  // 0 at the other columns; with onesDCPlane >= 0 onesDC is on that plane only, 0 on the others.  Off when onesDC
+  // is 0.
   float onesDC = 0.f;
-  int onesPlaneLo = 0, onesPlaneHi = 0, onesLo = 0, onesHi = 0;
+  int onesPlaneLo = 0, onesPlaneHi = 0, onesLo = 0, onesHi = 0, onesDCPlane = -1;
   // Forward modes over data that is a mirror extension along the pencil (mirN > 0): position n holds what
   // position mirM + mirror(n - mirM, mirN) holds, and only those positions were written; the loads read them.
   int mirM = 0, mirN = 0;

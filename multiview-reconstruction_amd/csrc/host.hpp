@@ -52,10 +52,12 @@ struct Plan {
   // grid rows (y, z) in [m - lo, m + D + hi) per axis (within the grid).  y_conv then stores only those y, and
   // z_inv transforms only those y planes and stores only those z.  Batched geometries stay batched.
   void restrict_rows(const int lo[3], const int hi[3], StridedGeom& gy, StridedGeom& gz) const;
-  // Sets q.skipOnes and the z geometry gz of the forward pass that reads the quotient pass's result: the rows
-  // outside the quotient region are rows of ones (not written by the quotient pass, put in by the z pass).
-  // Not for batched blocks, whose quotient regions differ per block.
-  void skip_ones_rows(RatioParams& q, StridedGeom& gz) const;
+  // Sets q.skipOnes, the quotient rows q.rowLo / rowN, and the z geometry gz of the forward pass that reads the
+  // quotient pass's result: the rows outside the quotient region are rows of ones (not written by the quotient
+  // pass, put in by the z pass).  With gy (the y convolution after that z pass) the z pass also leaves out the
+  // y planes outside the quotient region, planes of ones whose spectrum, P0 P2 at x and z frequency 0, the y
+  // convolution puts in.  Not for batched blocks, whose quotient regions differ per block.
+  void skip_ones_rows(RatioParams& q, StridedGeom& gz, StridedGeom* gy = nullptr) const;
   // For a forward transform of a mirror gather (g.mode MIRROR): sets g.uniqueRows, so that x_fwd transforms only
   // the rows that are not mirror copies, and makes the z pass after it (gzf) transform only those y planes and
   // read z through the mirror, and the y convolution after that (gyc) read y through it.  Not for batched

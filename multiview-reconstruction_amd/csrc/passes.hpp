@@ -219,9 +219,10 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W, MODE>())) void
     // ---- forward step 1: thread q = b < B
     if (q < B) {
       if (gm.onesDC != 0.f) {
-        // rows of ones outside the written region: their x spectrum is known and not read
+        // rows of ones outside the written region: their spectrum is known and not read
         const bool planeIn = plane >= gm.onesPlaneLo && plane < gm.onesPlaneHi;
-        const float2 one = make_float2(col0 + w == 0 ? gm.onesDC : 0.f, 0.f);
+        const bool dc = col0 + w == 0 && (gm.onesDCPlane < 0 || plane == gm.onesDCPlane);
+        const float2 one = make_float2(dc ? gm.onesDC : 0.f, 0.f);
 #pragma unroll
         for (int a = 0; a < A; ++a) {
           const int n = q + B * a;
@@ -367,6 +368,13 @@ void launch_strided2_mode(const StridedGeom& gm, const float2* in, float2* out, 
                           const float2* tw, hipStream_t s) {
   constexpr int NT = W * split2(L).A;
   const dim3 g2((gm.ncols + W - 1) / W, gm.nplanes);
+  if (gm.ncols <= 0 || gm.nplanes <= 0) {  // nothing to transform (an empty row range)
+    if (gm.evStart) {
+      (void)hipEventRecord(gm.evStart, s);
+      (void)hipEventRecord(gm.evStop, s);
+    }
+    return;
+  }
   if (gm.evStart)  // timed: the events hold the kernel's own start and stop
     hipExtLaunchKernelGGL((k_strided2<L, W, MODE>), g2, dim3(NT), 0, s, gm.evStart, gm.evStop, 0u, gm, in, out, spec,
                           scale, tw);

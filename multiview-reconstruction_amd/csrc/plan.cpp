@@ -109,7 +109,7 @@ void Plan::restrict_rows(const int lo[3], const int hi[3], StridedGeom& gy, Stri
   gz.storeHi = b[2];
 }
 
-void Plan::skip_ones_rows(RatioParams& q, StridedGeom& gz) const {
+void Plan::skip_ones_rows(RatioParams& q, StridedGeom& gz, StridedGeom* gy) const {
   // quotient rows (y, z) in grid coordinates, as the quotient pass selects them
   int lo[3], hi[3];
   for (int k = 1; k < 3; ++k) {
@@ -117,11 +117,25 @@ void Plan::skip_ones_rows(RatioParams& q, StridedGeom& gz) const {
     hi[k] = std::min(dom.m[k] + dom.D[k] + q.qhi[k], dom.m[k] - q.o[k] + q.N[k]);
   }
   q.skipOnes = 1;
+  for (int k = 1; k < 3; ++k) {
+    q.rowLo[k] = lo[k];
+    q.rowN[k] = std::max(0, hi[k] - lo[k]);
+  }
   gz.onesDC = (float)dom.P[0];  // the x transform of a row of ones
   gz.onesPlaneLo = lo[1];
   gz.onesPlaneHi = hi[1];
   gz.onesLo = lo[2];
   gz.onesHi = hi[2];
+  if (gy) {
+    gz.plane0 = lo[1];
+    gz.nplanes = std::max(0, hi[1] - lo[1]);
+    gy->onesDC = (float)dom.P[0] * (float)dom.P[2];  // the xz transform of a plane of ones
+    gy->onesDCPlane = 0;
+    gy->onesPlaneLo = 0;
+    gy->onesPlaneHi = dom.P[2];
+    gy->onesLo = lo[1];
+    gy->onesHi = hi[1];
+  }
 }
 
 void Plan::mirror_rows(GatherParams& g, StridedGeom& gzf, StridedGeom& gyc) const {

@@ -400,9 +400,13 @@ __global__ __launch_bounds__(XCfg<M>::NT, XCfg<M>::MINW_RATIO) void k_x2_ratio(D
   }
   const int row0 = wg * C::RW;
   if (threadIdx.x < C::RW) {
-    const int row = row0 + threadIdx.x;
+    // skipOnes: the work-groups cover the quotient rows only, index i -> grid row (rowLo1 + i % rowN1,
+    // rowLo2 + i / rowN1)
+    int row = row0 + threadIdx.x;
+    const bool live = row < (q_.skipOnes ? q_.rowN[1] * q_.rowN[2] : nrows);
+    if (q_.skipOnes && live) row = (q_.rowLo[2] + row / q_.rowN[1]) * d.P[1] + q_.rowLo[1] + row % q_.rowN[1];
     long long ir = kOuterRow;
-    if (row < nrows) {
+    if (live) {
       // quotient rows: the output region widened by qlo / qhi, inside the volume
       const int ly = row % d.P[1] - d.m[1], lz = row / d.P[1] - d.m[2];
       if (ly >= -q_.qlo[1] && ly < D[1] + q_.qhi[1] && lz >= -q_.qlo[2] && lz < D[2] + q_.qhi[2] &&
@@ -410,16 +414,9 @@ __global__ __launch_bounds__(XCfg<M>::NT, XCfg<M>::MINW_RATIO) void k_x2_ratio(D
         ir = ((long long)(o[2] + lz) * q_.N[1] + (o[1] + ly)) * q_.N[0] + o[0] - d.m[0];
     }
     imgRow[threadIdx.x] = ir;
-    outOff[threadIdx.x] = row < nrows ? (size_t)row * d.HxP : kNoRow;
+    outOff[threadIdx.x] = live ? (size_t)row * d.HxP : kNoRow;
   }
   __syncthreads();
-  if (q_.skipOnes) {
-    // a work-group whose rows are all rows of ones has nothing to write (uniform: every thread reads the table)
-    bool any = false;
-#pragma unroll
-    for (int i = 0; i < C::RW; ++i) any |= imgRow[i] != kOuterRow;
-    if (!any) return;
-  }
   const int r = threadIdx.x / C::T, q = threadIdx.x % C::T;
   float2* rb = buf + r * C::RS;
   const long long ir = imgRow[r];
@@ -689,7 +686,8 @@ void launch_x_ratio(const Dom& d, const RatioParams& q, const float2* Sin, float
   static_assert(x2_ok<M>(), "every compiled x length has a two-step split");
   using C = XCfg<M>;
   Dom dd = d;
-  const int rows = d.P[1] * d.P[2];
+  const int rows = q.skipOnes ? q.rowN[1] * q.rowN[2] : d.P[1] * d.P[2];
+  if (rows == 0) return;  // no quotient rows: every row is a row of ones, put in by the z pass
   dd.wgPerBlock = (rows + C::RW - 1) / C::RW;
   const int nb = d.btab ? d.nb : 1;
   hipLaunchKernelGGL(k_x2_ratio<M>, dim3(dd.wgPerBlock * nb), dim3(C::NT), 0, s, dd, q, Sin, Sout, twM, wr);
