@@ -39,6 +39,7 @@ template int launch_x_update<{M}>(const Dom&, const UpdateParams&, const float2*
 template void launch_x_store<{M}>(const Dom&, const StoreParams&, const float2*, const float2*, const float2*, hipStream_t);
 """
 S_FUNCS = """template void launch_strided<{L}>(const StridedGeom&, int, const float2*, float2*, const float2*, float, const float2*, hipStream_t);
+template int strided_span<{L}>();
 """
 
 
@@ -62,13 +63,16 @@ def generate():
          "template <int M> void launch_x_ratio(const Dom&, const RatioParams&, const float2*, float2*, const float2*, const float2*, hipStream_t);",
          "template <int M> int launch_x_update(const Dom&, const UpdateParams&, const float2*, const float2*, const float2*, hipStream_t);",
          "template <int M> void launch_x_store(const Dom&, const StoreParams&, const float2*, const float2*, const float2*, hipStream_t);",
-         "template <int L> void launch_strided(const StridedGeom&, int, const float2*, float2*, const float2*, float, const float2*, hipStream_t);"]
+         "template <int L> void launch_strided(const StridedGeom&, int, const float2*, float2*, const float2*, float, const float2*, hipStream_t);",
+         "template <int L> int strided_span();"]
     for name, fn, typ in [("get_x_fwd", "launch_x_fwd", "XFwdFn"), ("get_x_ratio", "launch_x_ratio", "XRatioFn"),
                           ("get_x_update", "launch_x_update", "XUpdateFn"), ("get_x_store", "launch_x_store", "XStoreFn")]:
         cases = " ".join(f"case {m}: return &{fn}<{m}>;" for m in HALF)
         d.append(f"{typ} {name}(int M) {{ switch (M) {{ {cases} default: return nullptr; }} }}")
     cases = " ".join(f"case {l}: return &launch_strided<{l}>;" for l in STRIDED)
     d.append(f"StridedFn get_strided(int L) {{ switch (L) {{ {cases} default: return nullptr; }} }}")
+    cases = " ".join(f"case {l}: return strided_span<{l}>();" for l in STRIDED)
+    d.append(f"int get_strided_span(int L) {{ switch (L) {{ {cases} default: return 0; }} }}")
     d.append("static const int kStrided[] = {" + ", ".join(map(str, STRIDED)) + ", 0};")
     d.append("static const int kHalf[] = {" + ", ".join(map(str, HALF)) + ", 0};")
     d.append("const int* strided_lengths() { return kStrided; }")

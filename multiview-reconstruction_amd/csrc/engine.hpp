@@ -113,7 +113,8 @@ struct StridedGeom {
   float onesDC = 0.f;
   int onesPlaneLo = 0, onesPlaneHi = 0, onesLo = 0, onesHi = 0, onesDCPlane = -1;
   // Forward modes over data that is a mirror extension along the pencil (mirN > 0): position n holds what
-  // position mirM + mirror(n - mirM, mirN) holds, and only those positions were written; the loads read them.
+  // position mirM + mirror(n - mirM, mirN) holds, and only those positions were written; the loads read them
+  // (byte offsets from the plane up to L lineStride 8 must fit 32 bits).
   int mirM = 0, mirN = 0;
   // timing: when set, the launch records the kernel's own start and stop in these events
   hipEvent_t evStart = nullptr, evStop = nullptr;
@@ -128,6 +129,8 @@ XRatioFn get_x_ratio(int M);
 XUpdateFn get_x_update(int M);
 XStoreFn get_x_store(int M);
 StridedFn get_strided(int L);
+// the largest pencil position a lane of the length-L strided pass addresses with its 32-bit offset, plus one
+int get_strided_span(int L);
 // lengths with compiled kernels, ascending; terminated by 0
 const int* strided_lengths();
 const int* half_lengths();

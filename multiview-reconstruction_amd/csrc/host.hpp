@@ -61,7 +61,7 @@ struct Plan {
   // For a forward transform of a mirror gather (g.mode MIRROR): sets g.uniqueRows, so that x_fwd transforms only
   // the rows that are not mirror copies, and makes the z pass after it (gzf) transform only those y planes and
   // read z through the mirror, and the y convolution after that (gyc) read y through it.  Not for batched
-  // blocks, whose gathers mirror the whole volume, not the block.
+  // blocks, whose gathers mirror the whole volume, not the block; a no-op for spectra of 4 GiB or more.
   void mirror_rows(GatherParams& g, StridedGeom& gzf, StridedGeom& gyc) const;
   float inv_scale() const { return 2.0f / ((float)dom.P[0] * (float)dom.P[1] * (float)dom.P[2]); }
 

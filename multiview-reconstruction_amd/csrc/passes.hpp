@@ -179,6 +179,35 @@ __device__ __forceinline__ void s2_inverse_exchange(float2* tile, float2* u, flo
   }
 }
 
+// Global accesses of a tile: a base that is uniform over the work-group (the tile's first column on its plane,
+// plus a multiple of the pencil stride, formed in scalar registers) and the lane's 32-bit byte offset (its column
+// and pencil position q).  They go through a raw buffer resource on the uniform base (4 GiB range, gfx9 format
+// dword 0x00020000), so that no per-access 64-bit address is formed in vector registers: the memory
+// instruction adds the lane's offset itself.
+constexpr int kBufferFormatGfx9 = 0x00020000;
+__device__ __forceinline__ float2 s2_ld(const float2* base, unsigned off) {
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(base), 0, 0xffffffff, kBufferFormatGfx9);
+  typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+  float2 x;
+  __builtin_memcpy(&x, &v, sizeof(x));
+  return x;
+}
+__device__ __forceinline__ void s2_st(float2* base, unsigned off, float2 x) {
+  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(base, 0, 0xffffffff, kBufferFormatGfx9);
+  typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+  u32x2 v;
+  __builtin_memcpy(&v, &x, sizeof(v));
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, 0);
+}
+// the largest pencil position q a lane's offset covers, plus one (the host checks that the offsets fit 32 bits)
+template <int L>
+int strided_span() {
+  constexpr Split2 S = split2(L);
+  return (S.A > S.B ? S.A : S.B) + 1;
+}
+
 template <int L, int W, int MODE>
 __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W, MODE>())) void k_strided2(
     StridedGeom gm, const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ spec,
@@ -211,9 +240,12 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W, MODE>())) void
     blockOff = (size_t)b * gm.blockStride;
   }
   plane += gm.plane0;
-  const float2* src = in + blockOff + (size_t)plane * gm.planeStride + col0 + w;
-  float2* dst = out + blockOff + (size_t)plane * gm.planeStride + col0 + w;
+  // Lanes of columns beyond ncols (the last tile) read the row pitch's padding (pitch = 16-column multiple) and
+  // compute values that are never stored.
   const size_t ls = gm.lineStride;
+  const float2* sb = in + blockOff + (size_t)plane * gm.planeStride + col0;
+  float2* db = out + blockOff + (size_t)plane * gm.planeStride + col0;
+  const unsigned lo = (unsigned)((w + (size_t)q * ls) * sizeof(float2));
   float2 v[A];
   if constexpr (MODE != 1) {
     // ---- forward step 1: thread q = b < B
@@ -226,30 +258,33 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W, MODE>())) void
 #pragma unroll
         for (int a = 0; a < A; ++a) {
           const int n = q + B * a;
-          const bool rd = colOk && planeIn && n >= gm.onesLo && n < gm.onesHi;
-          v[a] = rd ? src[(size_t)n * ls] : (colOk ? one : make_float2(0.f, 0.f));
+          const bool rd = planeIn && n >= gm.onesLo && n < gm.onesHi;
+          v[a] = rd ? s2_ld(sb + (size_t)(B * a) * ls, lo) : one;
         }
       } else if (gm.mirN > 0) {
         // a mirror extension along the pencil: the copies are read from their sources.  One reflection at
         // each face suffices when the pencil reaches less than mirN - 1 beyond either face (the usual case).
+        // The positions differ per lane: 32-bit byte offsets from the plane (the host enables this mode only
+        // where the pencil's span fits them).
+        const unsigned w8 = (unsigned)(w * sizeof(float2)), ls8 = (unsigned)(ls * sizeof(float2));
         if (gm.mirM < gm.mirN && L - gm.mirM <= 2 * gm.mirN - 1) {
 #pragma unroll
           for (int a = 0; a < A; ++a) {
             int n = q + B * a - gm.mirM;
             n = n < 0 ? -n : n;
             n = n >= gm.mirN ? 2 * gm.mirN - 2 - n : n;
-            v[a] = colOk ? src[(size_t)(gm.mirM + n) * ls] : make_float2(0.f, 0.f);
+            v[a] = s2_ld(sb, w8 + (unsigned)(gm.mirM + n) * ls8);
           }
         } else {
 #pragma unroll
           for (int a = 0; a < A; ++a) {
             const int n = mirror_index(q + B * a - gm.mirM, gm.mirN);
-            v[a] = colOk ? src[(size_t)(gm.mirM + n) * ls] : make_float2(0.f, 0.f);
+            v[a] = s2_ld(sb, w8 + (unsigned)(gm.mirM + n) * ls8);
           }
         }
       } else {
 #pragma unroll
-        for (int a = 0; a < A; ++a) v[a] = colOk ? src[(size_t)(q + B * a) * ls] : make_float2(0.f, 0.f);
+        for (int a = 0; a < A; ++a) v[a] = s2_ld(sb + (size_t)(B * a) * ls, lo);
       }
     }
     __syncthreads();  // twiddle table
@@ -270,9 +305,9 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W, MODE>())) void
     // so that their latency overlaps it
     float2 kv[MODE == 2 ? B : 1];
     if constexpr (MODE == 2) {
-      const float2* sp = spec + (size_t)plane * gm.planeStride + col0 + w;
+      const float2* spb = spec + (size_t)plane * gm.planeStride + col0;
 #pragma unroll
-      for (int k2 = 0; k2 < B; ++k2) kv[k2] = colOk ? sp[(size_t)(q + A * k2) * ls] : make_float2(0.f, 0.f);
+      for (int k2 = 0; k2 < B; ++k2) kv[k2] = s2_ld(spb + (size_t)(A * k2) * ls, lo);
     }
     // ---- exchange (round r: the values of k1 in [r AH, r AH + AH)), then forward step 2: thread q = k1 < A
     float2 u[B];
@@ -304,7 +339,7 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W, MODE>())) void
         for (int k2 = 0; k2 < B; ++k2) {
           float2 x = u[k2];
           if constexpr (MODE == 3) x = cscale(x, scale);
-          dst[(size_t)(q + A * k2) * ls] = x;
+          s2_st(db + (size_t)(A * k2) * ls, lo, x);
         }
       }
       return;
@@ -328,7 +363,7 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W, MODE>())) void
     // ---- inverse only: thread q = k1 < A holds X[q + A k2]
     float2 u[B];
 #pragma unroll
-    for (int k2 = 0; k2 < B; ++k2) u[k2] = colOk ? src[(size_t)(q + A * k2) * ls] : make_float2(0.f, 0.f);
+    for (int k2 = 0; k2 < B; ++k2) u[k2] = s2_ld(sb + (size_t)(A * k2) * ls, lo);
     __syncthreads();  // twiddle table
     rdft<B, 1>(u);
     if constexpr (R == 1) {
@@ -355,7 +390,7 @@ __global__ __launch_bounds__(W * split2(L).A, (s2_min_waves<L, W, MODE>())) void
 #pragma unroll
         for (int n1 = 0; n1 < A; ++n1) {
           const int n = q + B * n1;
-          if (n >= gm.storeLo && n < gm.storeHi) dst[(size_t)n * ls] = v[n1];
+          if (n >= gm.storeLo && n < gm.storeHi) s2_st(db + (size_t)(B * n1) * ls, lo, v[n1]);
         }
       }
     }

@@ -65,6 +65,12 @@ int Plan::create_geometry(const int D[3], const int Kd[3], const int* Pmin) {
   sy = get_strided(dom.P[1]);
   sz = get_strided(dom.P[2]);
   if (!xf || !xr || !xu || !xs || !sy || !sz) { set_error("plan: missing kernel instantiation"); return -1; }
+  // the strided passes address a lane's column and pencil position with a 32-bit byte offset from the plane
+  if (((size_t)get_strided_span(dom.P[1]) * dom.HxP + 16) * sizeof(float2) > 0xffffffffull ||
+      ((size_t)get_strided_span(dom.P[2]) * dom.P[1] * dom.HxP + 16) * sizeof(float2) > 0xffffffffull) {
+    set_error("plan: xy plane of the FFT grid too large for 32-bit lane offsets");
+    return -1;
+  }
   return 0;
 }
 
@@ -139,6 +145,9 @@ void Plan::skip_ones_rows(RatioParams& q, StridedGeom& gz, StridedGeom* gy) cons
 }
 
 void Plan::mirror_rows(GatherParams& g, StridedGeom& gzf, StridedGeom& gyc) const {
+  // the strided passes read the mirror copies with 32-bit byte offsets from the plane: the z pencil's span, the
+  // whole spectrum, must fit them (otherwise every row is transformed and read as written)
+  if ((ncpx() + 16) * sizeof(float2) > 0xffffffffull) return;
   g.uniqueRows = 1;
   // unique rows: FFT positions [-org, -org + N) per axis, inside the grid
   gzf.plane0 = -g.org[1];
