@@ -226,8 +226,10 @@ __global__ __launch_bounds__(kThreads) void k_update_mul(float* __restrict__ psi
   // Java rounds every float and double operation: no FMA contraction in the update
 #pragma clang fp contract(off)
   const size_t nvox = (size_t)D.x * D.y * D.z;
-  double sumc = 0.0, mxc = -1.0;
+  double sumc = 0.0;
+  float mxc = -1.f;
   const double invV = 1.0 / (double)V;
+  const RLConsts rc = rl_consts(lambda, minI, maxI);
   for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < nvox; i += (size_t)gridDim.x * kThreads) {
     const int lx = (int)(i % D.x);
     const size_t r = i / D.x;
@@ -241,12 +243,12 @@ __global__ __launch_bounds__(kThreads) void k_update_mul(float* __restrict__ psi
     prod = pow(prod, invV);
     sumW = fmin(1.0, sumW);
     const float last = psi[gi];
-    const float next = rl_tikhonov_adjust(last * (float)prod, lambda, minI, maxI);
+    const float next = rl_tikhonov_adjust(last * (float)prod, rc);
     const float res = last + (next - last) * (float)sumW;
     psi[gi] = res;
     const float ch = res - last;
     sumc += (double)ch;
-    mxc = java_max(mxc, (double)ch);
+    mxc = java_maxf(mxc, ch);
   }
   block_reduce_stats(sumc, mxc, partial + blockIdx.x);
 }
@@ -339,14 +341,16 @@ void launch_quotient(const float* img, float* blurred, size_t n, hipStream_t s) 
 __global__ __launch_bounds__(kThreads) void k_final_values(float* __restrict__ psi, const float* __restrict__ integral,
                                                            const float* __restrict__ weight, size_t n, double lambda,
                                                            float minI, float maxI, double2* __restrict__ partial) {
-  double sumc = 0.0, mxc = -1.0;
+  double sumc = 0.0;
+  float mxc = -1.f;
+  const RLConsts rc = rl_consts(lambda, minI, maxI);
   for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
     const float last = psi[i];
-    const float next = rl_next_value(last, integral[i], weight[i], lambda, minI, maxI);
+    const float next = rl_next_value(last, integral[i], weight[i], rc);
     psi[i] = next;
     const float ch = next - last;
     sumc += (double)ch;
-    mxc = java_max(mxc, (double)ch);
+    mxc = java_maxf(mxc, ch);
   }
   block_reduce_stats(sumc, mxc, partial + blockIdx.x);
 }

@@ -50,34 +50,77 @@ __device__ __forceinline__ float gather_value(const GatherParams& g, const Dom& 
 // ------------------------------------------------------------------ update of psi
 // DeconvolutionMethods.computeNextValue (D/iteration/sequential/DeconvolutionMethods.java:320-358),
 // float operations in Java order; Tikhonov in double.  Java rounds every float and double operation, so the
-// function body forbids FMA contraction (also where it is inlined).  Plain operators are used on purpose: the
-// __f*_rn helpers are header functions outside this pragma's scope, whose products the compiler may still fuse.
-__device__ __forceinline__ float rl_tikhonov_adjust(float value, double lambda, float minI, float maxI) {
+// function bodies forbid FMA contraction (also where they are inlined), and every result below is the one the
+// corresponding Java operation rounds to:
+//   * value / maxI (float): the double product value * RN(1/maxI) is within 2^-52 (relative) of the quotient,
+//     and a quotient of two floats is never that close to a float rounding midpoint without being on it (which
+//     it cannot be), so rounding the product to float gives the correctly rounded float quotient;
+//   * sqrt(y), y >= 1: Markstein's iteration from the hardware reciprocal square root with two residual
+//     corrections, the correctly rounded double square root (the sequence of the device library's sqrt, whose
+//     operand scaling is an identity for y >= 1; y = inf, from an overflowing value / maxI, is the one special
+//     value it can meet, and is passed through as the library does);
+//   * a / lambda: the quotient estimate a * R and one residual correction, R the reciprocal of lambda refined by
+//     two Newton steps (the device division's sequence, whose operand scaling is an identity for the normal a
+//     and lambda here).
+// The per-launch constants come from rl_consts.
+struct RLConsts {
+  double lambda, twoLambda, rLambda, rMax;
+  float minI, maxI;
+};
+
+__device__ __forceinline__ RLConsts rl_consts(double lambda, float minI, float maxI) {
+  RLConsts c;
+  c.lambda = lambda;
+  c.twoLambda = 2.0 * lambda;
+  c.minI = minI;
+  c.maxI = maxI;
+  c.rMax = 1.0 / (double)maxI;
+  double y = __builtin_amdgcn_rcp(lambda);
+  y = __builtin_fma(y, __builtin_fma(-lambda, y, 1.0), y);
+  y = __builtin_fma(y, __builtin_fma(-lambda, y, 1.0), y);
+  c.rLambda = y;
+  return c;
+}
+
+__device__ __forceinline__ float rl_tikhonov_adjust(float value, const RLConsts& c) {
 #pragma clang fp contract(off)
   float adjusted;
   if (value > 0.f) {
-    if (lambda > 0.0) {
-      const double vn = (double)(value / maxI);
-      const double t = (__dsqrt_rn(1.0 + (2.0 * lambda) * vn) - 1.0) / lambda;
-      adjusted = (float)t * maxI;
+    if (c.lambda > 0.0) {
+      const double vn = (double)(float)((double)value * c.rMax);  // (double)(value / maxI)
+      const double y = 1.0 + c.twoLambda * vn;                     // >= 1
+      const double r = __builtin_amdgcn_rsq(y);
+      double g = y * r, h = 0.5 * r;
+      const double e = __builtin_fma(-h, g, 0.5);
+      g = __builtin_fma(g, e, g);
+      h = __builtin_fma(h, e, h);
+      g = __builtin_fma(__builtin_fma(-g, g, y), h, g);
+      g = __builtin_fma(__builtin_fma(-g, g, y), h, g);  // sqrt(y)
+      const double a = g - 1.0;
+      const double q0 = a * c.rLambda;
+      double t = __builtin_fma(__builtin_fma(-c.lambda, q0, a), c.rLambda, q0);  // a / lambda
+      if (y == __builtin_inf()) t = y;  // value / maxI overflowed: sqrt and quotient are infinite
+      adjusted = (float)t * c.maxI;
     } else {
       adjusted = value;
     }
   } else {
-    adjusted = minI;
+    adjusted = c.minI;
   }
-  return isnan(adjusted) ? minI : fmaxf(minI, adjusted);
+  return adjusted > c.minI ? adjusted : c.minI;  // Math.max(minI, adjusted), NaN -> minI
 }
 
-__device__ __forceinline__ float rl_next_value(float last, float integral, float weight, double lambda, float minI,
-                                               float maxI) {
+__device__ __forceinline__ float rl_next_value(float last, float integral, float weight, const RLConsts& c) {
 #pragma clang fp contract(off)
-  const float next = rl_tikhonov_adjust(last * integral, lambda, minI, maxI);
+  const float next = rl_tikhonov_adjust(last * integral, c);
   return last + (next - last) * weight;
 }
 
 // Math.max of Java on doubles for the maxChange statistic: a NaN change stays NaN (fmax would drop it)
 __device__ __forceinline__ double java_max(double a, double b) { return a != a ? a : b != b ? b : fmax(a, b); }
+// the same over float changes (their maximum is the same number in float; a NaN stays; of equal zeros the
+// first is kept)
+__device__ __forceinline__ float java_maxf(float m, float ch) { return (ch > m || ch != ch) ? ch : m; }
 
 __device__ __forceinline__ void block_reduce_stats(double s, double mx, double2* out) {
   __shared__ double rs[kThreads / 64], rm[kThreads / 64];

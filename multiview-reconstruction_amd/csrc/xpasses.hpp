@@ -543,7 +543,9 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
   const size_t so = rowOff[r];
   float2 z[C::A];
   x2_inverse_half_regs<M>(so != kNoRow ? S + so : nullptr, rb, tw, twr, q, z);
-  double sum = 0.0, mx = -1.0;
+  double sum = 0.0;
+  float mx = -1.f;
+  const RLConsts rc = rl_consts(u.lambda, u.minValue, u.maxI);
   const int DX = D[0];
   if constexpr (C::long_rows) {
     // long rows: lane q < B holds the values z[a] = (x[2n], x[2n + 1]), n = q + B a, of its row and updates psi
@@ -570,11 +572,11 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
         for (int j = 0; j < 2 * H; ++j) {
           if (lx[j] < 0) continue;
           const int a = min(a0 + (j >> 1), C::A - 1);
-          const float next = rl_next_value(last[j], (j & 1) ? z[a].y : z[a].x, wt[j], u.lambda, u.minValue, u.maxI);
+          const float next = rl_next_value(last[j], (j & 1) ? z[a].y : z[a].x, wt[j], rc);
           psiRowPtr[lx[j]] = next;
           const float ch = next - last[j];
           sum += (double)ch;
-          mx = java_max(mx, (double)ch);
+          mx = java_maxf(mx, ch);
         }
       }
     }
@@ -606,12 +608,11 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
         if (!ok[j]) continue;
         const unsigned idx = base + j * C::NT;
         const int lx = (int)(idx - rr[j] * (unsigned)DX);
-        const float next = rl_next_value(last[j], fb[rr[j] * 2 * C::RS + d.m[0] + lx], wt[j], u.lambda, u.minValue,
-                                         u.maxI);
+        const float next = rl_next_value(last[j], fb[rr[j] * 2 * C::RS + d.m[0] + lx], wt[j], rc);
         u.psi[gi[j]] = next;
         const float ch = next - last[j];
         sum += (double)ch;
-        mx = java_max(mx, (double)ch);
+        mx = java_maxf(mx, ch);
       }
     }
   }
