@@ -428,31 +428,25 @@ __global__ __launch_bounds__(XCfg<M>::NT, XCfg<M>::MINW_RATIO) void k_x2_ratio(D
   x2_inverse_half_regs<M>(ir != kOuterRow ? Sin + so : nullptr, rb, tw, twr, q, z);
   float2 zf[C::B];
   if constexpr (!C::long_rows) {
-    // short rows (only B of T lanes hold values): park the blurred rows in LDS, form the quotient with a flat
-    // mapping over every thread, and run the forward transform from LDS (at M = 150 9 % faster than the
-    // quotient in registers)
+    // short rows (only B of T lanes hold values): park the blurred rows in LDS, form the quotient there with the
+    // row's T lanes over positions x = q + T i, all of a lane's image values in flight at once (offsets from the
+    // lane's row pointers known at compile time; at M = 150 4 % faster than a flat mapping over the work-group,
+    // 9 % faster than a wave walking its rows one by one), and run the forward transform from LDS
     x2_park_inverse<M>(rb, q, z);
-    float* fb = reinterpret_cast<float*>(buf);
-    constexpr int TOT = C::RW * C::PX;
-    constexpr int IT = (TOT + C::NT - 1) / C::NT;
-    float im[IT];
+    float* fr = reinterpret_cast<float*>(rb);
+    constexpr int NI = (C::PX + C::T - 1) / C::T;
+    const bool rowIn = ir != kOuterRow;
+    const float* imr = q_.img + (rowIn ? ir : 0);  // image value of FFT position x: imr[x], x in [x0f, x1f)
+    float im[NI];
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int idx = threadIdx.x + it * C::NT;
-      const int rr = min(idx / C::PX, C::RW - 1), x = idx - rr * C::PX;
-      const long long irr = imgRow[rr];
-      const bool ok = idx < TOT && irr != kOuterRow && x >= x0f && x < x1f;
-      im[it] = q_.img[ok ? irr + x : 0];
-      if (!ok) im[it] = 0.f;
+    for (int i = 0; i < NI; ++i) {
+      const int x = q + C::T * i;
+      im[i] = rowIn && x >= x0f && x < x1f ? imr[x] : 0.f;
     }
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int idx = threadIdx.x + it * C::NT;
-      if (idx < TOT) {
-        const int rr = idx / C::PX, x = idx - rr * C::PX;
-        float* p = fb + rr * 2 * C::RS + x;
-        *p = im[it] > 0.f ? im[it] / *p : 1.f;
-      }
+    for (int i = 0; i < NI; ++i) {
+      const int x = q + C::T * i;
+      if (x < C::PX) fr[x] = im[i] > 0.f ? im[i] / fr[x] : 1.f;
     }
     __syncthreads();
     x2_forward<M>(rb, tw, q, zf);
@@ -582,7 +576,9 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
     }
   } else {
     // short rows (only B of T lanes hold values): park the rows in LDS and update them with a flat mapping over
-    // every thread, six psi / weight pairs in flight (at M = 150 35 % faster than the register update)
+    // every thread, six psi / weight pairs in flight (at M = 150 35 % faster than the register update; the
+    // row's lanes over positions q + T j with 8 pairs in flight were 8 % slower, a wave walking its rows one by
+    // one 12-37 % slower, fewer instructions notwithstanding: the loop is bound by its rounds of loads)
     x2_park_inverse<M>(rb, q, z);
     const float* fb = reinterpret_cast<const float*>(buf);
     const unsigned total = (unsigned)(C::RW * DX);
