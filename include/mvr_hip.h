@@ -196,6 +196,10 @@ int mvr_decon_get_geometry(mvr_decon* d, int out[4]);
  * the launch.  mask bit c enables category c (0 x_fwd, 1 z_fwd, 2 y_conv, 3 z_inv, 4 x_ratio, 5 x_update,
  * 6 x_store, 7 update_mul, 8 reduce); 0 disables.  Totals are cleared by every call. */
 int mvr_decon_profile(mvr_decon* d, unsigned mask);
+/* the same, timing only the first launch of every group of `every` launches of each enabled category (every
+ * >= 1): the events of a timed launch delay the stream by a few microseconds, so a throughput measured over
+ * the same run is disturbed less; launches[c] counts the timed launches */
+int mvr_decon_profile_sampled(mvr_decon* d, unsigned mask, int every);
 int mvr_decon_pass_times(mvr_decon* d, double msTotal[9], long long launches[9]);
 void mvr_decon_destroy(mvr_decon* d);
 

@@ -347,6 +347,13 @@ int mvr_decon_profile(mvr_decon* d, unsigned mask) {
   return 0;
 }
 
+int mvr_decon_profile_sampled(mvr_decon* d, unsigned mask, int every) {
+  NULLCHECK(d);
+  if (every < 1) { set_error("mvr_decon_profile_sampled: every must be >= 1"); return -1; }
+  d->d.timer.reset(mask, every);
+  return 0;
+}
+
 int mvr_decon_pass_times(mvr_decon* d, double ms[9], long long n[9]) {
   NULLCHECK(d);
   return d->d.timer.collect(ms, n);

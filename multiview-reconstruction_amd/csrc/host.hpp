@@ -91,6 +91,8 @@ enum Pass { PASS_X_FWD = 0, PASS_Z_FWD, PASS_Y_CONV, PASS_Z_INV, PASS_X_RATIO, P
 struct Timer {
   bool on = false, pending = false;
   unsigned mask = 0;
+  int every = 1;                      // time one launch in `every` of each enabled category
+  long long seen[PASS_COUNT] = {0};   // launches of each category since the last reset
   std::vector<hipEvent_t> ev;
   std::vector<int> cat;
   size_t used = 0;
@@ -100,7 +102,9 @@ struct Timer {
   void kernel(int c, hipEvent_t* start, hipEvent_t* stop);
   int collect(double* ms, long long* n);  // accumulates into arrays of PASS_COUNT
   void release();
-  void reset(unsigned mask);  // enable the categories in mask (bit c = category c), clear totals
+  // enable the categories in mask (bit c = category c), timing the first launch of every group of `every`;
+  // clears the totals
+  void reset(unsigned mask, int every = 1);
   double ms[PASS_COUNT] = {0};
   long long cnt[PASS_COUNT] = {0};
 };

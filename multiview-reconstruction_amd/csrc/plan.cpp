@@ -189,7 +189,7 @@ int Plan::convolve(const GatherParams& g, const float2* spec, float2* work, floa
 
 // ------------------------------------------------------------------ timers
 void Timer::begin(hipStream_t s, int c) {
-  pending = on && ((mask >> c) & 1u);
+  pending = on && ((mask >> c) & 1u) && seen[c]++ % every == 0;
   if (!pending) return;
   if (used + 2 > ev.size()) {
     size_t old = ev.size();
@@ -202,7 +202,7 @@ void Timer::begin(hipStream_t s, int c) {
 }
 void Timer::kernel(int c, hipEvent_t* start, hipEvent_t* stop) {
   *start = *stop = nullptr;
-  if (!on || !((mask >> c) & 1u)) return;
+  if (!on || !((mask >> c) & 1u) || seen[c]++ % every != 0) return;
   if (used + 2 > ev.size()) {
     size_t old = ev.size();
     ev.resize(old + 256);
@@ -219,10 +219,11 @@ void Timer::end(hipStream_t s) {
   used++;
   pending = false;
 }
-void Timer::reset(unsigned m) {
+void Timer::reset(unsigned m, int e) {
   mask = m;
   on = m != 0;
-  for (int c = 0; c < PASS_COUNT; ++c) { ms[c] = 0; cnt[c] = 0; }
+  every = e < 1 ? 1 : e;
+  for (int c = 0; c < PASS_COUNT; ++c) { ms[c] = 0; cnt[c] = 0; seen[c] = 0; }
 }
 int Timer::collect(double* outMs, long long* outN) {
   if (used > 0) {

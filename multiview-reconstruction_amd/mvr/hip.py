@@ -94,6 +94,7 @@ def lib():
         L.mvr_decon_reset_psi.argtypes = [vp]
         L.mvr_decon_get_geometry.argtypes = [vp, _i32p]
         L.mvr_decon_profile.argtypes = [vp, ctypes.c_uint]
+        L.mvr_decon_profile_sampled.argtypes = [vp, ctypes.c_uint, ctypes.c_int]
         L.mvr_decon_pass_times.argtypes = [vp, _f64p, ctypes.POINTER(ctypes.c_longlong)]
         L.mvr_decon_destroy.argtypes = [vp]
         L.mvr_decon_destroy.restype = None
@@ -417,15 +418,19 @@ class Decon:
         _check(lib().mvr_decon_get_tile_widths(self.h, tw))
         return dict(P=(g[0], g[1], g[2]), row_pitch=g[3], tile_widths_yz=(tw[0], tw[1]))
 
-    def profile(self, passes=None):
-        """Time the given passes (names of PASSES; None = all, [] = off); clears the totals."""
+    def profile(self, passes=None, every=1):
+        """Time the given passes (names of PASSES; None = all, [] = off), the first launch of every group of
+        `every` launches of each; clears the totals."""
         if passes is None:
             mask = (1 << len(PASSES)) - 1
         else:
             mask = 0
             for p in passes:
                 mask |= 1 << PASSES.index(p)
-        _check(lib().mvr_decon_profile(self.h, mask))
+        if every == 1:
+            _check(lib().mvr_decon_profile(self.h, mask))
+        else:
+            _check(lib().mvr_decon_profile_sampled(self.h, mask, int(every)))
 
     def pass_times(self):
         ms = (ctypes.c_double * 9)()

@@ -434,6 +434,33 @@ def test_pass_timing_counts_and_leaves_results_unchanged(scheme):
     assert all(c == 0 for _, c in out[0][2].values())
 
 
+def test_sampled_pass_timing_counts_the_first_of_every_group():
+    # profile(passes, every=k) times the launches 0, k, 2k, ... of each enabled pass: ceil(n / k) of n launches;
+    # results are the same as untimed
+    inp = _small_problem(weights="zramp")
+    imgs, ws, ks = inp["imgs"], inp["weights"], inp["psfs"]
+    V, its = 3, 2
+    d = hip.Decon(imgs.shape[1:], V, "EFFICIENT_BAYESIAN", "SEQ", 6e-4)
+    for v in range(V):
+        d.set_view(v, imgs[v], ws[v], ks[v])
+    d.init()
+    d.run(its)
+    ref = d.psi()
+    for every in (1, 5, 7, 100):
+        d.reset_psi()
+        d.profile(["y_conv", "x_ratio"], every=every)
+        d.run(its)
+        t = d.pass_times()
+        n_y, n_r = 2 * V * its, V * its
+        assert t["y_conv"][1] == -(-n_y // every) and t["x_ratio"][1] == -(-n_r // every), (every, t)
+        assert t["y_conv"][0] > 0.0 and t["z_fwd"][1] == 0
+        np.testing.assert_array_equal(d.psi(), ref)
+    with pytest.raises(hip.MvrError):
+        d.profile(["y_conv"], every=0)
+    d.profile([])
+    d.close()
+
+
 @pytest.mark.parametrize("scheme", ["SEQ", "MUL"])
 def test_autotuned_lengths_keep_parity(oracle, scheme):
     # the Mul tuner times one view (V swapped to 1 during the trials), then rebuilds T2 and every view's spectra
