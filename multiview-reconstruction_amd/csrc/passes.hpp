@@ -6,6 +6,13 @@
 //   z_fwd  y(*K2^)  z_inv  x_update(C2R, Tikhonov update of psi, stats)
 // Every pass reads and writes its data once; the pointwise work of the
 // reference (quotient, update, spectrum multiply) is fused into the passes.
+// Each pass moves only the rows its neighbours use: the first convolution
+// transforms psi's own rows (its mirror copies are read from their sources),
+// the second does not store or read the quotient's rows and y planes of ones
+// (their spectra are known and put in by the pass that reads them), and the
+// inverse passes produce only the rows the next x pass reads (the StridedGeom
+// and RatioParams fields the plan sets per launch: Plan::restrict_rows,
+// skip_ones_rows, mirror_rows).
 //
 // x passes work on whole rows (real axis, half-length complex FFT with the
 // even/odd split); y and z passes work on tiles of W consecutive spectrum
