@@ -3,9 +3,11 @@
 //
 // rdft<N, DIR>(v) transforms N complex values held in registers in place
 // (natural-order output).  N is split recursively as N = R x M (R a base
-// radix 2, 3, 4, 5, 8); all twiddles of the split are compile-time constants
-// (computed by constexpr series in double, rounded to float), and the final
-// index permutation is a register renaming that the compiler resolves.
+// radix 2, 3, 4, 5, 8): by the prime-factor algorithm where R and M are
+// coprime (no twiddles between the steps), by Cooley-Tukey otherwise, whose
+// twiddles are compile-time constants (computed by constexpr series in double,
+// rounded to float); the index permutations are register renamings that the
+// compiler resolves.
 #pragma once
 #include "fft_core.hpp"
 
@@ -55,6 +57,19 @@ struct TwTab {
 };
 
 __host__ __device__ constexpr bool is_base_radix(int n) { return n == 2 || n == 3 || n == 4 || n == 5 || n == 8; }
+__host__ __device__ constexpr int cx_gcd(int a, int b) { return b == 0 ? a : cx_gcd(b, a % b); }
+// a base radix R with gcd(R, n / R) = 1 (n / R > 1), 0 if there is none
+__host__ __device__ constexpr int coprime_radix(int n) {
+  for (int R : {8, 4, 2, 5, 3})
+    if (n % R == 0 && n / R > 1 && cx_gcd(R, n / R) == 1) return R;
+  return 0;
+}
+// x with a x = 1 mod m (gcd(a, m) = 1, m > 1)
+__host__ __device__ constexpr int cx_modinv(int a, int m) {
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return 1;
+}
 __host__ __device__ constexpr int split_radix(int n) {
   return n % 4 == 0 ? 4 : n % 8 == 0 ? 8 : n % 2 == 0 ? 2 : n % 5 == 0 ? 5 : n % 3 == 0 ? 3 : 0;
 }
@@ -65,23 +80,51 @@ __host__ __device__ constexpr bool is_smooth(int n) {
   return n == 1;
 }
 
-template <int N, int DIR>
+// PF: coprime splits by the prime-factor algorithm (false: Cooley-Tukey throughout)
+template <int N, int DIR, bool PF = true>
 __device__ __forceinline__ void rdft(float2* v);
 
-template <int N, int DIR, bool BASE = is_base_radix(N)>
+template <int N, int DIR, bool PF, bool BASE = is_base_radix(N)>
 struct RDFT;
 
-template <int N, int DIR>
-struct RDFT<N, DIR, true> {
+template <int N, int DIR, bool PF>
+struct RDFT<N, DIR, PF, true> {
   __device__ __forceinline__ static void run(float2* v) { dft<N, DIR>(v); }
 };
 
-template <int N, int DIR>
-struct RDFT<N, DIR, false> {
-  static constexpr int R = split_radix(N);
+template <int N, int DIR, bool PF>
+struct RDFT<N, DIR, PF, false> {
+  // N = N1 x N2 with coprime factors (N1 a base radix): the prime-factor (Good-Thomas) algorithm, no twiddles
+  // between the steps.  Input x[(N2 n1 + N1 n2) mod N] (n1 < N1, n2 < N2); N1-point DFTs over n1, N2-point
+  // DFTs over n2; X[k1, k2] is X[(N2 a k1 + N1 b k2) mod N] with a = N2^-1 mod N1, b = N1^-1 mod N2, since then
+  // W_N^{n k} = W_N1^{n1 k1} W_N2^{n2 k2}.  Both index maps are register renamings.
+  static constexpr int P = PF ? coprime_radix(N) : 0;
+  // otherwise Cooley-Tukey N = R x M with inter-step twiddles
+  static constexpr int R = P > 0 ? P : split_radix(N);
   static constexpr int M = N / R;
   static_assert(R > 0 && M > 1, "register DFT length must be 2^a 3^b 5^c");
   __device__ __forceinline__ static void run(float2* v) {
+    if constexpr (P > 0) {
+      constexpr int N1 = P, N2 = N / P;
+      constexpr int a = cx_modinv(N2 % N1, N1), b = cx_modinv(N1 % N2, N2);
+      float2 t[N];
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) {
+        float2 u[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) u[n1] = v[(N2 * n1 + N1 * n2) % N];
+        dft<N1, DIR>(u);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) t[N2 * k1 + n2] = u[k1];
+      }
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) RDFT<N2, DIR, PF>::run(t + N2 * k1);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1)
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) v[(N2 * a * k1 + N1 * b * k2) % N] = t[N2 * k1 + k2];
+      return;
+    }
     constexpr TwTab<N, DIR> tw{};
     // step A: R-point DFTs over stride-M subsequences, twiddle W_N^{j k1}
 #pragma unroll
@@ -98,7 +141,7 @@ struct RDFT<N, DIR, false> {
     }
     // step B: M-point DFTs over the contiguous blocks [M k1, M k1 + M)
 #pragma unroll
-    for (int k1 = 0; k1 < R; ++k1) RDFT<M, DIR>::run(v + M * k1);
+    for (int k1 = 0; k1 < R; ++k1) RDFT<M, DIR, PF>::run(v + M * k1);
     // block position M k1 + k2 holds X[k1 + R k2]: rename into natural order
     float2 o[N];
 #pragma unroll
@@ -110,9 +153,9 @@ struct RDFT<N, DIR, false> {
   }
 };
 
-template <int N, int DIR>
+template <int N, int DIR, bool PF>
 __device__ __forceinline__ void rdft(float2* v) {
-  RDFT<N, DIR>::run(v);
+  RDFT<N, DIR, PF>::run(v);
 }
 
 // ------------------------------------------------------------------ two-step split of a pencil length
