@@ -207,6 +207,247 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a, int zp) {
   }
 }
 
+// Single-chunk layers with 16 output channels (the full-resolution 16 -> 16 layers) on large launches: the
+// work-group walks kConvZPDma planes and stages the next plane's halo into a second LDS buffer with LDS-DMA loads
+// (global_load_lds_dwordx4: 16 bytes per lane into LDS at the wave's base + 16 lane, no registers) while the
+// MFMAs of the current plane run.  One DMA instruction per 8-channel half writes 256 consecutive voxels of
+// that half, so the LDS layout is the same half-major [half][voxel][8] as above (halves padded to HVP voxels,
+// whole DMA rounds; the padding slots take the dummy loads and are never read).  Halo voxels outside the
+// volume load a valid dummy address and are zeroed after the DMA has landed.  The arithmetic is that of
+// k_conv_tile<TX, 1>: the results are bitwise equal.
+constexpr int kConvZPDma = 8;
+
+template <int TX>
+struct DmaTile {
+  static constexpr int TY = 256 / TX;
+  static constexpr int HX = TX + 2, HY = TY + 2;
+  static constexpr int HV = 3 * HY * HX;          // halo voxels
+  static constexpr int NR = (HV + 255) / 256;     // DMA rounds per half
+  static constexpr int HVP = NR * 256;            // halo slots per half
+  static constexpr int WC = 16;
+  // halo offset of tap t (t = 27: the zero-weight tap, whose A reads the centre tap 13)
+  __host__ __device__ static constexpr int shift(int t) {
+    const int tv = t < 27 ? t : 13;
+    return (tv / 9 * HY + tv / 3 % 3) * HX + tv % 3;
+  }
+  // the odd tap of pair p lies one of four offsets beyond the even one
+  static constexpr int DOFF[4] = {1, HX - 2, HX * HY - 2 * HX - 2, shift(27) - shift(26)};
+  __host__ __device__ static constexpr int dkind(int p) {
+    const int d = shift(2 * p + 1) - shift(2 * p);
+    return d == DOFF[0] ? 0 : d == DOFF[1] ? 1 : d == DOFF[2] ? 2 : d == DOFF[3] ? 3 : -1;
+  }
+  __host__ __device__ static constexpr bool pairs_ok() {
+    for (int p = 0; p < 14; ++p)
+      if (dkind(p) < 0) return false;
+    return true;
+  }
+};
+
+// per-lane state of the DMA staging: the lane's halo voxels v0 + 256 r (r < NR), their element offsets from the
+// plane before the output plane (the same for every plane), which are inside the xy extent, and their z offsets
+template <int TX>
+struct DmaLane {
+  const uint16_t* nbase;  // this sample's first plane
+  long long planeElems;
+  int x0, y0;
+  int off[DmaTile<TX>::NR];
+  int hz[DmaTile<TX>::NR];
+  unsigned okxy;          // bit r: voxel r inside the halo and the xy extent
+};
+
+// DMA of plane z's halo into `dst` (both halves); returns the mask of rounds whose voxel lies outside the volume
+template <int TX>
+__device__ __forceinline__ unsigned conv_dma_issue(const ConvArgs& a, const DmaLane<TX>& L, int z,
+                                                   uint16_t* __restrict__ dst) {
+  using T = DmaTile<TX>;
+  const int wave = threadIdx.x >> 6;
+  const long long zoff = (long long)(z - 1) * L.planeElems;  // the lane offsets count from plane z - 1
+  unsigned bad = 0;
+#pragma unroll
+  for (int r = 0; r < T::NR; ++r) {
+    const bool ok = ((L.okxy >> r) & 1u) && (unsigned)(z + L.hz[r] - 1) < (unsigned)a.D;
+    // a voxel outside the volume loads the sample's first voxel (a valid address) and is zeroed afterwards
+    const uint16_t* p = L.nbase + (ok ? zoff + L.off[r] : 0ll);
+    bad |= ok ? 0u : 1u << r;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      __builtin_amdgcn_global_load_lds((const void*)(p + h * 8),
+                                       (__attribute__((address_space(3))) void*)(dst + (h * T::HVP + wave * 64 +
+                                                                                       256 * r) * 8),
+                                       16, 0, 0);
+  }
+  return bad;
+}
+
+// after the lane's DMA has landed: zeros into the slots of its voxels outside the volume
+template <int TX>
+__device__ __forceinline__ void conv_dma_fix(unsigned bad, uint16_t* __restrict__ dst) {
+  using T = DmaTile<TX>;
+  if (!bad) return;
+#pragma unroll
+  for (int r = 0; r < T::NR; ++r) {
+    if (!((bad >> r) & 1u)) continue;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      *reinterpret_cast<uint4*>(dst + (h * T::HVP + (int)threadIdx.x + 256 * r) * 8) = make_uint4(0u, 0u, 0u, 0u);
+  }
+}
+
+// One plane's MFMAs: the next plane's DMA into `nxt` (when `more`), then the products of plane z from `cur` with
+// the weights `w` into acc.  The LDS buffers are distinct restrict parameters, so that the compiler can tell the
+// DMA's writes from this plane's reads and does not wait for the DMA before them.  The loop is unrolled and every
+// fragment address is one of four per-lane bases plus a constant (the A bases differ by the odd tap's offset
+// kind, the B base by the lane's tap parity), so the loop issues no address arithmetic.  Returns the mask of the
+// next plane's voxels outside the volume.
+template <int TX>
+__device__ __forceinline__ unsigned conv_dma_mfma(const ConvArgs& a, const DmaLane<TX>& L, int z, bool more,
+                                                  f32x4* acc, const uint16_t* __restrict__ cur,
+                                                  uint16_t* __restrict__ nxt, const uint16_t* __restrict__ w) {
+  using T = DmaTile<TX>;
+  static_assert(TX == 16 && T::pairs_ok(), "one 16-voxel segment per tile row, four odd-tap offsets");
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = lane & 15, g = lane >> 4;
+  const int hi = g >> 1;       // tap of the pair carried by this lane
+  const int hsel = g & 1;      // the lane's channel half
+  const unsigned bad = more ? conv_dma_issue<TX>(a, L, z + 1, nxt) : 0u;
+  // A: segment m of this wave at halo row 4 wave + m, voxel i; B: output channel i of the lane's tap
+  const int lb = hsel * T::HVP + 4 * wave * T::HX + i;
+  const uint16_t* ab[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) ab[k] = cur + (lb + hi * T::DOFF[k]) * 8;
+  const uint16_t* wb = w + (hsel * 28 * T::WC + hi * T::WC + i) * 8;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int p = 0; p < 14; ++p) {
+    const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(wb + 2 * p * T::WC * 8);
+    const uint16_t* a0 = ab[T::dkind(p)] + T::shift(2 * p) * 8;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const bf16x8 afr = *reinterpret_cast<const bf16x8*>(a0 + m * T::HX * 8);
+      acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr, acc[m], 0, 0, 0);
+    }
+  }
+  return bad;
+}
+
+// bias (read before any DMA: a load issued after one would wait for it), ReLU and bf16 stores of plane z: voxel
+// 4g + r of the segment, channel i
+template <int TX>
+__device__ __forceinline__ void conv_dma_store(const ConvArgs& a, const DmaLane<TX>& L, int n, int z,
+                                               const f32x4* acc, float bias) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = lane & 15, g = lane >> 4;
+  uint16_t* outPlane = a.out + (((size_t)n * a.D + z) * a.H) * a.W * a.out_cs + a.out_co + i;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int y = L.y0 + 4 * wave + m;
+    const int xs = L.x0 + 4 * g;
+    if (y < a.H) {
+      uint16_t* orow = outPlane + ((size_t)y * a.W + xs) * a.out_cs;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (xs + r >= a.W) continue;
+        float v = acc[m][r] + bias;
+        if (a.relu) v = fmaxf(v, 0.f);
+        orow[r * a.out_cs] = f2bf_rne(v);
+      }
+    }
+  }
+}
+
+template <int TX>
+__global__ __launch_bounds__(256) void k_conv_tile_dma(ConvArgs a, int zp) {
+  using T = DmaTile<TX>;
+  __shared__ __attribute__((aligned(16))) uint16_t tileA[2 * T::HVP * 8];
+  __shared__ __attribute__((aligned(16))) uint16_t tileB[2 * T::HVP * 8];
+  __shared__ __attribute__((aligned(16))) uint16_t wl[28 * T::WC * 16];
+
+  const int i = threadIdx.x & 15;
+  // work-group -> (n, z group, y block, x block), XCD-remapped as in k_conv_tile
+  const int nbx = (a.W + TX - 1) / TX, nby = (a.H + T::TY - 1) / T::TY, nzg = (a.D + zp - 1) / zp;
+  const int nbt = (int)gridDim.x;
+  int b = nbt % 8 == 0 ? (int)(blockIdx.x % 8) * (nbt / 8) + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+  const int bx = b % nbx; b /= nbx;
+  const int by = b % nby; b /= nby;
+  const int zg = b % nzg, n = b / nzg;
+  const int zs = zg * zp, ze = min(a.D, zs + zp);
+
+  DmaLane<TX> L;
+  L.planeElems = (long long)a.H * a.W * a.in_cs;
+  L.nbase = a.in + (size_t)n * a.D * L.planeElems;
+  L.x0 = bx * TX;
+  L.y0 = by * T::TY;
+  L.okxy = 0;
+  {
+    // the lane's halo voxels v = threadIdx.x + 256 r, decomposed incrementally; offsets fit 32 bits (at most
+    // three planes, checked at launch)
+    constexpr int DX = 256 % T::HX, DY = (256 / T::HX) % T::HY, DZ = 256 / (T::HX * T::HY);
+    const int v0 = threadIdx.x;
+    int hx = v0 % T::HX, hy = (v0 / T::HX) % T::HY, hz = v0 / (T::HX * T::HY);
+#pragma unroll
+    for (int r = 0; r < T::NR; ++r) {
+      if (r) {
+        hx += DX;
+        const int cy = hx >= T::HX;
+        hx -= cy ? T::HX : 0;
+        hy += DY + cy;
+        const int cz = hy >= T::HY;
+        hy -= cz ? T::HY : 0;
+        hz += DZ + cz;
+      }
+      const int xx = L.x0 + hx - 1, yy = L.y0 + hy - 1;
+      const bool ok = v0 + 256 * r < T::HV && (unsigned)xx < (unsigned)a.W && (unsigned)yy < (unsigned)a.H;
+      L.okxy |= ok ? 1u << r : 0u;
+      L.off[r] = ok ? ((hz * a.H + yy) * a.W + xx) * a.in_cs : 0;
+      L.hz[r] = hz;
+    }
+  }
+
+  // weights [28 taps][16 cout][16 cin] as two halves, staged once through registers
+  {
+    constexpr int NW = (28 * T::WC * 2 + 255) / 256;
+    uint4 wbuf[NW];
+    const int half = threadIdx.x & 1;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      const int e = threadIdx.x + 256 * k;
+      const int c = (e >> 1) % T::WC, tap = (e >> 1) / T::WC;
+      const bool ok = tap < 27;
+      wbuf[k] = *reinterpret_cast<const uint4*>(a.w + ((ok ? tap : 0) * a.cout + c) * a.cin + half * 8);
+      if (!ok) wbuf[k] = make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      const int e = threadIdx.x + 256 * k;
+      if (e < 28 * T::WC * 2) *reinterpret_cast<uint4*>(wl + ((e & 1) * 28 * T::WC + (e >> 1)) * 8) = wbuf[k];
+    }
+  }
+  const float bias = a.bias ? a.bias[i] : 0.f;
+  unsigned bad = conv_dma_issue<TX>(a, L, zs, tileA);
+  __builtin_amdgcn_s_waitcnt(0);
+  conv_dma_fix<TX>(bad, tileA);
+  __syncthreads();
+  // Planes alternate between the buffers.  A step: the next plane's DMA, this plane's MFMAs, the wait for this
+  // lane's DMA (and for the stores of the plane before, issued a step ago), the zeros, a barrier (the next plane is
+  // in LDS for every wave, every wave is done with the buffer the step after refills), and only then this plane's
+  // stores, which drain during the next step.
+  f32x4 acc[4];
+  for (int z = zs; z < ze; ++z) {
+    const bool more = z + 1 < ze;
+    const bool odd = (z - zs) & 1;
+    uint16_t* cur = odd ? tileB : tileA;
+    uint16_t* nxt = odd ? tileA : tileB;
+    bad = conv_dma_mfma<TX>(a, L, z, more, acc, cur, nxt, wl);
+    if (more) {
+      __builtin_amdgcn_s_waitcnt(0);
+      conv_dma_fix<TX>(bad, nxt);
+      __syncthreads();
+    }
+    conv_dma_store<TX>(a, L, n, z, acc, bias);
+  }
+}
+
 template <int TX, int NG>
 void launch_tile(const ConvArgs& a, hipStream_t s) {
   constexpr int TY = 256 / TX;
@@ -219,6 +460,14 @@ void launch_tile(const ConvArgs& a, hipStream_t s) {
 
 template <int TX>
 void launch_tx(const ConvArgs& a, hipStream_t s) {
+  constexpr int TY = 256 / TX;
+  const long long tiles = (long long)((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * a.D * a.N;
+  if (a.cin == 16 && a.cout == 16 && a.up_c == 0 && tiles >= kConvZPTiles) {
+    const unsigned nb = (unsigned)((a.W + TX - 1) / TX) * (unsigned)((a.H + TY - 1) / TY) *
+                        (unsigned)((a.D + kConvZPDma - 1) / kConvZPDma) * a.N;
+    hipLaunchKernelGGL((k_conv_tile_dma<TX>), dim3(nb), dim3(256), 0, s, a, kConvZPDma);
+    return;
+  }
   if ((a.cout / 16) % 2 == 0)
     launch_tile<TX, 2>(a, s);
   else
