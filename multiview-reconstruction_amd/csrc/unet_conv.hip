@@ -41,6 +41,34 @@ __device__ __forceinline__ uint16_t f2bf_rne(float f) {
 constexpr int kConvZP = 4;
 constexpr long long kConvZPTiles = 16384;
 
+// Tile geometry of the 3^3 convolutions (16 x 16 output voxels per work-group at TX = 16) and the halo offsets of
+// the taps; HVP pads each channel half of the halo to whole 256-voxel DMA rounds.
+template <int TX>
+struct ConvTile {
+  static constexpr int TY = 256 / TX;
+  static constexpr int HX = TX + 2, HY = TY + 2;
+  static constexpr int HV = 3 * HY * HX;          // halo voxels
+  static constexpr int NR = (HV + 255) / 256;     // DMA rounds per half
+  static constexpr int HVP = NR * 256;            // halo slots per half
+  static constexpr int WC = 16;
+  // halo offset of tap t (t = 27: the zero-weight tap, whose A reads the centre tap 13)
+  __host__ __device__ static constexpr int shift(int t) {
+    const int tv = t < 27 ? t : 13;
+    return (tv / 9 * HY + tv / 3 % 3) * HX + tv % 3;
+  }
+  // the odd tap of pair p lies one of four offsets beyond the even one
+  static constexpr int DOFF[4] = {1, HX - 2, HX * HY - 2 * HX - 2, shift(27) - shift(26)};
+  __host__ __device__ static constexpr int dkind(int p) {
+    const int d = shift(2 * p + 1) - shift(2 * p);
+    return d == DOFF[0] ? 0 : d == DOFF[1] ? 1 : d == DOFF[2] ? 2 : d == DOFF[3] ? 3 : -1;
+  }
+  __host__ __device__ static constexpr bool pairs_ok() {
+    for (int p = 0; p < 14; ++p)
+      if (dkind(p) < 0) return false;
+    return true;
+  }
+};
+
 template <int TX, int NG>
 __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a, int zp) {
   constexpr int TY = 256 / TX;          // tile rows
@@ -73,14 +101,6 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a, int zp) {
   for (int m = 0; m < 4; ++m)
 #pragma unroll
     for (int t = 0; t < NG; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-  // this wave's M-tiles: segments 4 * wave .. 4 * wave + 3 of the tile (row-major)
-  int segBase[4];
-#pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const int s = 4 * wave + m;
-    segBase[m] = (s / SEGX) * HX + (s % SEGX) * 16 + i;  // halo row offset (before the tap shift)
-  }
 
   // Staging of one (plane, 16-channel chunk) item: every thread issues all of its 16-byte loads (branch-free,
   // invalid lanes read a valid dummy address) before the first LDS write, so the loads are in flight together.
@@ -158,22 +178,29 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a, int zp) {
     issue(z, cb, wts);
     park(wts);
     __syncthreads();
-    // full unroll folds the tap arithmetic into constants; single-group layers keep registers for occupancy
-#pragma unroll(NG == 1 ? 2 : 14)
-    for (int p = 0; p < 14; ++p) {
-      const int tap = 2 * p + hi;                 // 27 -> zero weights; its A reads the centre tap
-      const int tv = tap < 27 ? tap : 13;
-      const int dz = tv / 9, dy = (tv / 3) % 3, dx = tv % 3;
-      const int shift = (dz * HY + dy) * HX + dx;
-      bf16x8 bfr[NG];
+    // Every fragment address is one of four per-lane bases plus a constant: the odd tap of a pair lies one of four
+    // offsets beyond the even one, and the B fragment's tap parity is the lane's (no tap arithmetic in the loop).
+    {
+      using G = ConvTile<TX>;
+      static_assert(TX == 16 && G::pairs_ok(), "one 16-voxel segment per tile row, four odd-tap offsets");
+      const int lb = hsel * HV + 4 * wave * HX + i;
+      const uint16_t* ab[4];
 #pragma unroll
-      for (int t = 0; t < NG; ++t)
-        bfr[t] = *reinterpret_cast<const bf16x8*>(wl + (hsel * 28 * WC + tap * WC + t * 16 + i) * 8);
+      for (int k = 0; k < 4; ++k) ab[k] = tile + (lb + hi * G::DOFF[k]) * 8;
+      const uint16_t* wb = wl + (hsel * 28 * WC + hi * WC + i) * 8;
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const bf16x8 afr = *reinterpret_cast<const bf16x8*>(tile + (hsel * HV + segBase[m] + shift) * 8);
+      for (int p = 0; p < 14; ++p) {
+        bf16x8 bfr[NG];
 #pragma unroll
-        for (int t = 0; t < NG; ++t) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[t], acc[m][t], 0, 0, 0);
+        for (int t = 0; t < NG; ++t) bfr[t] = *reinterpret_cast<const bf16x8*>(wb + (2 * p * WC + t * 16) * 8);
+        const uint16_t* a0 = ab[G::dkind(p)] + G::shift(2 * p) * 8;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const bf16x8 afr = *reinterpret_cast<const bf16x8*>(a0 + m * HX * 8);
+#pragma unroll
+          for (int t = 0; t < NG; ++t)
+            acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[t], acc[m][t], 0, 0, 0);
+        }
       }
     }
     if (cb != nchunk - 1) continue;
@@ -217,32 +244,6 @@ __global__ __launch_bounds__(256) void k_conv_tile(ConvArgs a, int zp) {
 // k_conv_tile<TX, 1>: the results are bitwise equal.
 constexpr int kConvZPDma = 8;
 
-template <int TX>
-struct DmaTile {
-  static constexpr int TY = 256 / TX;
-  static constexpr int HX = TX + 2, HY = TY + 2;
-  static constexpr int HV = 3 * HY * HX;          // halo voxels
-  static constexpr int NR = (HV + 255) / 256;     // DMA rounds per half
-  static constexpr int HVP = NR * 256;            // halo slots per half
-  static constexpr int WC = 16;
-  // halo offset of tap t (t = 27: the zero-weight tap, whose A reads the centre tap 13)
-  __host__ __device__ static constexpr int shift(int t) {
-    const int tv = t < 27 ? t : 13;
-    return (tv / 9 * HY + tv / 3 % 3) * HX + tv % 3;
-  }
-  // the odd tap of pair p lies one of four offsets beyond the even one
-  static constexpr int DOFF[4] = {1, HX - 2, HX * HY - 2 * HX - 2, shift(27) - shift(26)};
-  __host__ __device__ static constexpr int dkind(int p) {
-    const int d = shift(2 * p + 1) - shift(2 * p);
-    return d == DOFF[0] ? 0 : d == DOFF[1] ? 1 : d == DOFF[2] ? 2 : d == DOFF[3] ? 3 : -1;
-  }
-  __host__ __device__ static constexpr bool pairs_ok() {
-    for (int p = 0; p < 14; ++p)
-      if (dkind(p) < 0) return false;
-    return true;
-  }
-};
-
 // per-lane state of the DMA staging: the lane's halo voxels v0 + 256 r (r < NR), their element offsets from the
 // plane before the output plane (the same for every plane), which are inside the xy extent, and their z offsets
 template <int TX>
@@ -250,8 +251,8 @@ struct DmaLane {
   const uint16_t* nbase;  // this sample's first plane
   long long planeElems;
   int x0, y0;
-  int off[DmaTile<TX>::NR];
-  int hz[DmaTile<TX>::NR];
+  int off[ConvTile<TX>::NR];
+  int hz[ConvTile<TX>::NR];
   unsigned okxy;          // bit r: voxel r inside the halo and the xy extent
 };
 
@@ -259,7 +260,7 @@ struct DmaLane {
 template <int TX>
 __device__ __forceinline__ unsigned conv_dma_issue(const ConvArgs& a, const DmaLane<TX>& L, int z,
                                                    uint16_t* __restrict__ dst) {
-  using T = DmaTile<TX>;
+  using T = ConvTile<TX>;
   const int wave = threadIdx.x >> 6;
   const long long zoff = (long long)(z - 1) * L.planeElems;  // the lane offsets count from plane z - 1
   unsigned bad = 0;
@@ -282,7 +283,7 @@ __device__ __forceinline__ unsigned conv_dma_issue(const ConvArgs& a, const DmaL
 // after the lane's DMA has landed: zeros into the slots of its voxels outside the volume
 template <int TX>
 __device__ __forceinline__ void conv_dma_fix(unsigned bad, uint16_t* __restrict__ dst) {
-  using T = DmaTile<TX>;
+  using T = ConvTile<TX>;
   if (!bad) return;
 #pragma unroll
   for (int r = 0; r < T::NR; ++r) {
@@ -303,7 +304,7 @@ template <int TX>
 __device__ __forceinline__ unsigned conv_dma_mfma(const ConvArgs& a, const DmaLane<TX>& L, int z, bool more,
                                                   f32x4* acc, const uint16_t* __restrict__ cur,
                                                   uint16_t* __restrict__ nxt, const uint16_t* __restrict__ w) {
-  using T = DmaTile<TX>;
+  using T = ConvTile<TX>;
   static_assert(TX == 16 && T::pairs_ok(), "one 16-voxel segment per tile row, four odd-tap offsets");
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = lane & 15, g = lane >> 4;
@@ -358,7 +359,7 @@ __device__ __forceinline__ void conv_dma_store(const ConvArgs& a, const DmaLane<
 
 template <int TX>
 __global__ __launch_bounds__(256) void k_conv_tile_dma(ConvArgs a, int zp) {
-  using T = DmaTile<TX>;
+  using T = ConvTile<TX>;
   __shared__ __attribute__((aligned(16))) uint16_t tileA[2 * T::HVP * 8];
   __shared__ __attribute__((aligned(16))) uint16_t tileB[2 * T::HVP * 8];
   __shared__ __attribute__((aligned(16))) uint16_t wl[28 * T::WC * 16];
