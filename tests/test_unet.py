@@ -304,12 +304,14 @@ def test_receptive_field_within_inference_halo():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape,slab", [((150, 20, 28), 8), ((150, 20, 28), 36), ((512, 128, 64), 36)])
+@pytest.mark.parametrize("shape,slab", [((150, 20, 28), 8), ((150, 20, 28), 36), ((512, 128, 64), 36),
+                                        ((516, 132, 68), 36)])
 def test_unet_slab_inference_equals_single_window(shape, slab):
     # z slabs with their halo reproduce the single-window estimate exactly, also for a depth that is not a
     # multiple of 4 (the last slab carries the padded planes).  At 512 x 128 x 64 the single window's
-    # full-resolution convolutions have 16384 plane tiles of 16 x 16 and run 4 planes per work-group, the
-    # slabs' one
+    # full-resolution convolutions have 16384 plane tiles of 16 x 16: the 16 -> 16 layers run on the LDS-DMA
+    # kernel (8 planes per work-group), the others 4 planes per work-group, the slabs' one plane per work-group
+    # on the register-staged kernel.  516 x 132 x 68 adds partial tiles in x and y and a last z group of 4 planes
     from mvr import hip
     V = 3
     rng = np.random.default_rng(slab + shape[0])
