@@ -585,32 +585,73 @@ __global__ __launch_bounds__(XCfg<M>::NT) void k_x2_update(Dom d, UpdateParams u
     const float* fb = reinterpret_cast<const float*>(buf);
     const unsigned total = (unsigned)(C::RW * DX);
     constexpr int U = 6;
-    for (unsigned base = threadIdx.x; base < total; base += U * C::NT) {
-      float last[U], wt[U];
-      size_t gi[U];
-      unsigned rr[U];
-      bool ok[U];
+    if (!(DX & 1) && !(u.N[0] & 1) && !(o[0] & 1)) {
+      // even rows in an even-width volume at an even x origin: every pair of positions (2k, 2k + 1) lies in one
+      // row and is 8-byte aligned in psi and the weights, so a thread takes U2 such pairs per round (one row
+      // lookup and one address per pair, 8-byte loads and stores)
+      constexpr int U2 = 4;
+      const unsigned total2 = total / 2;
+      for (unsigned base = threadIdx.x; base < total2; base += U2 * C::NT) {
+        float2 last[U2], wt[U2];
+        size_t gi[U2];
+        unsigned rr[U2];
+        bool ok[U2];
 #pragma unroll
-      for (int j = 0; j < U; ++j) {
-        const unsigned idx = base + j * C::NT;
-        rr[j] = min(divD0.div(idx), (unsigned)C::RW - 1);
-        const int lx = (int)(idx - rr[j] * (unsigned)DX);
-        const long long prr = psiRow[rr[j]];
-        ok[j] = idx < total && prr >= 0;
-        gi[j] = ok[j] ? (size_t)(prr + lx) : 0;
-        last[j] = u.psi[gi[j]];
-        wt[j] = u.w[gi[j]];
+        for (int j = 0; j < U2; ++j) {
+          const unsigned e = 2 * (base + j * C::NT);  // first position of the pair
+          rr[j] = min(divD0.div(e), (unsigned)C::RW - 1);
+          const int lx = (int)(e - rr[j] * (unsigned)DX);
+          const long long prr = psiRow[rr[j]];
+          ok[j] = e < total && prr >= 0;
+          gi[j] = ok[j] ? (size_t)(prr + lx) : 0;
+          last[j] = *reinterpret_cast<const float2*>(u.psi + gi[j]);
+          wt[j] = *reinterpret_cast<const float2*>(u.w + gi[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < U2; ++j) {
+          if (!ok[j]) continue;
+          const unsigned e = 2 * (base + j * C::NT);
+          const int lx = (int)(e - rr[j] * (unsigned)DX);
+          const float* f = fb + rr[j] * 2 * C::RS + d.m[0] + lx;
+          float2 next;
+          next.x = rl_next_value(last[j].x, f[0], wt[j].x, rc);
+          next.y = rl_next_value(last[j].y, f[1], wt[j].y, rc);
+          *reinterpret_cast<float2*>(u.psi + gi[j]) = next;
+          const float c0 = next.x - last[j].x, c1 = next.y - last[j].y;
+          sum += (double)c0;
+          mx = java_maxf(mx, c0);
+          sum += (double)c1;
+          mx = java_maxf(mx, c1);
+        }
       }
+    } else {
+      for (unsigned base = threadIdx.x; base < total; base += U * C::NT) {
+        float last[U], wt[U];
+        size_t gi[U];
+        unsigned rr[U];
+        bool ok[U];
 #pragma unroll
-      for (int j = 0; j < U; ++j) {
-        if (!ok[j]) continue;
-        const unsigned idx = base + j * C::NT;
-        const int lx = (int)(idx - rr[j] * (unsigned)DX);
-        const float next = rl_next_value(last[j], fb[rr[j] * 2 * C::RS + d.m[0] + lx], wt[j], rc);
-        u.psi[gi[j]] = next;
-        const float ch = next - last[j];
-        sum += (double)ch;
-        mx = java_maxf(mx, ch);
+        for (int j = 0; j < U; ++j) {
+          const unsigned idx = base + j * C::NT;
+          rr[j] = min(divD0.div(idx), (unsigned)C::RW - 1);
+          const int lx = (int)(idx - rr[j] * (unsigned)DX);
+          const long long prr = psiRow[rr[j]];
+          ok[j] = idx < total && prr >= 0;
+          gi[j] = ok[j] ? (size_t)(prr + lx) : 0;
+          last[j] = u.psi[gi[j]];
+          wt[j] = u.w[gi[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+          if (!ok[j]) continue;
+          const unsigned idx = base + j * C::NT;
+          const int lx = (int)(idx - rr[j] * (unsigned)DX);
+          const float next = rl_next_value(last[j], fb[rr[j] * 2 * C::RS + d.m[0] + lx], wt[j], rc);
+          u.psi[gi[j]] = next;
+          const float ch = next - last[j];
+          sum += (double)ch;
+          mx = java_maxf(mx, ch);
+        }
       }
     }
   }
