@@ -378,7 +378,45 @@ __global__ void k_zero_u16(uint16_t* __restrict__ p, size_t n) {
 __global__ __launch_bounds__(256) void k_wsum_partials(InputArgs a, size_t n, double2* __restrict__ partial) {
   __shared__ double red[256], redw[256];
   double s = 0.0, sw = 0.0;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // Four voxels per step with 16-byte loads when every volume is 16-byte aligned: all views' loads are issued
+  // before the sums (views beyond V read view 0 and are not summed), and each voxel's float sums run over the
+  // views in order, as in the scalar loop below that takes the rest.
+  bool vec = true;
+  for (int c = 0; c < a.V; ++c) vec = vec && !(((uintptr_t)a.img[c] | (uintptr_t)a.w[c]) & 15);
+  const size_t n4 = vec ? n / 4 : 0;
+  for (size_t i = t0; i < n4; i += stride) {
+    float4 im[kNetMaxViews], wv[kNetMaxViews];
+#pragma unroll
+    for (int c = 0; c < kNetMaxViews; ++c) {
+      const int cc = c < a.V ? c : 0;
+      im[c] = reinterpret_cast<const float4*>(a.img[cc])[i];
+      wv[c] = reinterpret_cast<const float4*>(a.w[cc])[i];
+    }
+    float4 t = make_float4(0.f, 0.f, 0.f, 0.f), tw = t;
+#pragma unroll
+    for (int c = 0; c < kNetMaxViews; ++c) {
+      if (c >= a.V) break;
+      t.x += im[c].x * wv[c].x;
+      t.y += im[c].y * wv[c].y;
+      t.z += im[c].z * wv[c].z;
+      t.w += im[c].w * wv[c].w;
+      tw.x += wv[c].x;
+      tw.y += wv[c].y;
+      tw.z += wv[c].z;
+      tw.w += wv[c].w;
+    }
+    s += (double)t.x;
+    s += (double)t.y;
+    s += (double)t.z;
+    s += (double)t.w;
+    sw += (double)tw.x;
+    sw += (double)tw.y;
+    sw += (double)tw.z;
+    sw += (double)tw.w;
+  }
+  for (size_t i = 4 * n4 + t0; i < n; i += stride) {
     float t = 0.f, tw = 0.f;
     for (int c = 0; c < a.V; ++c) {
       t += a.img[c][i] * a.w[c][i];
