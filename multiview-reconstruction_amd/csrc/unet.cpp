@@ -164,8 +164,8 @@ int UNet::ensure_acts(Acts& a, int N_, int D_, int H_, int W_, bool grads) {
   return a.alloc(N_, D_, H_, W_, grads);
 }
 
-void UNet::conv(int l, const uint16_t* in, int in_cs, int in_co, uint16_t* out, int out_cs, int out_co, int N, int D,
-                int H, int W, bool relu, const uint16_t* up, int up_cs, int up_c) {
+bool UNet::conv(int l, const uint16_t* in, int in_cs, int in_co, uint16_t* out, int out_cs, int out_co, int N, int D,
+                int H, int W, bool relu, const uint16_t* up, int up_cs, int up_c, uint16_t* pool, int pool_cs) {
   ConvArgs c{};
   c.up_in = up;
   c.up_cs = up_cs;
@@ -182,7 +182,12 @@ void UNet::conv(int l, const uint16_t* in, int in_cs, int in_co, uint16_t* out, 
   c.N = N; c.D = D; c.H = H; c.W = W;
   c.taps = 27;
   c.relu = relu ? 1 : 0;
+  c.pool_out = pool;
+  c.pool_cs = pool_cs;
+  const bool fused = pool && conv_fuses_pool(c);
+  if (!fused) c.pool_out = nullptr;
   if (!launch_conv_fwd(c, stream)) convFailed = true;
+  return fused;
 }
 
 // data gradient of layer l: din = conv(dout, flipped transposed weights), no bias
@@ -230,8 +235,9 @@ int UNet::forward(Acts& a, bool training, const float* psiScale, float* psiOut, 
   const int D1 = D0 / 2, H1 = H0 / 2, W1 = W0 / 2, D2 = D0 / 4, H2 = H0 / 4, W2 = W0 / 4;
   convFailed = false;
   conv(0, a.x0, 16, 0, a.a1, 16, 0, N, D0, H0, W0, true);
-  conv(1, a.a1, 16, 0, a.cat1, 48, 32, N, D0, H0, W0, true);                  // e1 -> cat1[32:48)
-  launch_pool_fwd(a.cat1, 48, 32, a.p1, 16, 16, N, D1, H1, W1, stream);
+  // e1 -> cat1[32:48), pooled into p1 by the convolution where it can
+  if (!conv(1, a.a1, 16, 0, a.cat1, 48, 32, N, D0, H0, W0, true, nullptr, 0, 0, a.p1, 16))
+    launch_pool_fwd(a.cat1, 48, 32, a.p1, 16, 16, N, D1, H1, W1, stream);
   conv(2, a.p1, 16, 0, a.a2, 32, 0, N, D1, H1, W1, true);
   conv(3, a.a2, 32, 0, a.cat2, 96, 64, N, D1, H1, W1, true);                 // e2 -> cat2[64:96)
   launch_pool_fwd(a.cat2, 96, 64, a.p2, 32, 32, N, D2, H2, W2, stream);

@@ -25,6 +25,10 @@ struct ConvArgs {
   // up_cs) at half the coordinates, channels [up_c, cin) from in (its channel 0 = input channel up_c)
   const uint16_t* up_in;
   int up_cs, up_c;
+  // optional fused 2x2x2 average pooling of the stored (bf16) output into pool_out [n][D/2][H/2][W/2][pool_cs],
+  // channels [pool_co, pool_co + cout): done where conv_fuses_pool(args) holds, else left to the caller
+  uint16_t* pool_out;
+  int pool_cs, pool_co;
 };
 
 struct WgradArgs {
@@ -86,6 +90,7 @@ struct AdamArgs {
 bool launch_conv_fwd(const ConvArgs& a, hipStream_t s);  // false: shape not supported, nothing launched
 // LDS-tiled 3^3 convolution (unet_conv.hip); false when the shape is not one it covers
 bool launch_conv_tiled(const ConvArgs& a, hipStream_t s);
+bool conv_fuses_pool(const ConvArgs& a);  // the launch also writes the pooled output (pool_out set, DMA kernel)
 void launch_conv_wgrad(const WgradArgs& a, int nparts, hipStream_t s);
 void launch_sum_partials(const float* partial, int nparts, size_t count, float* grad, hipStream_t s);
 void launch_bias_grad(const uint16_t* dout, int cs, int co0, int cout, size_t nvox, float* partial, int nparts,

@@ -333,13 +333,20 @@ __device__ __forceinline__ unsigned conv_dma_mfma(const ConvArgs& a, const DmaLa
 }
 
 // bias (read before any DMA: a load issued after one would wait for it), ReLU and bf16 stores of plane z: voxel
-// 4g + r of the segment, channel i
+// 4g + r of the segment, channel i.  With a pooled output, pool[ky][kx] (the lane's 2 x 2 output voxels of the
+// plane pair) sums the stored values in the order of k_pool_fwd: z, then y, then x (even plane first: rows
+// m = 2 ky, 2 ky + 1, voxels r = 2 kx, 2 kx + 1), so the fused pooling rounds exactly as the pooling kernel.
 template <int TX>
 __device__ __forceinline__ void conv_dma_store(const ConvArgs& a, const DmaLane<TX>& L, int n, int z,
-                                               const f32x4* acc, float bias) {
+                                               const f32x4* acc, float bias, float (*pool)[2]) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = lane & 15, g = lane >> 4;
   uint16_t* outPlane = a.out + (((size_t)n * a.D + z) * a.H) * a.W * a.out_cs + a.out_co + i;
+  const bool pooling = a.pool_out != nullptr;
+  if (pooling && !(z & 1)) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pool[k >> 1][k & 1] = 0.f;
+  }
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     const int y = L.y0 + 4 * wave + m;
@@ -351,7 +358,23 @@ __device__ __forceinline__ void conv_dma_store(const ConvArgs& a, const DmaLane<
         if (xs + r >= a.W) continue;
         float v = acc[m][r] + bias;
         if (a.relu) v = fmaxf(v, 0.f);
-        orow[r * a.out_cs] = f2bf_rne(v);
+        const uint16_t b = f2bf_rne(v);
+        orow[r * a.out_cs] = b;
+        if (pooling) pool[m >> 1][r >> 1] += __uint_as_float((uint32_t)b << 16);
+      }
+    }
+  }
+  if (pooling && (z & 1)) {
+    // the plane pair is complete: output voxel (z / 2, y / 2, x / 2) of the pooled tensor (H, W even)
+    const int H2 = a.H / 2, W2 = a.W / 2;
+    uint16_t* pp = a.pool_out + (((size_t)n * (a.D / 2) + z / 2) * H2) * W2 * a.pool_cs + a.pool_co + i;
+#pragma unroll
+    for (int ky = 0; ky < 2; ++ky) {
+      const int y2 = (L.y0 + 4 * wave) / 2 + ky;
+#pragma unroll
+      for (int kx = 0; kx < 2; ++kx) {
+        const int x2 = (L.x0 + 4 * g) / 2 + kx;
+        if (y2 < H2 && x2 < W2) pp[((size_t)y2 * W2 + x2) * a.pool_cs] = f2bf_rne(pool[ky][kx] * 0.125f);
       }
     }
   }
@@ -434,6 +457,7 @@ __global__ __launch_bounds__(256) void k_conv_tile_dma(ConvArgs a, int zp) {
   // in LDS for every wave, every wave is done with the buffer the step after refills), and only then this plane's
   // stores, which drain during the next step.
   f32x4 acc[4];
+  float pool[2][2];
   for (int z = zs; z < ze; ++z) {
     const bool more = z + 1 < ze;
     const bool odd = (z - zs) & 1;
@@ -445,7 +469,7 @@ __global__ __launch_bounds__(256) void k_conv_tile_dma(ConvArgs a, int zp) {
       conv_dma_fix<TX>(bad, nxt);
       __syncthreads();
     }
-    conv_dma_store<TX>(a, L, n, z, acc, bias);
+    conv_dma_store<TX>(a, L, n, z, acc, bias, pool);
   }
 }
 
@@ -459,11 +483,21 @@ void launch_tile(const ConvArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_conv_tile<TX, NG>), dim3(nb, a.cout / (16 * NG)), dim3(256), 0, s, a, zp);
 }
 
+// the 16 -> 16 single-chunk layers of large launches take the DMA kernel (a pooled output needs even dimensions
+// and the work-groups' even-aligned plane pairs, which kConvZPDma gives)
+template <int TX>
+bool takes_dma(const ConvArgs& a) {
+  constexpr int TY = 256 / TX;
+  const long long tiles = (long long)((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * a.D * a.N;
+  return a.cin == 16 && a.cout == 16 && a.up_c == 0 && tiles >= kConvZPTiles &&
+         (!a.pool_out || !((a.D | a.H | a.W) & 1));
+}
+static_assert(kConvZPDma % 2 == 0, "plane pairs of the fused pooling stay in one work-group");
+
 template <int TX>
 void launch_tx(const ConvArgs& a, hipStream_t s) {
   constexpr int TY = 256 / TX;
-  const long long tiles = (long long)((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * a.D * a.N;
-  if (a.cin == 16 && a.cout == 16 && a.up_c == 0 && tiles >= kConvZPTiles) {
+  if (takes_dma<TX>(a)) {
     const unsigned nb = (unsigned)((a.W + TX - 1) / TX) * (unsigned)((a.H + TY - 1) / TY) *
                         (unsigned)((a.D + kConvZPDma - 1) / kConvZPDma) * a.N;
     hipLaunchKernelGGL((k_conv_tile_dma<TX>), dim3(nb), dim3(256), 0, s, a, kConvZPDma);
@@ -476,6 +510,11 @@ void launch_tx(const ConvArgs& a, hipStream_t s) {
 }
 
 }  // namespace
+
+bool conv_fuses_pool(const ConvArgs& a) {
+  return a.pool_out && a.taps == 27 && a.in_cs % 16 == 0 && takes_dma<16>(a) &&
+         3ll * a.H * a.W * a.in_cs < (1ll << 31);
+}
 
 // 3^3 convolutions with cin and cout multiples of 16
 bool launch_conv_tiled(const ConvArgs& a, hipStream_t s) {

@@ -80,8 +80,11 @@ struct UNet {
            const int o[3], float invScale);
   // up != nullptr: input channels [0, up_c) are nearest x2 upsampled from up (channel stride up_cs), the
   // rest come from in + in_co
-  void conv(int l, const uint16_t* in, int in_cs, int in_co, uint16_t* out, int out_cs, int out_co, int N, int D,
-            int H, int W, bool relu, const uint16_t* up = nullptr, int up_cs = 0, int up_c = 0);
+  // pool != nullptr: the 2x2x2 average pooling of the output into pool (channel stride pool_cs) where the
+  // convolution fuses it; returns whether it did (else the caller pools)
+  bool conv(int l, const uint16_t* in, int in_cs, int in_co, uint16_t* out, int out_cs, int out_co, int N, int D,
+            int H, int W, bool relu, const uint16_t* up = nullptr, int up_cs = 0, int up_c = 0,
+            uint16_t* pool = nullptr, int pool_cs = 0);
   void dgrad(int l, const uint16_t* dout, int dout_cs, int dout_co, uint16_t* din, int din_cs, int din_co, int N,
              int D, int H, int W);
   int wgrad(int l, const uint16_t* in, int in_cs, const uint16_t* dout, int dout_cs, int dout_co, int N, int D, int H,
